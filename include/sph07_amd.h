@@ -1,0 +1,155 @@
+/*
+ * sph07_amd.h -- C ABI of the MI355X-native per-timestep particle loop of the
+ * sph07 SPH solver.
+ *
+ * Every entry point replaces one member function of the reference's
+ * MSolution / MSimulationInit surface (MSolution.h:25-65,
+ * MSimulationInit.h) and keeps its meaning: the same phase order, the same
+ * error convention (0 = OK, 1 = ERROR as in global.h:10-11, plus the specific
+ * codes below), and state that is mutated in place -- here device-resident
+ * per context instead of the MSimulationData singleton.
+ *
+ * Plain C types only: no HIP or torch types cross this boundary.  One context
+ * is one scene bound to one HIP stream; calls on a context must be serialised
+ * by the caller; several contexts may live side by side.
+ */
+#ifndef SPH07_AMD_H_
+#define SPH07_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (0/1 as the reference; the rest are specific) ---- */
+#define SPH_OK              0
+#define SPH_ERROR           1   /* generic, message in sph_last_error */
+#define SPH_ERR_NBR_OVERFLOW 2  /* "Neighbour list exceeded maximum." MSolution.cpp:201-206 */
+#define SPH_ERR_OUT_OF_GRID 3   /* particle outside the grid (unchecked UB in the reference) */
+#define SPH_ERR_UNSUPPORTED 4   /* deck option outside the hot path */
+#define SPH_ERR_HIP         5   /* HIP runtime failure */
+
+/* One material (material cards, MSimulationData.cpp:454-536, after
+   MMaterial::CheckDefaults MMaterial.cpp:71-81). */
+typedef struct sph_material_desc {
+    int32_t model;            /* 9 = fluid */
+    int32_t eos;              /* 28 Tait (Monaghan 1994) or 29 linear (Morris 1997) */
+    int32_t visc_type;        /* 1 Morris viscosity; 2 adds Monaghan alpha (extension) */
+    int32_t pad_;
+    double  rho;              /* rho0 */
+    double  av_l;             /* kinematic viscosity nu */
+    double  av_q;
+    double  surface_tension;  /* sigma (CSF) */
+    double  mass;             /* total material mass (massopt 0) */
+    double  h;                /* smoothing length */
+    double  eosinput[4];
+    double  strinput0;        /* pressure cut-off applied at initialisation */
+    double  av_alpha;         /* Monaghan alpha (visc_type 2) */
+} sph_material_desc;
+
+/* One scene: the content of a .mcm deck (MSimulationData::GetInput,
+   MSimulationData.cpp:186-219).  Arrays are 0-based, particle-major. */
+typedef struct sph_scene_desc {
+    int32_t ndim, disctype, tcrit_opt, maxnbr;          /* cards 1, 5, 6 */
+    int32_t surface_tension, rigid_body, boundary, baseaccel;
+    int32_t massopt, init_h_opt, init_rhoe, nummat;     /* card 4 */
+    int32_t np, rb_id, veloc_opt, h_opt;
+    int32_t boundary_code[2][3];                        /* boundary card: 0 free, 2 periodic */
+    double  endtime, tssf, itss;                        /* card 2 */
+    double  boundary_x[2][3];
+    double  base_a[3];                                  /* base acceleration card */
+    double  rb_total_mass, rb_inertia;                  /* rigid body card */
+    const sph_material_desc *mat;  /* nummat entries; material n is mat[n-1] */
+    const double  *x;              /* np*3 node coordinates */
+    const double  *v;              /* np*3 initial velocities, or NULL */
+    const int32_t *mat_id;         /* np, 1-based material number */
+    const int32_t *color;          /* np, CSF colour (NULL: all 1) */
+    const int32_t *dispbc;         /* np: 0 free, 1-7 as ConstrainQuantity, 8 moving wall */
+    const double  *node_mass;      /* np, massopt 1 */
+    const double  *node_h;         /* reserved (init_h_opt must be 0) */
+    const double  *node_rho;       /* np, init_rhoe 1 */
+} sph_scene_desc;
+
+/* Scalar state (MGlobalVars.h:18-31, MRigidBody.h:18-30). */
+typedef struct sph_globals {
+    double  dt, dtold, ptime, critts;
+    double  coord_minmax[2][3];
+    double  hmax;
+    double  rb_R[3], rb_V[3], rb_OMEGA[3], rb_A[3], rb_ALPHA[3];
+    int32_t timestep, ngp, done, error;
+    int32_t np, grid_lim[3];
+} sph_globals;
+
+/* Arithmetic arm of the pair sweeps. */
+#define SPH_ARM_FP64        0   /* reference arithmetic, reference summation order */
+
+/* Fields for sph_download (np entries; vector fields np*3). */
+enum {
+    SPH_F_X = 0, SPH_F_V, SPH_F_A, SPH_F_RHO, SPH_F_P, SPH_F_C, SPH_F_MASS, SPH_F_H,
+    SPH_F_TRD, SPH_F_MINDIST, SPH_F_NORMAL, SPH_F_NNORMAL, SPH_F_N, SPH_F_KAPPA,
+    SPH_F_CRITTS, SPH_F_RHOOLD, SPH_F_ACTIVE, SPH_F_VABS
+};
+
+/* --- lifetime --- */
+
+/* MSimulationInit::Initialize (MSimulationInit.cpp:31-99): uploads the deck,
+   computes mass, rho/p/c, h, the first neighbour pass, the critical step and
+   the first dt on the device.  *out is NULL on error. */
+int  sph_create(const sph_scene_desc *desc, int arm, void **out, char *err, int errlen);
+/* MSolution::DestroyInstance (MSolution.h:18) */
+void sph_destroy(void *ctx);
+/* last error message of a context (PrintScreenLog text, MSolution.cpp:217-221) */
+const char *sph_last_error(void *ctx);
+int  sph_desc_size(void);
+int  sph_material_desc_size(void);
+
+/* --- MSolution phase surface (MSolution.h:25-65) --- */
+
+/* MSolution::Neighbours -> SetupLinkedList (MSolution.cpp:96-148):
+   grid (MNeighbourVars::InitGrid, MNeighbourVars.cpp:18-66) and binning. */
+int  sph_neighbours(void *ctx);
+/* MSolution::GhostSetup (MSolution.cpp:811-918): periodic images. */
+int  sph_ghost_setup(void *ctx);
+/* MSolution::LinkedListNeighbours (MSolution.cpp:150-215). */
+int  sph_linked_list_neighbours(void *ctx);
+/* MSolution::Strain + ClassicStrain (MSolution.cpp:327-413). */
+int  sph_strain(void *ctx);
+/* MSolution::CalculateSurfaceNormals (MSolution.cpp:1429-1493). */
+int  sph_calculate_surface_normals(void *ctx);
+/* MSolution::Constitutive + RhoUpdate (MSolution.cpp:436-477). */
+int  sph_constitutive(void *ctx);
+/* MSolution::CalcCriticalTimestep (MSolution.cpp:34-94); critts may be NULL. */
+int  sph_calc_critical_timestep(void *ctx, double *critts);
+/* The dt rule inside MSolution::Solution (MSolution.cpp:275-277). */
+int  sph_time_advance(void *ctx);
+/* MSolution::Momentum (MSolution.cpp:479-504) incl. RigidBodyAcceleration. */
+int  sph_momentum(void *ctx);
+/* MSolution::BoundCond (MSolution.cpp:700-726). */
+int  sph_boundcond(void *ctx);
+/* MSolution::UpdateVelocity (MSolution.cpp:602-698) incl. RigidBodyVelocity. */
+int  sph_update_velocity(void *ctx);
+/* MSolution::MoveParticles (MSolution.cpp:769-809). */
+int  sph_move_particles(void *ctx);
+/* timestep++ and the stop test of MSolution::Solution (MSolution.cpp:317-319). */
+int  sph_end_step(void *ctx);
+/* MSolution::Solution (MSolution.cpp:238-324) without output: up to
+   max_steps fused device-resident iterations; stops early when the stop test
+   fires.  *steps_done receives the iterations executed. */
+int  sph_solution(void *ctx, int max_steps, int *steps_done);
+
+/* --- state access --- */
+int  sph_get_globals(void *ctx, sph_globals *out);
+/* Copies one field, in the deck's particle order, to host memory. */
+int  sph_download(void *ctx, int field, double *host_out);
+/* Neighbour lists of the last LinkedListNeighbours, in summation order.
+   Canonical ids as the oracle: real j -> j; periodic image of j shifted by
+   (sx,sy,sz) -> -(1 + j*27 + (sx+1)*9 + (sy+1)*3 + (sz+1)). */
+int  sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr);
+/* Blocks until all work queued on the context's stream has finished. */
+int  sph_synchronize(void *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

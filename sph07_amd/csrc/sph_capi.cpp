@@ -1,0 +1,821 @@
+// sph_capi.cpp -- extern "C" boundary of the MI355X step loop and the host
+// side of MSimulationInit::Initialize (MSimulationInit.cpp:31-99).
+//
+// A context owns one scene's device state (structure of arrays in HBM), its
+// HIP stream and a pinned mirror of the device control block.  The phase
+// entry points synchronise and return the reference's status codes; the
+// fused sph_solution queues whole steps and synchronises once per chunk.
+#include "../../include/sph07_amd.h"
+#include "sph_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+using namespace sph;
+
+namespace {
+
+constexpr double kPI = 3.141592654;   // global.h:12
+
+struct Context {
+    hipStream_t st = nullptr;
+    Params P{};
+    Arrays A{};
+    Ctl *hctl = nullptr;          // pinned mirror of the control block
+    std::string err;
+    int N = 0, ndim = 0, maxnbr = 0, arm = 0;
+    std::vector<void *> part_allocs, cell_allocs;
+    long long *canon = nullptr;
+};
+
+#define HIPCHK(ctx, expr)                                                          \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            (ctx)->err = std::string("HIP error: ") + hipGetErrorString(e_) +      \
+                         " at " #expr;                                             \
+            return SPH_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+template <class T>
+int dalloc(Context *c, std::vector<void *> &list, T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(c, hipMalloc((void **)p, n * sizeof(T)));
+    list.push_back(*p);
+    return 0;
+}
+
+void free_list(std::vector<void *> &list)
+{
+    for (void *p : list) hipFree(p);
+    list.clear();
+}
+
+// per-particle arrays of capacity Ncap = N + gcap
+int alloc_particles(Context *c, int gcap)
+{
+    Params &P = c->P;
+    Arrays &A = c->A;
+    const size_t ncap = (size_t)c->N + gcap;
+    std::vector<void *> &L = c->part_allocs;
+    P.gcap = gcap;
+    P.Ncap = (int)ncap;
+    int rc = 0;
+    for (int d = 0; d < 3; d++) {
+        A.x[d] = A.xs[d] = A.v[d] = A.a[d] = A.nrm[d] = A.nn[d] = nullptr;
+        if (d >= c->ndim) continue;
+        rc |= dalloc(c, L, &A.x[d], ncap);
+        rc |= dalloc(c, L, &A.xs[d], ncap);
+        rc |= dalloc(c, L, &A.v[d], ncap);
+        rc |= dalloc(c, L, &A.a[d], c->N);
+        if (P.st) {
+            rc |= dalloc(c, L, &A.nrm[d], c->N);
+            rc |= dalloc(c, L, &A.nn[d], c->N);
+        }
+    }
+    rc |= dalloc(c, L, &A.rho, ncap);
+    rc |= dalloc(c, L, &A.rho_new, ncap);
+    rc |= dalloc(c, L, &A.p, ncap);
+    rc |= dalloc(c, L, &A.prho2, ncap);
+    rc |= dalloc(c, L, &A.vol, ncap);
+    rc |= dalloc(c, L, &A.m, ncap);
+    rc |= dalloc(c, L, &A.info, ncap);
+    rc |= dalloc(c, L, &A.trd, c->N);
+    rc |= dalloc(c, L, &A.mindist, c->N);
+    rc |= dalloc(c, L, &A.critts_p, c->N);
+    A.rhoold = nullptr;
+    if (P.st) {
+        rc |= dalloc(c, L, &A.Nf, c->N);
+        rc |= dalloc(c, L, &A.kappa, c->N);
+    } else {
+        A.Nf = A.kappa = nullptr;
+    }
+    rc |= dalloc(c, L, &A.nnbr, c->N);
+    rc |= dalloc(c, L, &A.nbr, (size_t)c->maxnbr * ncap);
+    rc |= dalloc(c, L, &A.cell_of, c->N);
+    rc |= dalloc(c, L, &A.slot, c->N);
+    rc |= dalloc(c, L, &A.tmp, c->N);
+    rc |= dalloc(c, L, &A.entries, c->N);
+    rc |= dalloc(c, L, &A.gsrc, gcap);
+    rc |= dalloc(c, L, &A.gshift, gcap);
+    return rc;
+}
+
+int alloc_cells(Context *c, int cap)
+{
+    Arrays &A = c->A;
+    std::vector<void *> &L = c->cell_allocs;
+    free_list(L);
+    c->P.ncell_cap = cap;
+    int rc = 0;
+    rc |= dalloc(c, L, &A.cell_count, (size_t)cap + 1);
+    rc |= dalloc(c, L, &A.cell_start, (size_t)cap + 1);
+    rc |= dalloc(c, L, &A.gcount, (size_t)cap);
+    rc |= dalloc(c, L, &A.gfirst, (size_t)cap);
+    rc |= dalloc(c, L, &A.slab_cnt, (size_t)cap);
+    rc |= dalloc(c, L, &A.slab_off, (size_t)cap);
+    rc |= dalloc(c, L, &A.scan_aux, (size_t)cap / 2048 + 4);
+    return rc;
+}
+
+// Grows the image capacity: per-particle arrays re-allocated, the real
+// particles' persistent fields copied over.
+int grow_particles(Context *c, int new_gcap)
+{
+    Arrays old = c->A;
+    std::vector<void *> old_list;
+    old_list.swap(c->part_allocs);
+    const int rigid_keep = 0;
+    (void)rigid_keep;
+    if (alloc_particles(c, new_gcap)) return SPH_ERR_HIP;
+    Arrays &A = c->A;
+    const size_t n = (size_t)c->N;
+    auto cp = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (!dst || !src) return 0;
+        HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
+        return 0;
+    };
+    int rc = 0;
+    for (int d = 0; d < c->ndim; d++) {
+        rc |= cp(A.x[d], old.x[d], n * 8);
+        rc |= cp(A.xs[d], old.xs[d], n * 8);
+        rc |= cp(A.v[d], old.v[d], n * 8);
+        rc |= cp(A.a[d], old.a[d], n * 8);
+        if (c->P.st) {
+            rc |= cp(A.nrm[d], old.nrm[d], n * 8);
+            rc |= cp(A.nn[d], old.nn[d], n * 8);
+        }
+    }
+    rc |= cp(A.rho, old.rho, n * 8);
+    rc |= cp(A.rho_new, old.rho_new, n * 8);
+    rc |= cp(A.p, old.p, n * 8);
+    rc |= cp(A.prho2, old.prho2, n * 8);
+    rc |= cp(A.vol, old.vol, n * 8);
+    rc |= cp(A.m, old.m, n * 8);
+    rc |= cp(A.info, old.info, n * 4);
+    rc |= cp(A.trd, old.trd, n * 8);
+    rc |= cp(A.mindist, old.mindist, n * 8);
+    rc |= cp(A.critts_p, old.critts_p, n * 8);
+    if (c->P.st) {
+        rc |= cp(A.Nf, old.Nf, n * 8);
+        rc |= cp(A.kappa, old.kappa, n * 8);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    free_list(old_list);
+    return rc;
+}
+
+int sync_ctl(Context *c)
+{
+    HIPCHK(c, hipMemcpyAsync(c->hctl, c->A.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int push_ctl(Context *c)
+{
+    HIPCHK(c, hipMemcpyAsync(c->A.ctl, c->hctl, sizeof(Ctl), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+const char *err_text(int code)
+{
+    switch (code) {
+    case kErrNbrOverflow: return "ERROR: Neighbour list exceeded maximum.";
+    case kErrOutOfGrid: return "ERROR: particle outside the neighbour grid.";
+    case 1: return "ERROR: a cell received periodic images from two passes (unsupported).";
+    default: return "ERROR: device error.";
+    }
+}
+
+// Checks the device error flag after a synchronisation.  Capacity errors are
+// repaired (capacity grown, flag cleared) and reported as 'retry' (-1).
+int check_device(Context *c)
+{
+    if (sync_ctl(c)) return SPH_ERR_HIP;
+    Ctl &h = *c->hctl;
+    if (h.err == 0) return 0;
+    if (h.err == kErrGridCap) {
+        long long want = (long long)h.ncell * 3 / 2 + 1024;
+        if (want > 0x7fffffffLL - 1) {
+            c->err = "ERROR: neighbour grid too large.";
+            return SPH_ERROR;
+        }
+        if (alloc_cells(c, (int)want)) return SPH_ERR_HIP;
+        h.err = 0;
+        if (push_ctl(c)) return SPH_ERR_HIP;
+        return -1;
+    }
+    if (h.err == kErrGhostCap) {
+        int want = std::max(2 * c->P.gcap, h.err_particle + 1024);
+        if (grow_particles(c, want)) return SPH_ERR_HIP;
+        h.err = 0;
+        if (push_ctl(c)) return SPH_ERR_HIP;
+        return -1;
+    }
+    c->err = err_text(h.err);
+    if (h.err_particle >= 0) c->err += " (particle " + std::to_string(h.err_particle + 1) + ")";
+    return h.err == kErrNbrOverflow ? SPH_ERR_NBR_OVERFLOW
+           : h.err == kErrOutOfGrid ? SPH_ERR_OUT_OF_GRID
+                                    : SPH_ERROR;
+}
+
+// Runs a phase sequence, repairing capacity errors by re-running it.
+template <class F>
+int run_phase(Context *c, F &&f)
+{
+    for (int attempt = 0; attempt < 8; attempt++) {
+        f();
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            c->err = std::string("HIP launch error: ") + hipGetErrorString(e);
+            return SPH_ERR_HIP;
+        }
+        int rc = check_device(c);
+        if (rc != -1) return rc;
+    }
+    c->err = "ERROR: capacity repair did not converge.";
+    return SPH_ERROR;
+}
+
+void fused_step(Context *c)
+{
+    const Params &P = c->P;
+    const Arrays &A = c->A;
+    launch_setup_linked_list(P, A, c->st);
+    if (P.periodic) launch_ghost_setup(P, A, c->st);
+    launch_neighbours(P, A, 1, c->st);
+    launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, c->st);
+    if (P.st) launch_normals(P, A, c->st);
+    launch_advance(P, A, c->st);
+    launch_momentum(P, A, 1, c->st);
+    launch_integrate(P, A, M_BC | M_VEL | M_MOVE | M_RHO, c->st);
+    launch_finish(P, A, 1, 1, c->st);
+}
+
+// ---- host part of MSimulationInit::Initialize -------------------------------
+int build_context(Context *c, const sph_scene_desc *d)
+{
+    Params &P = c->P;
+    if (d->ndim < 1 || d->ndim > 3) { c->err = "ndim must be 1, 2 or 3"; return SPH_ERROR; }
+    if (d->disctype != 2) { c->err = "only disctype 2 (Morris) is on the GPU path"; return SPH_ERR_UNSUPPORTED; }
+    if (d->veloc_opt == 1) { c->err = "XSPH (veloc_opt 1) is outside the GPU path"; return SPH_ERR_UNSUPPORTED; }
+    if (d->h_opt != 0) { c->err = "variable h (h_opt > 0) is outside the GPU path"; return SPH_ERR_UNSUPPORTED; }
+    if (d->init_h_opt != 0) { c->err = "init_h_opt 1 is not supported"; return SPH_ERR_UNSUPPORTED; }
+    if (d->nummat < 1 || d->nummat > kMaxMat) { c->err = "material count out of range"; return SPH_ERROR; }
+    if (d->np < 1) { c->err = "no particles"; return SPH_ERROR; }
+    if (d->tcrit_opt < 1 || d->tcrit_opt > 3) { c->err = "ERROR: Wrong critical time step option."; return SPH_ERROR; }
+    const int N = d->np, D = d->ndim;
+    c->N = N;
+    c->ndim = D;
+    c->maxnbr = d->maxnbr > 0 ? d->maxnbr : 40;
+    P.N = N;
+    P.ndim = D;
+    P.tcrit_opt = d->tcrit_opt;
+    P.maxnbr = c->maxnbr;
+    P.st = d->surface_tension != 0;
+    P.rigid = d->rigid_body != 0;
+    P.baseaccel = d->baseaccel != 0;
+    P.nmat = d->nummat;
+    P.rb_id = d->rb_id;
+    P.rb_M = d->rb_total_mass;
+    P.rb_I = d->rb_inertia;
+    P.tssf = d->tssf == 0.0 ? 0.8 : d->tssf;
+    P.endtime = d->endtime;
+    P.want_mindist = d->tcrit_opt == 2;
+    P.periodic = 0;
+    for (int side = 0; side < 2; side++)
+        for (int m = 0; m < 3; m++) {
+            P.bcode[side][m] = d->boundary ? d->boundary_code[side][m] : 0;
+            P.bx[side][m] = d->boundary_x[side][m];
+        }
+    for (int m = 0; m < 3; m++) {
+        const int c0 = P.bcode[0][m], c1 = P.bcode[1][m];
+        if ((c0 != 0 && c0 != 2) || (c1 != 0 && c1 != 2)) {
+            c->err = c0 == 1 || c1 == 1 ? "ERROR: Symmetry boundary conditions not implemented yet."
+                                        : "ERROR: Boundary conditions code wrong.";
+            return SPH_ERROR;
+        }
+        if ((c0 == 2) != (c1 == 2)) { c->err = "Error in definition of periodic boundary conditions."; return SPH_ERROR; }
+        if (c0 == 2 && m >= D) { c->err = "periodic axis beyond ndim"; return SPH_ERROR; }
+        P.btype[m] = 1 + (c0 > 0 ? 1 : 0) + (c1 > 0 ? 2 : 0);
+        if (c0 == 2) {
+            P.periodic = 1;
+            P.minadd[m][m] = P.bx[1][m] - P.bx[0][m];
+            P.maxadd[m][m] = -P.bx[1][m] + P.bx[0][m];
+        }
+        P.base_a[m] = P.baseaccel ? d->base_a[m] : 0.0;
+        if (m >= D && P.base_a[m] != 0.0) { c->err = "base acceleration beyond ndim"; return SPH_ERR_UNSUPPORTED; }
+    }
+    // materials (MMaterial cards; EOScalc sound speed, MSimulationInit.cpp:384-440)
+    std::vector<double> mat_eos3(kMaxMat + 1, 1.0);
+    P.hmax = 0.0;
+    for (int k = 1; k <= d->nummat; k++) {
+        const sph_material_desc &md = d->mat[k - 1];
+        if (md.model != 9) { c->err = "only fluid material model 9 is on the GPU path"; return SPH_ERR_UNSUPPORTED; }
+        if (md.eos != 28 && md.eos != 29) { c->err = "EOS must be 28 or 29"; return SPH_ERR_UNSUPPORTED; }
+        Mat &mt = P.mat[k];
+        mt.rho0 = md.rho;
+        mt.nu = md.av_l;
+        mt.sigma = md.surface_tension;
+        mt.h = md.h;
+        mt.eos = md.eos;
+        mt.visc_type = md.visc_type;
+        mt.alpha = md.av_alpha;
+        mt.eos_b = md.eosinput[0];
+        mt.eos_gamma = md.eosinput[1];
+        mt.c = md.eos == 29 ? md.eosinput[0] : sqrt(md.eosinput[0] * md.eosinput[1] / md.rho);
+        mat_eos3[k] = md.eosinput[3] == 0.0 ? 1.0 : md.eosinput[3];
+        if (P.hmax < md.h) P.hmax = md.h;   // InitH option 0 (:336-341)
+    }
+    if (P.hmax <= 0.0) { c->err = "smoothing length must be positive"; return SPH_ERROR; }
+    // material-pair kernel constants (MKernelBSpline.cpp:145-156)
+    for (int a = 1; a <= d->nummat; a++)
+        for (int b = 1; b <= d->nummat; b++) {
+            PairConst &pc = P.pc[a][b];
+            const double havg = 0.5 * (P.mat[a].h + P.mat[b].h);
+            pc.havg = havg;
+            if (D == 3) pc.const1 = 1.0 / (kPI * (havg * havg * havg));
+            else if (D == 2) pc.const1 = 10.0 / (7.0 * kPI * (havg * havg));
+            else pc.const1 = 2.0 / (3.0 * havg);
+            pc.const2 = 0.25 * pc.const1;
+            pc.eps2 = 0.01 * (havg * havg);
+            const double hav = 2.0 * (0.5 * (P.mat[a].h + P.mat[b].h));
+            pc.cut2 = hav * hav;
+        }
+    for (int m = 0; m < D; m++)
+        if (P.btype[m] == 4) {
+            const double gs = 1.0 / (2.0 * P.hmax);
+            const int gl = (int)(gs * (P.bx[1][m] - P.bx[0][m]));
+            if (gl + 2 < 3) { c->err = "periodic extent shorter than one cell"; return SPH_ERR_UNSUPPORTED; }
+        }
+    return SPH_OK;
+}
+
+int create_impl(Context *c, const sph_scene_desc *d)
+{
+    int rc = build_context(c, d);
+    if (rc) return rc;
+    Params &P = c->P;
+    const int N = c->N, D = c->ndim;
+    // ---- particle state in deck order ----
+    std::vector<double> hx[3], hv[3], hm(N), hrho(N), hp(N);
+    std::vector<uint32_t> hinfo(N);
+    std::vector<int> rigid_idx;
+    std::vector<int> per_mat(kMaxMat + 1, 0);
+    for (int i = 0; i < N; i++) {
+        const int mt = d->mat_id[i];
+        if (mt < 1 || mt > d->nummat) {
+            c->err = "Node " + std::to_string(i + 1) + " has incorrect material type.";
+            return SPH_ERROR;
+        }
+        per_mat[mt]++;
+    }
+    for (int dd = 0; dd < D; dd++) {
+        hx[dd].resize(N);
+        hv[dd].resize(N);
+    }
+    double cmm[2][3];
+    for (int m = 0; m < 3; m++) { cmm[0][m] = 1.0e20; cmm[1][m] = -1.0e20; }
+    for (int i = 0; i < N; i++) {
+        const int mt = d->mat_id[i];
+        for (int dd = 0; dd < D; dd++) {
+            hx[dd][i] = d->x[3 * (size_t)i + dd];
+            hv[dd][i] = d->v ? d->v[3 * (size_t)i + dd] : 0.0;
+            // MIN / MAX of global.h:21-22 (InitVariables, MSimulationInit.cpp:222-227)
+            cmm[0][dd] = cmm[0][dd] < hx[dd][i] ? cmm[0][dd] : hx[dd][i];
+            cmm[1][dd] = cmm[1][dd] > hx[dd][i] ? cmm[1][dd] : hx[dd][i];
+            if (P.btype[dd] == 4 && (hx[dd][i] < P.bx[0][dd] || hx[dd][i] >= P.bx[1][dd])) {
+                c->err = "particle outside the periodic interval at start";
+                return SPH_ERR_UNSUPPORTED;
+            }
+        }
+        // InitMass (MSimulationInit.cpp:144-177)
+        if (d->massopt == 1) {
+            if (!d->node_mass) { c->err = "massopt 1 needs node masses"; return SPH_ERROR; }
+            hm[i] = d->node_mass[i];
+        } else {
+            hm[i] = d->mat[mt - 1].mass / per_mat[mt];
+        }
+        // InitRho_P_C / EOScalc (MSimulationInit.cpp:267-311, 384-440)
+        const sph_material_desc &md = d->mat[mt - 1];
+        const double rho0 = md.rho;
+        const double eos3 = md.eosinput[3] == 0.0 ? 1.0 : md.eosinput[3];
+        hrho[i] = (d->init_rhoe == 1 && d->node_rho) ? d->node_rho[i] : rho0 / eos3;
+        double p;
+        if (md.eos == 28) p = md.eosinput[0] * (pow(hrho[i] / rho0, md.eosinput[1]) - 1.0);
+        else p = md.eosinput[0] * (hrho[i] - rho0);
+        if (p > md.strinput0) p = md.strinput0;
+        hp[i] = p;
+        const int disp = d->dispbc ? d->dispbc[i] : 0;
+        if (disp < 0 || disp > 8) { c->err = "dispbc out of range"; return SPH_ERROR; }
+        const int color = d->color ? d->color[i] : 1;
+        if (color < -32768 || color > 32767) { c->err = "colour out of range"; return SPH_ERROR; }
+        uint32_t f = (uint32_t)mt | ((uint32_t)disp << kDispShift) | kActive |
+                     ((uint32_t)(uint16_t)(int16_t)color << kColorShift);
+        if (P.rigid && mt == P.rb_id) {
+            f |= kRigid;
+            rigid_idx.push_back(i);
+        }
+        hinfo[i] = f;
+    }
+    if (P.rigid && rigid_idx.empty()) { c->err = "rigid body has no particles"; return SPH_ERROR; }
+    P.n_rigid = (int)rigid_idx.size();
+
+    // ---- capacities ----
+    long long ncell0 = 1;
+    {
+        const double gs = 1.0 / (2.0 * P.hmax);
+        for (int m = 0; m < D; m++) {
+            long long gl;
+            if (P.btype[m] == 4) gl = (long long)(gs * (P.bx[1][m] - P.bx[0][m])) + 2;
+            else gl = (long long)(gs * (cmm[1][m] - (cmm[0][m] - 0.01 * P.hmax))) + 1;
+            ncell0 *= std::max(gl, 1LL);
+        }
+    }
+    long long cap = ncell0 + ncell0 / 4 + 4096;
+    if (cap > 0x7ffffff0LL) { c->err = "neighbour grid too large"; return SPH_ERROR; }
+    int gcap = 0;
+    if (P.periodic) {
+        const double gs = 1.0 / (2.0 * P.hmax);
+        long long g = 4096;
+        for (int m = 0; m < D; m++)
+            if (P.btype[m] == 4) {
+                const long long gl = (long long)(gs * (P.bx[1][m] - P.bx[0][m]));
+                g += 2 * ((long long)N * 3 / std::max(gl, 1LL)) ;
+            }
+        gcap = (int)std::min<long long>(g, 2LL * N + 4096);
+    }
+    HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    HIPCHK(c, hipHostMalloc((void **)&c->hctl, sizeof(Ctl), hipHostMallocDefault));
+    memset(c->hctl, 0, sizeof(Ctl));
+    if (dalloc(c, c->part_allocs, &c->A.ctl, 1)) return SPH_ERR_HIP;
+    // the control block is not part of part_allocs' lifetime: keep it separate
+    c->part_allocs.pop_back();
+    if (alloc_particles(c, gcap)) return SPH_ERR_HIP;
+    if (alloc_cells(c, (int)cap)) return SPH_ERR_HIP;
+    if (P.rigid) {
+        if (dalloc(c, c->cell_allocs, &c->A.rigid_idx, rigid_idx.size())) return SPH_ERR_HIP;
+        // rigid index list must survive cell re-allocation: move to its own slot
+        c->cell_allocs.pop_back();
+        HIPCHK(c, hipMemcpy(c->A.rigid_idx, rigid_idx.data(), rigid_idx.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    }
+    Arrays &A = c->A;
+    for (int dd = 0; dd < D; dd++) {
+        HIPCHK(c, hipMemcpy(A.x[dd], hx[dd].data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(A.xs[dd], hx[dd].data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(A.v[dd], hv[dd].data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(A.a[dd], 0, N * 8));
+        if (P.st) {
+            HIPCHK(c, hipMemset(A.nrm[dd], 0, N * 8));
+            HIPCHK(c, hipMemset(A.nn[dd], 0, N * 8));
+        }
+    }
+    HIPCHK(c, hipMemcpy(A.m, hm.data(), N * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(A.rho, hrho.data(), N * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(A.rho_new, hrho.data(), N * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(A.p, hp.data(), N * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(A.prho2, 0, N * 8));
+    HIPCHK(c, hipMemcpy(A.info, hinfo.data(), N * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(A.trd, 0, N * 8));
+    HIPCHK(c, hipMemset(A.nnbr, 0, N * 4));
+    if (P.st) {
+        HIPCHK(c, hipMemset(A.Nf, 0, N * 8));
+        HIPCHK(c, hipMemset(A.kappa, 0, N * 8));
+    }
+    // ---- control block ----
+    Ctl &h = *c->hctl;
+    h.timestep = 1;
+    h.ptime = 0.0;
+    h.dt = h.dtold = 0.0;
+    for (int s = 0; s < 2; s++)
+        for (int m = 0; m < 3; m++) {
+            h.cmm[s][m] = cmm[s][m];
+            h.bbox_key[s][m] = dkey(s == 0 ? 1.0e20 : -1.0e20);
+        }
+    h.critts_key = dkey(1.0e20);
+    if (push_ctl(c)) return SPH_ERR_HIP;
+    // ---- first neighbours (no images), critical step, first dt (:71-82) ----
+    rc = run_phase(c, [&] {
+        launch_setup_linked_list(c->P, c->A, c->st);
+        launch_neighbours(c->P, c->A, 0, c->st);
+        launch_strain(c->P, c->A, M_CRIT, c->st);
+    });
+    if (rc) return rc;
+    const double critts = dunkey(h.critts_key);
+    h.critts = critts;
+    h.critts_key = dkey(1.0e20);
+    if (fabs(d->itss) > 1.e-10) h.dt = d->itss < critts * P.tssf ? d->itss : critts * P.tssf;
+    else h.dt = critts * P.tssf;
+    return push_ctl(c);
+}
+
+Context *ctx_of(void *p) { return (Context *)p; }
+
+int download_vec(Context *c, double *const *src, double *out)
+{
+    const int N = c->N;
+    std::vector<double> tmp(N);
+    for (int d = 0; d < 3; d++) {
+        if (d < c->ndim && src[d]) {
+            HIPCHK(c, hipMemcpy(tmp.data(), src[d], N * 8, hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; i++) out[3 * (size_t)i + d] = tmp[i];
+        } else {
+            for (int i = 0; i < N; i++) out[3 * (size_t)i + d] = 0.0;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+// =============================== C ABI =====================================
+extern "C" {
+
+int sph_desc_size(void) { return (int)sizeof(sph_scene_desc); }
+int sph_material_desc_size(void) { return (int)sizeof(sph_material_desc); }
+
+int sph_create(const sph_scene_desc *desc, int arm, void **out, char *err, int errlen)
+{
+    if (out) *out = nullptr;
+    if (!desc || !out) return SPH_ERROR;
+    Context *c = new Context();
+    c->arm = arm;
+    int rc = arm == SPH_ARM_FP64 ? create_impl(c, desc) : SPH_ERR_UNSUPPORTED;
+    if (arm != SPH_ARM_FP64) c->err = "unknown arithmetic arm";
+    if (rc) {
+        if (err && errlen > 0) snprintf(err, (size_t)errlen, "%s", c->err.c_str());
+        sph_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return SPH_OK;
+}
+
+void sph_destroy(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return;
+    if (c->st) hipStreamSynchronize(c->st);
+    free_list(c->part_allocs);
+    free_list(c->cell_allocs);
+    if (c->A.ctl) hipFree(c->A.ctl);
+    if (c->A.rigid_idx) hipFree(c->A.rigid_idx);
+    if (c->canon) hipFree(c->canon);
+    if (c->hctl) hipHostFree(c->hctl);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char *sph_last_error(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return c ? c->err.c_str() : "null context";
+}
+
+int sph_neighbours(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] { launch_setup_linked_list(c->P, c->A, c->st); });
+}
+
+int sph_ghost_setup(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    if (!c->P.periodic) return SPH_OK;
+    return run_phase(c, [&] { launch_ghost_setup(c->P, c->A, c->st); });
+}
+
+int sph_linked_list_neighbours(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] { launch_neighbours(c->P, c->A, 0, c->st); });
+}
+
+int sph_strain(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] {
+        launch_shift(c->P, c->A, c->st);
+        launch_strain(c->P, c->A, M_STRAIN, c->st);
+    });
+}
+
+int sph_calculate_surface_normals(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    if (!c->P.st) return SPH_OK;
+    return run_phase(c, [&] { launch_normals(c->P, c->A, c->st); });
+}
+
+int sph_constitutive(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] { launch_strain(c->P, c->A, M_CONST | M_INPLACE, c->st); });
+}
+
+int sph_calc_critical_timestep(void *ctx, double *critts)
+{
+    Context *c = ctx_of(ctx);
+    int rc = run_phase(c, [&] { launch_strain(c->P, c->A, M_CRIT, c->st); });
+    if (rc == 0 && critts) *critts = dunkey(c->hctl->critts_key);
+    return rc;
+}
+
+int sph_time_advance(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] { launch_advance(c->P, c->A, c->st); });
+}
+
+int sph_momentum(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] { launch_momentum(c->P, c->A, 0, c->st); });
+}
+
+int sph_boundcond(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] { launch_integrate(c->P, c->A, M_BC, c->st); });
+}
+
+int sph_update_velocity(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] {
+        launch_rigid_kin(c->P, c->A, c->st);
+        launch_integrate(c->P, c->A, M_VEL, c->st);
+    });
+}
+
+int sph_move_particles(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] {
+        launch_integrate(c->P, c->A, M_MOVE, c->st);
+        launch_finish(c->P, c->A, 1, 0, c->st);
+    });
+}
+
+int sph_end_step(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    return run_phase(c, [&] { launch_finish(c->P, c->A, 0, 1, c->st); });
+}
+
+int sph_solution(void *ctx, int max_steps, int *steps_done)
+{
+    Context *c = ctx_of(ctx);
+    if (steps_done) *steps_done = 0;
+    if (sync_ctl(c)) return SPH_ERR_HIP;
+    const int t0 = c->hctl->timestep;
+    int remaining = max_steps;
+    int guard = 0;
+    while (remaining > 0 && !c->hctl->done) {
+        const int chunk = std::min(remaining, 64);
+        for (int k = 0; k < chunk; k++) fused_step(c);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            c->err = std::string("HIP launch error: ") + hipGetErrorString(e);
+            return SPH_ERR_HIP;
+        }
+        int rc = check_device(c);
+        if (rc > 0) {
+            if (steps_done) *steps_done = c->hctl->timestep - t0;
+            return rc;
+        }
+        if (rc == -1 && ++guard > 16) {
+            c->err = "ERROR: capacity repair did not converge.";
+            return SPH_ERROR;
+        }
+        remaining = max_steps - (c->hctl->timestep - t0);
+    }
+    if (steps_done) *steps_done = c->hctl->timestep - t0;
+    return SPH_OK;
+}
+
+int sph_synchronize(void *ctx)
+{
+    Context *c = ctx_of(ctx);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return SPH_OK;
+}
+
+int sph_get_globals(void *ctx, sph_globals *out)
+{
+    Context *c = ctx_of(ctx);
+    if (sync_ctl(c)) return SPH_ERR_HIP;
+    const Ctl &h = *c->hctl;
+    memset(out, 0, sizeof *out);
+    out->dt = h.dt;
+    out->dtold = h.dtold;
+    out->ptime = h.ptime;
+    out->critts = h.critts;
+    for (int s = 0; s < 2; s++)
+        for (int m = 0; m < 3; m++) out->coord_minmax[s][m] = h.cmm[s][m];
+    out->hmax = c->P.hmax;
+    for (int m = 0; m < 3; m++) {
+        out->rb_R[m] = h.rbR[m];
+        out->rb_V[m] = h.rbV[m];
+        out->rb_OMEGA[m] = h.rbW[m];
+        out->rb_A[m] = h.rbA[m];
+        out->rb_ALPHA[m] = h.rbAL[m];
+        out->grid_lim[m] = h.glim[m];
+    }
+    out->timestep = h.timestep;
+    out->ngp = h.ngp;
+    out->done = h.done;
+    out->error = h.err;
+    out->np = c->N;
+    return SPH_OK;
+}
+
+int sph_download(void *ctx, int field, double *out)
+{
+    Context *c = ctx_of(ctx);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const int N = c->N;
+    const Arrays &A = c->A;
+    std::vector<uint32_t> info;
+    auto scalar = [&](const double *src) -> int {
+        if (!src) {
+            c->err = "field not kept for this scene";
+            return SPH_ERROR;
+        }
+        HIPCHK(c, hipMemcpy(out, src, N * 8, hipMemcpyDeviceToHost));
+        return 0;
+    };
+    auto get_info = [&]() -> int {
+        info.resize(N);
+        HIPCHK(c, hipMemcpy(info.data(), A.info, N * 4, hipMemcpyDeviceToHost));
+        return 0;
+    };
+    switch (field) {
+    case SPH_F_X: return download_vec(c, A.x, out);
+    case SPH_F_V: return download_vec(c, A.v, out);
+    case SPH_F_A: return download_vec(c, A.a, out);
+    case SPH_F_NORMAL: return download_vec(c, A.nrm, out);
+    case SPH_F_NNORMAL: return download_vec(c, A.nn, out);
+    case SPH_F_RHO: return scalar(A.rho);
+    case SPH_F_P: return scalar(A.p);
+    case SPH_F_MASS: return scalar(A.m);
+    case SPH_F_TRD: return scalar(A.trd);
+    case SPH_F_MINDIST: return scalar(A.mindist);
+    case SPH_F_N: return scalar(A.Nf);
+    case SPH_F_KAPPA: return scalar(A.kappa);
+    case SPH_F_CRITTS: return scalar(A.critts_p);
+    case SPH_F_C:
+    case SPH_F_H:
+    case SPH_F_ACTIVE:
+        if (get_info()) return SPH_ERR_HIP;
+        for (int i = 0; i < N; i++) {
+            const Mat &mt = c->P.mat[info_mat(info[i])];
+            out[i] = field == SPH_F_C ? mt.c : field == SPH_F_H ? mt.h : ((info[i] & kActive) ? 1.0 : 0.0);
+        }
+        return SPH_OK;
+    case SPH_F_VABS: {
+        std::vector<double> v(3 * (size_t)N);
+        if (download_vec(c, A.v, v.data())) return SPH_ERR_HIP;
+        for (int i = 0; i < N; i++) {
+            double s = 0.0;
+            for (int d = 0; d < c->ndim; d++) s += v[3 * (size_t)i + d] * v[3 * (size_t)i + d];
+            out[i] = sqrt(s);
+        }
+        return SPH_OK;
+    }
+    default:
+        c->err = "field not available";
+        return SPH_ERROR;
+    }
+}
+
+int sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr)
+{
+    Context *c = ctx_of(ctx);
+    const int N = c->N;
+    if (!c->canon) HIPCHK(c, hipMalloc((void **)&c->canon, (size_t)N * c->maxnbr * 8));
+    const int mn = std::min(maxnbr, c->maxnbr);
+    launch_canon(c->P, c->A, c->canon, mn, c->st);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(nnbr, c->A.nnbr, N * 4, hipMemcpyDeviceToHost));
+    std::vector<long long> tmp((size_t)N * mn);
+    HIPCHK(c, hipMemcpy(tmp.data(), c->canon, tmp.size() * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < nnbr[i] && k < mn; k++) canon[(size_t)i * maxnbr + k] = tmp[(size_t)i * mn + k];
+    return SPH_OK;
+}
+
+}  // extern "C"

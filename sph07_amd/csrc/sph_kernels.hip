@@ -1,0 +1,1137 @@
+// sph_kernels.hip -- HIP kernels of the per-timestep particle loop, FP64 arm.
+//
+// Compiled with -ffp-contract=off: every FP64 expression keeps the operand
+// order of the reference line it restates and no FMA is formed, so a
+// particle's sums are bit-identical to the reference's sequential loops
+// (the pair sweeps visit neighbours in the reference's list order).
+//
+// One thread per particle.  Kernels read the device control block and return
+// at once when a previous kernel of the step flagged an error or the stop
+// test fired, so a queued sequence of steps is a no-op after either.
+#include "sph_kernels.hpp"
+
+#include <math.h>
+
+namespace sph {
+
+#define CTL_STOP(ctl) ((ctl)->err != 0 || (ctl)->done != 0)
+
+__device__ __forceinline__ void set_err(Ctl *ctl, int code, int who)
+{
+    if (atomicCAS(&ctl->err, 0, code) == 0) ctl->err_particle = who;
+}
+
+// ---------------------------------------------------------------------------
+// wave / block primitives (64-lane wavefronts)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over the block; *total receives the block sum.
+__device__ __forceinline__ int block_excl_scan(int v, int *total)
+{
+    __shared__ int wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        int s = lane < kBlock / 64 ? wsum[lane] : 0;
+        s = wave_incl_scan(s);
+        if (lane < kBlock / 64) wsum[lane] = s;
+    }
+    __syncthreads();
+    int base = w ? wsum[w - 1] : 0;
+    *total = wsum[kBlock / 64 - 1];
+    __syncthreads();
+    return base + inc - v;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// B-spline gradient, MKernelBSpline::GradW (MKernelBSpline.cpp:137-187)
+// with the material-pair constants precomputed on the host.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void gradw64(double dw[D], const double *xi, const double *xj,
+                                        const PairConst &pc)
+{
+    double xij[D], rad = 0.0;
+#pragma unroll
+    for (int n = 0; n < D; n++) {
+        xij[n] = xi[n] - xj[n];
+        rad += xij[n] * xij[n];
+    }
+    rad = sqrt(rad);
+    const double z = rad / pc.havg;
+    double dW;
+    if (z < 1.0)
+        dW = pc.const1 * ((2.25 * z - 3.0) * z);
+    else if (z < 2.0)
+        dW = -3.0 * pc.const2 * ((2.0 - z) * (2.0 - z));
+    else
+        dW = 0.0;
+    const double den = rad * pc.havg;
+#pragma unroll
+    for (int n = 0; n < D; n++) dw[n] = (rad == 0.0) ? 0.0 : dW * xij[n] / den;
+}
+
+// ---------------------------------------------------------------------------
+// grid: MNeighbourVars::InitGrid (MNeighbourVars.cpp:18-66)
+// ---------------------------------------------------------------------------
+__global__ void k_grid_setup(Params P, Ctl *ctl)
+{
+    if (CTL_STOP(ctl)) return;
+    long long ncell = 1;
+    for (int i = 0; i < 3; i++) {
+        ctl->gmin[i] = 0.0;
+        ctl->glim[i] = 1;
+        ctl->gsize[i] = 1.0 / (2.0 * P.hmax);
+    }
+    for (int i = 0; i < P.ndim; i++) {
+        if (P.btype[i] == 4) {
+            int gl = (int)(ctl->gsize[i] * (P.bx[1][i] - P.bx[0][i]));
+            gl = gl > 0 ? gl : 0;
+            ctl->gsize[i] = gl / (P.bx[1][i] - P.bx[0][i]);
+            ctl->gmin[i] = P.bx[0][i] - 1.0 / ctl->gsize[i];
+            ctl->glim[i] = gl + 2;
+        } else {
+            ctl->gmin[i] = ctl->cmm[0][i] - 0.01 * P.hmax;
+            ctl->glim[i] = (int)(ctl->gsize[i] * (ctl->cmm[1][i] - ctl->gmin[i])) + 1;
+        }
+    }
+    for (int i = 0; i < 3; i++) ncell *= (ctl->glim[i] > 0 ? ctl->glim[i] : 0);
+    if (ncell <= 0 || ncell > P.ncell_cap) {
+        ctl->ncell = (int)(ncell > 0x7fffffffLL ? 0x7fffffff : ncell);
+        set_err(ctl, kErrGridCap, -1);
+        return;
+    }
+    ctl->ncell = (int)ncell;
+}
+
+// cell_count[0..ncell] and image counts cleared
+__global__ void k_zero_cells(Params P, Arrays A)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int n = ctl->ncell;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= n; c += gridDim.x * blockDim.x) {
+        A.cell_count[c] = 0;
+        if (P.periodic && c < n) A.gcount[c] = 0;
+    }
+}
+
+// MSolution::SetupLinkedList (MSolution.cpp:116-148): cell of each computable
+// particle (MNeighbourVars::GetBoxCoords :83-89, truncation, no clamp) and an
+// arbitrary slot inside the cell; the order inside a cell is fixed by k_bin_rank.
+template <int D>
+__global__ void k_bin_count(Params P, Arrays A)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    uint32_t f = A.info[i];
+    if (!(f & kActive)) { A.cell_of[i] = -1; return; }
+    int bc[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; d++) bc[d] = (int)(ctl->gsize[d] * (A.x[d][i] - ctl->gmin[d]));
+    if (ctl->timestep == 1 && (bc[0] < 0 || bc[0] > ctl->glim[0] - 1)) {
+        A.info[i] = f & ~kActive;          // outside the (single) process slab
+        A.cell_of[i] = -1;
+        return;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        if (bc[d] < 0 || bc[d] >= ctl->glim[d]) {
+            set_err(ctl, kErrOutOfGrid, i);
+            A.cell_of[i] = -1;
+            return;
+        }
+    const int c = ctl->glim[2] * (ctl->glim[1] * bc[0] + bc[1]) + bc[2];
+    A.cell_of[i] = c;
+    A.slot[i] = atomicAdd(&A.cell_count[c], 1);
+}
+
+__global__ void k_bin_scatter(Params P, Arrays A)
+{
+    if (CTL_STOP(A.ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const int c = A.cell_of[i];
+    if (c < 0) return;
+    A.tmp[A.cell_start[c] + A.slot[i]] = i;
+}
+
+// Order inside a cell: descending particle number, the order head insertion
+// leaves in the reference's linked lists (MSolution.cpp:143-144).
+__global__ void k_bin_rank(Params P, Arrays A)
+{
+    if (CTL_STOP(A.ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const int c = A.cell_of[i];
+    if (c < 0) return;
+    const int s = A.cell_start[c], e = A.cell_start[c + 1];
+    int r = 0;
+    for (int t = s; t < e; t++) r += A.tmp[t] > i;
+    A.entries[s + r] = i;
+}
+
+// ---------------------------------------------------------------------------
+// device-wide exclusive scan of n = *n_ptr ints (wave scans + LDS)
+// ---------------------------------------------------------------------------
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ void k_scan_reduce(const int *in, const int *n_ptr, int *aux, const Ctl *ctl)
+{
+    if (CTL_STOP(ctl)) return;
+    const int n = *n_ptr;
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) s += (base + q < n) ? in[base + q] : 0;
+    int total;
+    block_excl_scan(s, &total);
+    if (threadIdx.x == 0) aux[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of the tile sums, total written to out[n]
+__global__ void k_scan_aux(int *aux, int nblocks, const int *n_ptr, int *out, const Ctl *ctl)
+{
+    if (CTL_STOP(ctl)) return;
+    int carry = 0;
+    for (int b0 = 0; b0 < nblocks; b0 += kBlock) {
+        const int b = b0 + threadIdx.x;
+        int v = b < nblocks ? aux[b] : 0;
+        int total;
+        int ex = block_excl_scan(v, &total);
+        if (b < nblocks) aux[b] = carry + ex;
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[*n_ptr] = carry;
+}
+
+__global__ void k_scan_down(const int *in, int *out, const int *n_ptr, const int *aux, const Ctl *ctl)
+{
+    if (CTL_STOP(ctl)) return;
+    const int n = *n_ptr;
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems], s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+        v[q] = (base + q < n) ? in[base + q] : 0;
+        s += v[q];
+    }
+    int total;
+    int ex = block_excl_scan(s, &total) + aux[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+        if (base + q < n) out[base + q] = ex;
+        ex += v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// periodic images: MSolution::GhostSetup / GenerateGhostParticles
+// (MSolution.cpp:811-918), MGhostParticleData::CreateGhostFromReal
+// (MGhostParticleData.cpp:12-36).  One pass per periodic side, in the
+// reference's pass order; the source slab is enumerated z-major, x fastest.
+// ---------------------------------------------------------------------------
+struct PassGeom {
+    int lo[3], hi[3], off[3], n;
+};
+
+__device__ __forceinline__ PassGeom pass_geom(const Ctl *ctl, int m, int side, const double *add)
+{
+    PassGeom g;
+    for (int d = 0; d < 3; d++) {
+        g.lo[d] = 0;
+        g.hi[d] = ctl->glim[d] - 1;
+        g.off[d] = 0;
+        if (add[d] != 0.0) g.off[d] = (add[d] >= 0 ? 1 : -1) * (ctl->glim[d] - 2);
+    }
+    g.lo[m] = g.hi[m] = side == 0 ? 1 : ctl->glim[m] - 2;
+    g.n = (g.hi[0] - g.lo[0] + 1) * (g.hi[1] - g.lo[1] + 1) * (g.hi[2] - g.lo[2] + 1);
+    return g;
+}
+
+__device__ __forceinline__ void slab_coords(const PassGeom &g, int s, int &i, int &j, int &k)
+{
+    const int nx = g.hi[0] - g.lo[0] + 1, ny = g.hi[1] - g.lo[1] + 1;
+    i = g.lo[0] + s % nx;
+    j = g.lo[1] + (s / nx) % ny;
+    k = g.lo[2] + s / (nx * ny);
+}
+
+// A source cell contributes only when its list head is a real particle:
+// it holds particles and no image yet (`if (id<0) continue`, :906-909).
+__global__ void k_reset_ngp(Ctl *ctl)
+{
+    if (CTL_STOP(ctl)) return;
+    ctl->ngp = 0;   // MGhostParticleData::DeleteContents (MGhostParticleData.cpp:48-53)
+}
+
+__global__ void k_ghost_count(Params P, Arrays A, int m, int side)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const double *add = side == 0 ? &P.minadd[0][m] : &P.maxadd[0][m];
+    double addv[3] = {add[0], add[3], add[6]};
+    PassGeom g = pass_geom(ctl, m, side, addv);
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.n; s += gridDim.x * blockDim.x) {
+        int i, j, k;
+        slab_coords(g, s, i, j, k);
+        const int c = ctl->glim[2] * (ctl->glim[1] * i + j) + k;
+        const int cnt = A.cell_start[c + 1] - A.cell_start[c];
+        A.slab_cnt[s] = (cnt > 0 && A.gcount[c] == 0) ? cnt : 0;
+    }
+}
+
+__global__ void k_ghost_scan(Params P, Arrays A, int m, int side)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const double *add = side == 0 ? &P.minadd[0][m] : &P.maxadd[0][m];
+    double addv[3] = {add[0], add[3], add[6]};
+    PassGeom g = pass_geom(ctl, m, side, addv);
+    int carry = ctl->ngp;
+    for (int s0 = 0; s0 < g.n; s0 += kBlock) {
+        const int s = s0 + threadIdx.x;
+        int v = s < g.n ? A.slab_cnt[s] : 0;
+        int total;
+        int ex = block_excl_scan(v, &total);
+        if (s < g.n) A.slab_off[s] = carry + ex;
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (carry > P.gcap) set_err(ctl, kErrGhostCap, carry);
+        else ctl->ngp = carry;
+    }
+}
+
+__global__ void k_ghost_emit(Params P, Arrays A, int m, int side)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const double *add = side == 0 ? &P.minadd[0][m] : &P.maxadd[0][m];
+    double addv[3] = {add[0], add[3], add[6]};
+    PassGeom g = pass_geom(ctl, m, side, addv);
+    int shift[3];
+    for (int d = 0; d < 3; d++) shift[d] = addv[d] != 0.0 ? (addv[d] >= 0 ? 1 : -1) : 0;
+    const int shift_code = (shift[0] + 1) * 9 + (shift[1] + 1) * 3 + (shift[2] + 1);
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.n; s += gridDim.x * blockDim.x) {
+        const int cnt = A.slab_cnt[s];
+        if (cnt == 0) continue;
+        int i, j, k;
+        slab_coords(g, s, i, j, k);
+        const int c = ctl->glim[2] * (ctl->glim[1] * i + j) + k;
+        const int tc = ctl->glim[2] * (ctl->glim[1] * (i + g.off[0]) + (j + g.off[1])) + (k + g.off[2]);
+        if (A.gcount[tc] != 0) {
+            // a cell fed by two passes would need a merge of image runs
+            set_err(ctl, 1, -2);
+            continue;
+        }
+        const int g0 = A.slab_off[s];
+        A.gfirst[tc] = g0;
+        A.gcount[tc] = cnt;
+        const int s0 = A.cell_start[c];
+        for (int r = 0; r < cnt; r++) {
+            const int src = A.entries[s0 + r];   // descending particle number
+            const int gi = P.N + g0 + r;
+            for (int d = 0; d < P.ndim; d++) {
+                const double xv = A.x[d][src] + addv[d];
+                A.x[d][gi] = xv;
+                A.xs[d][gi] = xv;
+                A.v[d][gi] = A.v[d][src];
+            }
+            const double rho = A.rho[src], mass = A.m[src];
+            A.m[gi] = mass;
+            A.rho[gi] = rho;
+            A.rho_new[gi] = rho;
+            A.vol[gi] = mass / rho;
+            A.p[gi] = 0.0;        // images keep p = 0 (not copied, :28)
+            A.prho2[gi] = 0.0;
+            A.info[gi] = (A.info[src] & (kMatMask | kRigid | (0xFFFFu << kColorShift))) | kGhost;
+            A.gsrc[g0 + r] = src;
+            A.gshift[g0 + r] = shift_code;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MSolution::LinkedListNeighbours (MSolution.cpp:150-215): cells r(x) ->
+// s(y) -> t(z) ascending, in each cell images (newest first) then real
+// particles (descending number); strict d^2 < (2*AVG(h_i,h_j))^2.
+// With shift != 0 the epilogue also writes the half-step-back positions of
+// MSolution::Strain (:334-338) for the fused step.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void k_neighbours(Params P, Arrays A, int shift)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const uint32_t fi = A.info[i];
+    const double mi = A.m[i];
+    A.vol[i] = mi / A.rho[i];
+    if (!(fi & kActive)) { A.nnbr[i] = 0; return; }
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) xi[d] = A.x[d][i];
+    int bc[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        bc[d] = (int)(ctl->gsize[d] * (xi[d] - ctl->gmin[d]));
+        lo[d] = bc[d] - 1 > 0 ? bc[d] - 1 : 0;
+        hi[d] = bc[d] + 1 < ctl->glim[d] - 1 ? bc[d] + 1 : ctl->glim[d] - 1;
+    }
+    const int gl1 = ctl->glim[1], gl2 = ctl->glim[2];
+    const int mat_i = info_mat(fi);
+    const bool onemat = P.nmat == 1;
+    double cut2 = P.pc[mat_i][mat_i].cut2;
+    double mind = 1.0e20;
+    int nn = 0;
+    const size_t stride = (size_t)P.Ncap;
+    for (int r = lo[0]; r <= hi[0]; r++)
+        for (int s = lo[1]; s <= hi[1]; s++)
+            for (int t = lo[2]; t <= hi[2]; t++) {
+                const int c = gl2 * (gl1 * r + s) + t;
+                // images first (most recently created first), then reals
+                int ng = P.periodic ? A.gcount[c] : 0;
+                const int g0 = ng ? A.gfirst[c] : 0;
+                const int s0 = A.cell_start[c], s1 = A.cell_start[c + 1];
+                for (int q = ng + (s1 - s0) - 1; q >= 0; q--) {
+                    int j;
+                    if (q >= s1 - s0) j = P.N + g0 + (q - (s1 - s0));
+                    else j = A.entries[s1 - 1 - q];
+                    if (j == i) continue;
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        const double dd = xi[d] - A.x[d][j];
+                        d2 += dd * dd;
+                    }
+                    if (!onemat) cut2 = P.pc[mat_i][info_mat(A.info[j])].cut2;
+                    if (d2 < cut2) {
+                        if (P.want_mindist) {
+                            const double r1 = sqrt(d2);
+                            mind = mind < r1 ? mind : r1;
+                        }
+                        if (nn < P.maxnbr) {
+                            A.nbr[(size_t)nn * stride + i] = j;
+                            nn++;
+                        } else {
+                            set_err(ctl, kErrNbrOverflow, i);
+                            A.nnbr[i] = nn;
+                            return;
+                        }
+                    }
+                }
+            }
+    A.nnbr[i] = nn;
+    if (P.want_mindist) A.mindist[i] = mind;
+    if (shift) {
+        const double dt = ctl->dt;
+#pragma unroll
+        for (int d = 0; d < D; d++) A.xs[d][i] = xi[d] - 0.5 * A.v[d][i] * dt;
+    }
+}
+
+// MSolution::Strain half-step back (MSolution.cpp:334-338), unfused path
+template <int D>
+__global__ void k_shift(Params P, Arrays A)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const bool act = A.info[i] & kActive;
+    const double dt = ctl->dt;
+#pragma unroll
+    for (int d = 0; d < D; d++) A.xs[d][i] = act ? A.x[d][i] - 0.5 * A.v[d][i] * dt : A.x[d][i];
+}
+
+// ---------------------------------------------------------------------------
+// EOS of model 9 (MConstitutive.cpp:135-186)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double eos_p(const Mat &mt, double rho)
+{
+    if (mt.eos == 29) return (mt.c * mt.c) * (rho - mt.rho0);
+    return mt.eos_b * (pow(rho / mt.rho0, mt.eos_gamma) - 1.0);
+}
+
+// critical step of one particle, MSolution::CalcCriticalTimestep (:34-94)
+template <int D>
+__device__ __forceinline__ double crit_one(const Params &P, const Arrays &A, const Ctl *ctl, int i,
+                                           const Mat &mt, double rho_i)
+{
+    double ct;
+    const double h = mt.h;
+    if (P.tcrit_opt == 3) {
+        ct = 0.25 * h / mt.c;
+        const double b = 0.125 * (h * h) / mt.nu;
+        ct = ct < b ? ct : b;
+        double aabs = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) aabs += A.a[d][i] * A.a[d][i];
+        aabs = sqrt(aabs);
+        const double c3 = 0.25 * sqrt(h / aabs);
+        ct = ct < c3 ? ct : c3;
+        if (P.st) {
+            const double c4 = 0.25 * sqrt(rho_i * ((h * h) * h) / 2 / 3.141592654 / mt.sigma);
+            ct = ct < c4 ? ct : c4;
+        }
+    } else {
+        double va = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) va += A.v[d][i] * A.v[d][i];
+        va = sqrt(va);
+        const double len = P.tcrit_opt == 1 ? h : A.mindist[i];
+        ct = len / (mt.c + va);
+        if (ctl->timestep == 1) ct = 1.e-09;
+    }
+    return ct;
+}
+
+__device__ __forceinline__ void block_min_to(unsigned long long *key, double v)
+{
+    __shared__ double red[kBlock / 64];
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int k = 1; k < kBlock / 64; k++) m = fmin(m, red[k]);
+        atomicMin(key, dkey(m));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 1: MSolution::ClassicStrain (:362-413) -> trace of D; optionally the
+// RhoUpdate + EOS epilogue (:463-477, MConstitutive.cpp:178-186) and the
+// per-particle critical step with a block min into the control block.
+// ---------------------------------------------------------------------------
+enum { S_STRAIN = 1, S_CONST = 2, S_CRIT = 4, S_INPLACE = 8 };
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double crit = 1.0e20;
+    if (i < P.N) {
+        const uint32_t fi = A.info[i];
+        const bool act = fi & kActive;
+        const int mat_i = info_mat(fi);
+        const Mat &mt = P.mat[mat_i];
+        const double dt = ctl->dt;
+        double tr = 0.0;
+        if ((mask & S_STRAIN) && act) {
+            double xi[D], vi[D], g[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                xi[d] = A.xs[d][i];
+                vi[d] = A.v[d][i];
+                g[d] = 0.0;
+            }
+            const int nn = A.nnbr[i];
+            const size_t stride = (size_t)P.Ncap;
+            for (int k = 0; k < nn; k++) {
+                const int j = A.nbr[(size_t)k * stride + i];
+                const double volj = A.vol[j];
+                const PairConst &pc = P.pc[mat_i][P.nmat == 1 ? mat_i : info_mat(A.info[j])];
+                double xj[D], dw[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) xj[d] = A.xs[d][j];
+                gradw64<D>(dw, xi, xj, pc);
+#pragma unroll
+                for (int d = 0; d < D; d++) g[d] += volj * (A.v[d][j] - vi[d]) * dw[d];
+            }
+#pragma unroll
+            for (int d = 0; d < D; d++) tr += 0.5 * (g[d] + g[d]);
+            A.trd[i] = tr;
+            // half step forward again (:353-357), not bit-idempotent by design
+#pragma unroll
+            for (int d = 0; d < D; d++) A.x[d][i] = xi[d] + 0.5 * vi[d] * dt;
+        } else if (mask & S_CONST) {
+            tr = A.trd[i];
+        }
+        double rho_i = A.rho[i];
+        if (mask & S_CONST) {
+            if (act) {
+                rho_i = rho_i * (1.0 - tr * dt);
+                const double p = eos_p(mt, rho_i);
+                A.p[i] = p;
+                A.prho2[i] = p / (rho_i * rho_i);
+            }
+            A.rho_new[i] = rho_i;
+            if (mask & S_INPLACE) A.rho[i] = rho_i;
+        }
+        if (mask & S_CRIT) {
+            const double ct = crit_one<D>(P, A, ctl, i, mt, rho_i);
+            if (!(mask & S_CONST)) A.critts_p[i] = ct;
+            if (act && !(fi & kRigid)) crit = ct;
+        }
+    }
+    if (mask & S_CRIT) block_min_to(&ctl->critts_key, crit);
+}
+
+// ---------------------------------------------------------------------------
+// CSF, MSolution::CalculateSurfaceNormals (:1429-1493): colour-gradient normal
+// and its normalised form, then the divergence of the normalised normal.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_normals(Params P, Arrays A)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const uint32_t fi = A.info[i];
+    if (!(fi & kActive)) return;
+    const int mat_i = info_mat(fi), col_i = info_color(fi);
+    double xi[D], nrm[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        xi[d] = A.x[d][i];
+        nrm[d] = 0.0;
+    }
+    const int nn = A.nnbr[i];
+    const size_t stride = (size_t)P.Ncap;
+    for (int k = 0; k < nn; k++) {
+        const int j = A.nbr[(size_t)k * stride + i];
+        const uint32_t fj = A.info[j];
+        const double volj = A.m[j] / A.rho[j];
+        const PairConst &pc = P.pc[mat_i][info_mat(fj)];
+        double xj[D], dw[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) xj[d] = A.x[d][j];
+        gradw64<D>(dw, xi, xj, pc);
+        const double f = volj * (double)(info_color(fj) - col_i);
+#pragma unroll
+        for (int d = 0; d < D; d++) nrm[d] += f * dw[d];
+    }
+    double in = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) in += nrm[d] * nrm[d];
+    in = sqrt(in);
+    const double Nf = in > 1.e-2 * P.mat[mat_i].h ? 1.0 : 0.0;
+    A.Nf[i] = Nf;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        A.nrm[d][i] = nrm[d];
+        A.nn[d][i] = Nf == 1.0 ? (1. / in) * nrm[d] : 0.0;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_curvature(Params P, Arrays A)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const uint32_t fi = A.info[i];
+    if (!(fi & kActive)) return;
+    const int mat_i = info_mat(fi);
+    double xi[D], nni[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        xi[d] = A.x[d][i];
+        nni[d] = A.nn[d][i];
+    }
+    const double Ni = A.Nf[i];
+    double div = 0.0;
+    const int nn = A.nnbr[i];
+    const size_t stride = (size_t)P.Ncap;
+    for (int k = 0; k < nn; k++) {
+        const int j = A.nbr[(size_t)k * stride + i];
+        const uint32_t fj = A.info[j];
+        const bool img = fj & kGhost;        // images: N = 0, normals zero
+        const double volj = A.m[j] / A.rho[j];
+        const PairConst &pc = P.pc[mat_i][info_mat(fj)];
+        double xj[D], dw[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) xj[d] = A.x[d][j];
+        gradw64<D>(dw, xi, xj, pc);
+        double prod = 0;
+#pragma unroll
+        for (int d = 0; d < D; d++) prod += ((img ? 0.0 : A.nn[d][j]) - nni[d]) * dw[d];
+        const double Nj = img ? 0.0 : A.Nf[j];
+        div += (Ni < Nj ? Ni : Nj) * volj * prod;
+    }
+    A.kappa[i] = div;
+}
+
+// dt rule of MSolution::Solution (:275-277); resets the min key
+__global__ void k_advance(Params P, Ctl *ctl)
+{
+    if (CTL_STOP(ctl)) return;
+    const double critts = dunkey(ctl->critts_key);
+    ctl->critts = critts;
+    ctl->critts_key = dkey(1.0e20);
+    ctl->dtold = ctl->dt;
+    const double a = P.tssf * critts, b = 1.1 * ctl->dtold;
+    ctl->dt = a < b ? a : b;
+    ctl->ptime += ctl->dt;
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 2: MSolution::MorrisAcceleration (:550-600); visc_type 2 adds the
+// Monaghan (1992) artificial viscosity, Pi_ij = -alpha*cbar*mu_ij/rhobar for
+// approaching pairs (extension; the reference's visc_type field is unused).
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const uint32_t fi = A.info[i];
+    double ai[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) ai[d] = 0.0;
+    if (fi & kActive) {
+        const int mat_i = info_mat(fi);
+        const Mat &mi = P.mat[mat_i];
+        const bool rig_i = fi & kRigid;
+        const double nu_i = mi.nu, rho_i = A.rho_new[i], pr_i = A.prho2[i];
+        double xi[D], vi[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            xi[d] = A.x[d][i];
+            vi[d] = A.v[d][i];
+        }
+        const int nn = A.nnbr[i];
+        const size_t stride = (size_t)P.Ncap;
+        for (int k = 0; k < nn; k++) {
+            const int j = A.nbr[(size_t)k * stride + i];
+            const uint32_t fj = A.info[j];
+            const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
+            const PairConst &pc = P.pc[mat_i][mat_j];
+            const double mj = A.m[j], rho_j = A.rho_new[j];
+            double xj[D], vj[D], dw[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                xj[d] = A.x[d][j];
+                vj[d] = A.v[d][j];
+            }
+            gradw64<D>(dw, xi, xj, pc);
+            double rab2 = 0.0, prod = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                rab2 += (xi[d] - xj[d]) * (xi[d] - xj[d]);
+                prod += (xi[d] - xj[d]) * dw[d];
+            }
+            const double sp = -mj * (A.prho2[j] + pr_i);
+#pragma unroll
+            for (int d = 0; d < D; d++) ai[d] = ai[d] + sp * dw[d];
+            if (rig_i && (fj & kRigid)) continue;
+            const double nu_j = P.mat[mat_j].nu;
+            const double tv = mj * (nu_i * rho_i + nu_j * rho_j) * prod / (rho_i * rho_j) / (rab2 + pc.eps2);
+#pragma unroll
+            for (int d = 0; d < D; d++) ai[d] = ai[d] + tv * (vi[d] - vj[d]);
+            if (mi.visc_type == 2) {
+                double vr = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; d++) vr += (vi[d] - vj[d]) * (xi[d] - xj[d]);
+                if (vr < 0.0) {
+                    const Mat &mj_ = P.mat[mat_j];
+                    const double alpha = 0.5 * (mi.alpha + mj_.alpha);
+                    const double mu = pc.havg * vr / (rab2 + pc.eps2);
+                    const double pij = -alpha * (0.5 * (mi.c + mj_.c)) * mu / (0.5 * (rho_i + rho_j));
+                    const double sv = -mj * pij;
+#pragma unroll
+                    for (int d = 0; d < D; d++) ai[d] = ai[d] + sv * dw[d];
+                }
+            }
+        }
+        if (P.st) {
+            const double f = -mi.sigma / rho_i * A.kappa[i];
+#pragma unroll
+            for (int d = 0; d < D; d++) ai[d] = ai[d] + f * A.nrm[d][i];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
+}
+
+// ---------------------------------------------------------------------------
+// Rigid body: MSolution::RigidBodyAcceleration (:1496-1564) with
+// MRigidBody::CalculateCenterOfMass (MRigidBody.cpp:37-49); one block,
+// wave-shuffle FP64 reductions in a fixed order (deterministic).
+// ---------------------------------------------------------------------------
+constexpr int kRigidBlock = 1024;
+
+__device__ __forceinline__ double block_sum_big(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kRigidBlock / 64; k++) s += sh[k];
+    return s;   // valid in thread 0
+}
+
+__global__ __launch_bounds__(kRigidBlock) void k_rigid_reduce(Params P, Arrays A, int kin)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    __shared__ double sh[kRigidBlock / 64];
+    __shared__ double R[3];
+    const int nr = P.n_rigid;
+    double px[3] = {0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < nr; q += kRigidBlock) {
+        const int k = A.rigid_idx[q];
+        for (int d = 0; d < P.ndim; d++) px[d] += A.x[d][k];
+    }
+    for (int d = 0; d < 3; d++) {
+        double s = block_sum_big(px[d], sh);
+        if (threadIdx.x == 0) R[d] = s * (1. / nr);
+    }
+    __syncthreads();
+    double pa[3] = {0.0, 0.0, 0.0}, pal[3] = {0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < nr; q += kRigidBlock) {
+        const int k = A.rigid_idx[q];
+        if (!(A.info[k] & kActive)) continue;
+        double ak[3] = {0.0, 0.0, 0.0}, rl[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < P.ndim; d++) {
+            ak[d] = A.a[d][k];
+            rl[d] = A.x[d][k] - R[d];
+        }
+        const double mk = A.m[k];
+        for (int d = 0; d < 3; d++) pa[d] += mk * (0.0 + ak[d]);
+        pal[0] += mk * (rl[1] * ak[2] - rl[2] * ak[1]);
+        pal[1] += mk * (rl[2] * ak[0] - rl[0] * ak[2]);
+        pal[2] += mk * (rl[0] * ak[1] - rl[1] * ak[0]);
+    }
+    double sa[3], sal[3];
+    for (int d = 0; d < 3; d++) {
+        sa[d] = block_sum_big(pa[d], sh);
+        sal[d] = block_sum_big(pal[d], sh);
+    }
+    if (threadIdx.x == 0) {
+        const double dtn = 0.5 * (ctl->dt + ctl->dtold);
+        for (int d = 0; d < 3; d++) {
+            ctl->rbR[d] = R[d];
+            double Ad = sa[d] * (1. / P.rb_M);
+            ctl->rbAL[d] = sal[d] * (1. / P.rb_I);
+            if (P.baseaccel) Ad += P.base_a[d];
+            ctl->rbA[d] = Ad;
+            if (kin) {
+                ctl->rbV[d] = ctl->rbV[d] + ctl->rbA[d] * dtn;
+                ctl->rbW[d] = ctl->rbW[d] + ctl->rbAL[d] * dtn;
+            }
+        }
+    }
+}
+
+// first half of MSolution::RigidBodyVelocity (:1567-1577), unfused path
+__global__ void k_rigid_kin(Ctl *ctl)
+{
+    if (CTL_STOP(ctl)) return;
+    const double dtn = 0.5 * (ctl->dt + ctl->dtold);
+    for (int d = 0; d < 3; d++) {
+        ctl->rbV[d] = ctl->rbV[d] + ctl->rbA[d] * dtn;
+        ctl->rbW[d] = ctl->rbW[d] + ctl->rbAL[d] * dtn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-particle integration: MSolution::BoundCond (:700-726), UpdateVelocity
+// (:602-637) + the particle half of RigidBodyVelocity (:1578-1588),
+// MoveParticles (:769-809) with the periodic wrap and the bounding box.
+// dispbc 8 (extension) keeps its position.
+// ---------------------------------------------------------------------------
+enum { I_BC = 1, I_VEL = 2, I_MOVE = 4, I_RHO = 8 };
+
+__device__ __forceinline__ void constrain(double *q, int code, int D)
+{
+    switch (code) {
+    case 1: q[0] = 0.0; break;
+    case 2: if (D > 1) q[1] = 0.0; break;
+    case 3: if (D > 2) q[2] = 0.0; break;
+    case 4: q[0] = 0.0; if (D > 1) q[1] = 0.0; break;
+    case 5: if (D > 1) q[1] = 0.0; if (D > 2) q[2] = 0.0; break;
+    case 6: if (D > 2) q[2] = 0.0; q[0] = 0.0; break;
+    case 7: case 8: for (int d = 0; d < D; d++) q[d] = 0.0; break;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_integrate(Params P, Arrays A, int mask)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        lo[d] = 1.0e20;
+        hi[d] = -1.0e20;
+    }
+    if (i < P.N) {
+        const uint32_t fi = A.info[i];
+        const bool act = fi & kActive, rig = fi & kRigid;
+        const int disp = info_disp(fi);
+        if ((mask & I_BC) && act) {
+            double a[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) a[d] = A.a[d][i];
+            if (P.baseaccel && !rig)
+#pragma unroll
+                for (int d = 0; d < D; d++) a[d] += P.base_a[d];
+            constrain(a, disp, D);
+#pragma unroll
+            for (int d = 0; d < D; d++) A.a[d][i] = a[d];
+        }
+        double x[D], v[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            x[d] = A.x[d][i];
+            v[d] = A.v[d][i];
+        }
+        if ((mask & I_VEL) && act) {
+            if (!rig) {
+                const double dtn = 0.5 * (ctl->dt + ctl->dtold);
+#pragma unroll
+                for (int d = 0; d < D; d++) v[d] = v[d] + dtn * A.a[d][i];
+            } else {
+                double rl[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int d = 0; d < D; d++) rl[d] = x[d] - ctl->rbR[d];
+                const double *V = ctl->rbV, *W = ctl->rbW;
+                double vr[3];
+                vr[0] = V[0] + W[1] * rl[2] - W[2] * rl[1];
+                vr[1] = V[1] + W[2] * rl[0] - W[0] * rl[2];
+                vr[2] = V[2] + W[0] * rl[1] - W[1] * rl[0];
+#pragma unroll
+                for (int d = 0; d < D; d++) v[d] = vr[d];
+            }
+#pragma unroll
+            for (int d = 0; d < D; d++) A.v[d][i] = v[d];
+        }
+        if (mask & I_MOVE) {
+            if (act && disp != 8) {
+                const double dt = ctl->dt;
+#pragma unroll
+                for (int d = 0; d < D; d++) x[d] += v[d] * dt;
+            }
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                if (P.bcode[1][d] == 2 && x[d] >= P.bx[1][d]) {
+                    const double over = x[d] - P.bx[1][d];
+                    x[d] = P.bx[0][d] + over;
+                }
+                if (P.bcode[0][d] == 2 && x[d] <= P.bx[0][d]) {
+                    const double over = P.bx[0][d] - x[d];
+                    x[d] = P.bx[0][d] - over;
+                }
+                A.x[d][i] = x[d];
+                lo[d] = x[d];
+                hi[d] = x[d];
+            }
+        }
+        if (mask & I_RHO) A.rho[i] = A.rho_new[i];
+    }
+    if (mask & I_MOVE) {
+        __shared__ double red[2][D][kBlock / 64];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double l = wave_min(lo[d]), h = wave_max(hi[d]);
+            if (lane == 0) {
+                red[0][d][w] = l;
+                red[1][d][w] = h;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                double l = red[0][d][0], h = red[1][d][0];
+                for (int k = 1; k < kBlock / 64; k++) {
+                    l = fmin(l, red[0][d][k]);
+                    h = fmax(h, red[1][d][k]);
+                }
+                atomicMin(&ctl->bbox_key[0][d], dkey(l));
+                atomicMax(&ctl->bbox_key[1][d], dkey(h));
+            }
+        }
+    }
+}
+
+// bounding box of MoveParticles into coord_minmax; with end != 0 also the
+// step counter and stop test of MSolution::Solution (:317-319)
+__global__ void k_finish(Params P, Ctl *ctl, int bbox, int end)
+{
+    if (CTL_STOP(ctl)) return;
+    if (bbox) {
+        for (int d = 0; d < 3; d++) {
+            ctl->cmm[0][d] = d < P.ndim ? dunkey(ctl->bbox_key[0][d]) : 1.0e20;
+            ctl->cmm[1][d] = d < P.ndim ? dunkey(ctl->bbox_key[1][d]) : -1.0e20;
+            ctl->bbox_key[0][d] = dkey(1.0e20);
+            ctl->bbox_key[1][d] = dkey(-1.0e20);
+        }
+    }
+    if (end) {
+        ctl->timestep++;
+        if (ctl->ptime - ctl->dt >= P.endtime) ctl->done = 1;
+    }
+}
+
+// canonical neighbour ids (tests and the sph_get_neighbours entry point)
+__global__ void k_canon(Params P, Arrays A, long long *out, int maxnbr)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const int nn = A.nnbr[i];
+    for (int k = 0; k < nn && k < maxnbr; k++) {
+        const int j = A.nbr[(size_t)k * P.Ncap + i];
+        long long cid;
+        if (j < P.N) cid = j;
+        else {
+            const int g = j - P.N;
+            cid = -(1 + (long long)A.gsrc[g] * 27 + A.gshift[g]);
+        }
+        out[(size_t)i * maxnbr + k] = cid;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
+
+#define DISPATCH_D(D_, CALL)                                   \
+    switch (D_) {                                              \
+    case 1: { constexpr int DD = 1; CALL; } break;             \
+    case 2: { constexpr int DD = 2; CALL; } break;             \
+    default: { constexpr int DD = 3; CALL; } break;            \
+    }
+
+void launch_scan(const Params &P, const Arrays &A, const int *in, int *out, const int *n_ptr,
+                 int cap, hipStream_t st)
+{
+    const int nb = (cap + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, st, in, n_ptr, A.scan_aux, A.ctl);
+    hipLaunchKernelGGL(k_scan_aux, dim3(1), dim3(kBlock), 0, st, A.scan_aux, nb, n_ptr, out, A.ctl);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, st, in, out, n_ptr, A.scan_aux, A.ctl);
+}
+
+void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1), 0, st, P, A.ctl);
+    hipLaunchKernelGGL(k_zero_cells, dim3(nblk(P.ncell_cap + 1) < 1024 ? nblk(P.ncell_cap + 1) : 1024),
+                       dim3(kBlock), 0, st, P, A);
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_bin_count<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
+    launch_scan(P, A, A.cell_count, A.cell_start, &A.ctl->ncell, P.ncell_cap, st);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A);
+    hipLaunchKernelGGL(k_bin_rank, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A);
+}
+
+void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reset_ngp, dim3(1), dim3(1), 0, st, A.ctl);
+    if (!P.periodic) return;
+    for (int m = 0; m < 3; m++)
+        for (int side = 0; side < 2; side++) {
+            const int code = (m == 0) ? P.bcode[0][0] : P.bcode[side][m];
+            if (code != 2) continue;
+            hipLaunchKernelGGL(k_ghost_count, dim3(256), dim3(kBlock), 0, st, P, A, m, side);
+            hipLaunchKernelGGL(k_ghost_scan, dim3(1), dim3(kBlock), 0, st, P, A, m, side);
+            hipLaunchKernelGGL(k_ghost_emit, dim3(256), dim3(kBlock), 0, st, P, A, m, side);
+        }
+}
+
+void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st)
+{
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, shift));
+}
+
+void launch_shift(const Params &P, const Arrays &A, hipStream_t st)
+{
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_shift<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
+}
+
+void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st)
+{
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_strain<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, mask));
+}
+
+void launch_normals(const Params &P, const Arrays &A, hipStream_t st)
+{
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_normals<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_curvature<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
+}
+
+void launch_advance(const Params &P, const Arrays &A, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, P, A.ctl);
+}
+
+void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st)
+{
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_momentum<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
+    if (P.rigid)
+        hipLaunchKernelGGL(k_rigid_reduce, dim3(1), dim3(kRigidBlock), 0, st, P, A, rigid_kin);
+}
+
+void launch_rigid_kin(const Params &P, const Arrays &A, hipStream_t st)
+{
+    if (P.rigid) hipLaunchKernelGGL(k_rigid_kin, dim3(1), dim3(1), 0, st, A.ctl);
+}
+
+void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st)
+{
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_integrate<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, mask));
+}
+
+void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, st, P, A.ctl, bbox, end);
+}
+
+void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_canon, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, out, maxnbr);
+}
+
+}  // namespace sph

@@ -1,0 +1,26 @@
+// sph_kernels.hpp -- launchers of the step-loop kernels (sph_kernels.hip).
+#pragma once
+
+#include "sph_state.hpp"
+
+namespace sph {
+
+// masks of launch_strain
+enum { M_STRAIN = 1, M_CONST = 2, M_CRIT = 4, M_INPLACE = 8 };
+// masks of launch_integrate
+enum { M_BC = 1, M_VEL = 2, M_MOVE = 4, M_RHO = 8 };
+
+void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st);
+void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st);
+void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st);
+void launch_shift(const Params &P, const Arrays &A, hipStream_t st);
+void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st);
+void launch_normals(const Params &P, const Arrays &A, hipStream_t st);
+void launch_advance(const Params &P, const Arrays &A, hipStream_t st);
+void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st);
+void launch_rigid_kin(const Params &P, const Arrays &A, hipStream_t st);
+void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st);
+void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st);
+void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st);
+
+}  // namespace sph

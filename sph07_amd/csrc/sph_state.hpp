@@ -1,0 +1,119 @@
+// sph_state.hpp -- device-resident state of one scene and the by-value
+// kernel parameter blocks shared by the HIP kernels and the host driver.
+//
+// Layout in HBM (structure of arrays, one array per field, FP64):
+//   [0, N)          real particles, deck order (storage index == deck index)
+//   [N, N + Gcap)   periodic images ("ghosts"), rebuilt every step; image g
+//                   (1-based creation number, MGhostParticleData.cpp:30-31)
+//                   lives at N + g - 1.
+// Neighbour lists are neighbour-major, nbr[k * Ncap + i], so that entry k of
+// 64 consecutive particles is one coalesced 256-byte row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+constexpr int kMaxMat = 8;
+constexpr int kBlock = 256;
+
+// packed per-particle flags (info[])
+constexpr uint32_t kMatMask = 0xFFu;         // bits 0-7 material (1-based)
+constexpr int kDispShift = 8;                // bits 8-11 dispbc
+constexpr uint32_t kActive = 1u << 12;       // MCommonParticle::ToCompute()
+constexpr uint32_t kRigid = 1u << 13;        // MSolution::InRigidBody
+constexpr uint32_t kGhost = 1u << 14;        // periodic image
+constexpr int kColorShift = 16;              // bits 16-31 colour (int16)
+
+__host__ __device__ inline int info_mat(uint32_t f) { return (int)(f & kMatMask); }
+__host__ __device__ inline int info_disp(uint32_t f) { return (int)((f >> kDispShift) & 0xFu); }
+__host__ __device__ inline int info_color(uint32_t f) { return (int)(int16_t)(f >> kColorShift); }
+
+// error codes mirrored from include/sph07_amd.h
+constexpr int kErrNbrOverflow = 2;
+constexpr int kErrOutOfGrid = 3;
+constexpr int kErrGridCap = 16;   // grid larger than the cell arrays (host grows, retries)
+constexpr int kErrGhostCap = 17;  // more images than the image arrays (host grows, retries)
+
+// Device control block: every scalar the step loop mutates.
+struct Ctl {
+    double dt, dtold, ptime, critts;
+    double gmin[3], gsize[3];
+    double cmm[2][3];
+    double rbR[3], rbV[3], rbW[3], rbA[3], rbAL[3];
+    unsigned long long critts_key;
+    unsigned long long bbox_key[2][3];
+    int glim[3];
+    int ncell;
+    int timestep, done, err, ngp;
+    int err_particle;
+    int pad_[3];
+};
+
+// Per-material constants (MMaterial.h:17-40 after CheckDefaults).
+struct Mat {
+    double rho0, nu, sigma, c, eos_b, eos_gamma, alpha, h;
+    int eos, visc_type;
+};
+
+// Per material-pair kernel constants: havg = AVG(h_i,h_j) and the B-spline
+// normalisations of MKernelBSpline.cpp:145-156, evaluated once with the
+// reference's operation order (bit-identical to recomputing them per pair).
+struct PairConst {
+    double havg, const1, const2, eps2, cut2;   // eps2 = 0.01*SQR(havg), cut2 = SQR(2*havg)
+};
+
+struct Params {
+    int N, Ncap;          // real particles, stride of per-particle arrays (N + Gcap)
+    int gcap;             // image capacity
+    int ndim, tcrit_opt, maxnbr;
+    int st, rigid, baseaccel, periodic;
+    int nmat, rb_id, n_rigid;
+    int ncell_cap;
+    int want_mindist;
+    int btype[3];
+    int bcode[2][3];
+    double bx[2][3];
+    double base_a[3];
+    double minadd[3][3], maxadd[3][3];
+    double hmax, tssf, endtime;
+    double rb_M, rb_I;
+    Mat mat[kMaxMat + 1];                      // 1-based
+    PairConst pc[kMaxMat + 1][kMaxMat + 1];    // 1-based
+};
+
+struct Arrays {
+    double *x[3], *xs[3], *v[3], *a[3];
+    double *rho, *rho_new, *p, *prho2, *vol, *m;
+    double *trd, *mindist, *critts_p, *rhoold;
+    double *nrm[3], *nn[3], *Nf, *kappa;
+    uint32_t *info;
+    int *nnbr, *nbr;
+    // binning of real particles
+    int *cell_of, *slot, *cell_count, *cell_start, *tmp, *entries;
+    // periodic images: per target cell the first image index and the count
+    // (the images of one cell are consecutive, created from one source cell)
+    int *gcount, *gfirst, *gsrc, *gshift;
+    int *slab_cnt, *slab_off;
+    int *scan_aux;
+    int *rigid_idx;
+    Ctl *ctl;
+};
+
+// Ordered 64-bit key of a double: unsigned order == numeric order.
+__host__ __device__ inline unsigned long long dkey(double d)
+{
+    unsigned long long b;
+    __builtin_memcpy(&b, &d, 8);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double dunkey(unsigned long long k)
+{
+    unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    double d;
+    __builtin_memcpy(&d, &b, 8);
+    return d;
+}
+
+}  // namespace sph

@@ -1,0 +1,205 @@
+"""ctypes binding of the C ABI in ``include/sph07_amd.h``.
+
+``Scene`` mirrors the reference's MSolution surface (MSolution.h:25-65):
+one method per phase, the same order, the same OK/ERROR convention (raised
+as :class:`SphError` here).  The library is the HIP build in this package;
+there is no CPU fallback -- if it is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from .deck import CDeck, CMaterial, Deck
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsph07_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "sph07_amd.h")
+
+ARM_FP64 = 0
+
+F_X, F_V, F_A, F_RHO, F_P, F_C, F_MASS, F_H, F_TRD, F_MINDIST, F_NORMAL, F_NNORMAL, \
+    F_N, F_KAPPA, F_CRITTS, F_RHOOLD, F_ACTIVE, F_VABS = range(18)
+_VEC = {F_X, F_V, F_A, F_NORMAL, F_NNORMAL}
+
+
+class SphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class CGlobals(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("dtold", ctypes.c_double), ("ptime", ctypes.c_double),
+        ("critts", ctypes.c_double), ("coord_minmax", (ctypes.c_double * 3) * 2),
+        ("hmax", ctypes.c_double),
+        ("rb_R", ctypes.c_double * 3), ("rb_V", ctypes.c_double * 3),
+        ("rb_OMEGA", ctypes.c_double * 3), ("rb_A", ctypes.c_double * 3),
+        ("rb_ALPHA", ctypes.c_double * 3),
+        ("timestep", ctypes.c_int32), ("ngp", ctypes.c_int32), ("done", ctypes.c_int32),
+        ("error", ctypes.c_int32), ("np", ctypes.c_int32), ("grid_lim", ctypes.c_int32 * 3),
+    ]
+
+
+_lib = None
+
+
+def header_symbols():
+    """Names of the functions declared in include/sph07_amd.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s+\**(sph_\w+)\s*\(", txt, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                          "(python -m sph07_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    L.sph_create.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ip]
+    L.sph_destroy.argtypes = [vp]
+    L.sph_destroy.restype = None
+    L.sph_last_error.argtypes = [vp]
+    L.sph_last_error.restype = ctypes.c_char_p
+    for name in ("sph_neighbours", "sph_ghost_setup", "sph_linked_list_neighbours", "sph_strain",
+                 "sph_calculate_surface_normals", "sph_constitutive", "sph_time_advance",
+                 "sph_momentum", "sph_boundcond", "sph_update_velocity", "sph_move_particles",
+                 "sph_end_step", "sph_synchronize"):
+        getattr(L, name).argtypes = [vp]
+    L.sph_calc_critical_timestep.argtypes = [vp, dp]
+    L.sph_solution.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int)]
+    L.sph_get_globals.argtypes = [vp, ctypes.POINTER(CGlobals)]
+    L.sph_download.argtypes = [vp, ip, ctypes.c_void_p]
+    L.sph_get_neighbours.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ip]
+    assert L.sph_desc_size() == ctypes.sizeof(CDeck), "sph_scene_desc layout mismatch"
+    assert L.sph_material_desc_size() == ctypes.sizeof(CMaterial), "material layout mismatch"
+    _lib = L
+    return L
+
+
+class Scene:
+    """One device-resident scene (MSimulationInit::Initialize on creation)."""
+
+    def __init__(self, deck: Deck, arm: int = ARM_FP64):
+        L = lib()
+        c, self._keep = deck.to_c()
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = L.sph_create(ctypes.byref(c), arm, ctypes.byref(h), err, 512)
+        if rc != 0:
+            raise SphError(rc, err.value.decode())
+        self._h = h
+        self.np = deck.np
+        self.ndim = deck.ndim
+        self.maxnbr = deck.maxnbr
+        self.arm = arm
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SphError(rc, lib().sph_last_error(self._h).decode())
+
+    # --- MSolution phase surface ---
+    def Neighbours(self):
+        self._chk(lib().sph_neighbours(self._h))
+
+    def GhostSetup(self):
+        self._chk(lib().sph_ghost_setup(self._h))
+
+    def LinkedListNeighbours(self):
+        self._chk(lib().sph_linked_list_neighbours(self._h))
+
+    def Strain(self):
+        self._chk(lib().sph_strain(self._h))
+
+    def CalculateSurfaceNormals(self):
+        self._chk(lib().sph_calculate_surface_normals(self._h))
+
+    def Constitutive(self):
+        self._chk(lib().sph_constitutive(self._h))
+
+    def CalcCriticalTimestep(self) -> float:
+        out = ctypes.c_double(0.0)
+        self._chk(lib().sph_calc_critical_timestep(self._h, ctypes.byref(out)))
+        return out.value
+
+    def TimeAdvance(self):
+        self._chk(lib().sph_time_advance(self._h))
+
+    def Momentum(self):
+        self._chk(lib().sph_momentum(self._h))
+
+    def BoundCond(self):
+        self._chk(lib().sph_boundcond(self._h))
+
+    def UpdateVelocity(self):
+        self._chk(lib().sph_update_velocity(self._h))
+
+    def MoveParticles(self):
+        self._chk(lib().sph_move_particles(self._h))
+
+    def EndStep(self):
+        self._chk(lib().sph_end_step(self._h))
+
+    def phase(self, k: int):
+        """Phase k of one iteration, numbered as the oracle's phases."""
+        [lambda: (self.Neighbours(), self.GhostSetup(), self.LinkedListNeighbours()),
+         self.Strain, self.CalculateSurfaceNormals, self.Constitutive,
+         lambda: (self.CalcCriticalTimestep(), self.TimeAdvance()),
+         self.Momentum, self.BoundCond, self.UpdateVelocity,
+         lambda: (self.MoveParticles(), self.EndStep())][k]()
+
+    def Solution(self, nsteps: int) -> int:
+        done = ctypes.c_int(0)
+        self._chk(lib().sph_solution(self._h, nsteps, ctypes.byref(done)))
+        return done.value
+
+    def synchronize(self):
+        self._chk(lib().sph_synchronize(self._h))
+
+    # --- state access ---
+    def globals(self) -> dict:
+        g = CGlobals()
+        self._chk(lib().sph_get_globals(self._h, ctypes.byref(g)))
+        out = {k: getattr(g, k) for k in ("dt", "dtold", "ptime", "critts", "hmax", "timestep",
+                                          "ngp", "done", "error", "np")}
+        for s, nm in ((0, "cmm_min"), (1, "cmm_max")):
+            for m in range(3):
+                out[f"{nm}{m}"] = g.coord_minmax[s][m]
+        for nm, arr in (("rbR", g.rb_R), ("rbV", g.rb_V), ("rbW", g.rb_OMEGA),
+                        ("rbA", g.rb_A), ("rbAL", g.rb_ALPHA)):
+            for m in range(3):
+                out[f"{nm}{m}"] = arr[m]
+        out["grid_lim"] = tuple(g.grid_lim)
+        return out
+
+    def field(self, f: int) -> np.ndarray:
+        out = np.zeros((self.np, 3) if f in _VEC else self.np)
+        self._chk(lib().sph_download(self._h, f, out.ctypes.data))
+        return out
+
+    def neighbours(self):
+        nn = np.zeros(self.np, np.int32)
+        canon = np.zeros((self.np, self.maxnbr), np.int64)
+        self._chk(lib().sph_get_neighbours(self._h, nn.ctypes.data, canon.ctypes.data, self.maxnbr))
+        return nn, canon

@@ -1,0 +1,138 @@
+"""Parity of the HIP FP64 arm against the CPU FP64 oracle, through the C ABI.
+
+Scenes are small versions of every workload (periodic channel, 3D dam break
+with the Monaghan term, rigid cylinder, lid-driven cavity, CSF droplet).
+Bars (north-star): neighbour sets identical as sets -- and here also in
+order; density, acceleration, velocity and position within 1e-10 relative
+after N steps.  Scenes without a rigid body are expected bit-identical,
+because the device sums each particle's neighbours in the reference order
+with the reference's operation order and no FMA; the rigid-body reduction
+uses a fixed tree order, so those scenes are held to the 1e-10 bar only.
+"""
+import numpy as np
+import pytest
+
+from sph07_amd import scenes
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def small_scenes():
+    return {
+        "poiseuille": lambda: scenes.poiseuille(nx=40, ny=40),
+        "dam_break": lambda: scenes.dam_break(nx_fluid=10, ny=8, nz_fluid=10, nx_tank=20,
+                                              nz_tank=14, maxnbr=100),
+        "cylinder": lambda: scenes.cylinder(nx=60, ny=100, dx=2e-4, radius=2e-3,
+                                            centre=(0.006, 0.014)),
+        "cavity": lambda: scenes.cavity(n=30),
+        "droplet": lambda: scenes.droplet(n_box=40),
+    }
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+def gpu_scene(deck):
+    from sph07_amd.native import Scene
+    return Scene(deck)
+
+
+FIELDS = [("x", orc.F_X), ("v", orc.F_V), ("a", orc.F_A), ("rho", orc.F_RHO), ("p", orc.F_P)]
+
+
+def compare_state(g, o, rtol, exact, what):
+    for name, f in FIELDS:
+        a, b = g.field(f), o.field(f)
+        if exact:
+            assert np.array_equal(a, b), f"{what}: {name} differs (rel {rel(a, b):.3e})"
+        else:
+            assert rel(a, b) <= rtol, f"{what}: {name} rel {rel(a, b):.3e}"
+
+
+@pytest.mark.parametrize("name", list(small_scenes()))
+def test_initial_state(name):
+    deck = small_scenes()[name]()
+    o = orc.Oracle(deck)
+    g = gpu_scene(deck)
+    go, gg = o.globals(), g.globals()
+    assert gg["dt"] == go["dt"], (gg["dt"], go["dt"])
+    nn_o, c_o = o.neighbours()
+    nn_g, c_g = g.neighbours()
+    assert np.array_equal(nn_o, nn_g)
+    for i in range(deck.np):
+        assert np.array_equal(c_o[i, :nn_o[i]], c_g[i, :nn_g[i]]), f"particle {i} list differs"
+
+
+@pytest.mark.parametrize("name", list(small_scenes()))
+def test_phases_step1(name):
+    """Every MSolution phase of the first two iterations, one at a time."""
+    deck = small_scenes()[name]()
+    exact = not deck.rigid_body
+    o = orc.Oracle(deck)
+    g = gpu_scene(deck)
+    checks = {
+        orc.PH_NEIGHBOURS: [],
+        orc.PH_STRAIN: [("trd", orc.F_TRD), ("x", orc.F_X)],
+        orc.PH_NORMALS: [("kappa", orc.F_KAPPA), ("normal", orc.F_NORMAL)] if deck.surface_tension else [],
+        orc.PH_CONSTITUTIVE: [("rho", orc.F_RHO), ("p", orc.F_P)],
+        orc.PH_TIMESTEP: [],
+        orc.PH_MOMENTUM: [("a", orc.F_A)],
+        orc.PH_BOUNDCOND: [("a", orc.F_A)],
+        orc.PH_VELOCITY: [("v", orc.F_V)],
+        orc.PH_MOVE: [("x", orc.F_X)],
+    }
+    for step in range(2):
+        for ph in range(orc.NPHASES):
+            o.phase(ph)
+            g.phase(ph)
+            if ph == orc.PH_NEIGHBOURS:
+                nn_o, c_o = o.neighbours()
+                nn_g, c_g = g.neighbours()
+                assert np.array_equal(nn_o, nn_g), f"step {step}: neighbour counts differ"
+                for i in range(deck.np):
+                    assert np.array_equal(c_o[i, :nn_o[i]], c_g[i, :nn_g[i]])
+            for fname, f in checks[ph]:
+                a, b = g.field(f), o.field(f)
+                if exact:
+                    assert np.array_equal(a, b), f"step {step} phase {ph} {fname}: rel {rel(a, b):.3e}"
+                else:
+                    assert rel(a, b) <= RTOL, f"step {step} phase {ph} {fname}: rel {rel(a, b):.3e}"
+        go, gg = o.globals(), g.globals()
+        for k in ("dt", "dtold", "ptime", "critts"):
+            assert gg[k] == go[k] or abs(gg[k] - go[k]) <= RTOL * abs(go[k]), (step, k, gg[k], go[k])
+        assert gg["timestep"] == go["timestep"]
+
+
+@pytest.mark.parametrize("name,steps", [("poiseuille", 200), ("dam_break", 60), ("cylinder", 60),
+                                        ("cavity", 200), ("droplet", 100)])
+def test_fused_steps(name, steps):
+    """The fused device-resident loop (sph_solution) against the oracle."""
+    deck = small_scenes()[name]()
+    o = orc.Oracle(deck)
+    g = gpu_scene(deck)
+    n_o = o.run(steps)
+    n_g = g.Solution(steps)
+    assert n_o == n_g == steps
+    compare_state(g, o, RTOL, exact=not deck.rigid_body, what=f"{name} after {steps} steps")
+    go, gg = o.globals(), g.globals()
+    for k in ("dt", "ptime", "cmm_min0", "cmm_max0", "cmm_min1", "cmm_max1"):
+        assert abs(gg[k] - go[k]) <= RTOL * max(abs(go[k]), 1e-300), (k, gg[k], go[k])
+    if deck.rigid_body:
+        for k in ("rbR0", "rbR1", "rbV0", "rbV1", "rbW2"):
+            assert abs(gg[k] - go[k]) <= 1e-9 * max(abs(go[k]), 1e-12), (k, gg[k], go[k])
+
+
+def test_neighbour_overflow_is_reported():
+    """maxnbr too small: the reference's 'Neighbour list exceeded maximum.'"""
+    from sph07_amd.native import SphError
+    deck = scenes.poiseuille(nx=20, ny=20, maxnbr=8)
+    with pytest.raises(SphError) as e:
+        gpu_scene(deck)
+    assert e.value.code == 2 and "exceeded maximum" in str(e.value)
