@@ -1,0 +1,145 @@
+"""Benchmark of the per-timestep particle loop on one MI355X.
+
+Workload: the 3D dam break (tank 1.6 x 0.67 x 0.6 m, water column
+0.4 x 0.67 x 0.3 m, dx = 5e-3 m, h = 1.3 dx, 3 wall layers: 1,111,320
+particles), synthetic lattice input, random-free.  A "step" is one full
+iteration of MSolution::Solution over all particles, device-resident.
+
+Prints one JSON line: particle-updates/s (real particles, walls included,
+advanced one step), the roofline of the dominant kernel measured with HIP
+events on the library's stream, and the CPU oracle timed on a bounded sample
+of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s on 1 GPU vs plain FP64 GPU path; p50/p99 scene latency"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic bytes per particle per launch (each field touched once; the
+# neighbours' re-reads and the list entries are not counted) and algorithmic
+# FP64 flops per neighbour pair, for ndim = 3.
+BYTES_3D = {
+    "k_neighbours": 8 * 3 + 8 + 8 + 4 + 8 * 3 + 4 + 8 * 3 + 8,   # x,m,rho,info,v | nnbr,xs,vol
+    "k_strain": 8 * 3 + 8 * 3 + 8 + 8 + 4 + 4 + 8 * 3 + 8 + 8 * 3 + 8 + 8 + 8,
+    "k_momentum (+k_rigid_reduce)": 8 * 3 + 8 * 3 + 8 + 8 + 8 + 4 + 4 + 8 * 3,
+    "k_integrate": 4 + 8 * 3 + 8 * 3 + 8 * 3 + 8 * 3 + 8 * 3 + 8 + 8,
+}
+FLOPS_PER_PAIR_3D = {"k_strain": 41, "k_momentum (+k_rigid_reduce)": 70 + 12}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="extra steps run with per-kernel HIP-event timing")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--small", action="store_true", help="reduced dam break (for quick checks)")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from sph07_amd import scenes
+    from sph07_amd.native import Scene
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    kw = dict(nx_fluid=20, ny=34, nz_fluid=15, nx_tank=80, nz_tank=30) if args.small else {}
+    deck = scenes.dam_break(**kw)
+    N = deck.np
+    print(f"bench: dam_break {N} particles ({deck.n_fluid} fluid), FP64 arm", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sc = Scene(deck)
+    t_init = time.perf_counter() - t0
+    sc.Solution(args.warmup)
+    sc.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = sc.Solution(args.steps)
+    sc.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert done == args.steps, (done, args.steps)
+    value = N * args.steps / elapsed
+    nn, _ = sc.neighbours()
+    pairs = float(nn.astype(np.int64).sum())
+    print(f"bench: {args.steps} steps in {elapsed:.3f} s -> {value:.4e} particle-updates/s",
+          file=sys.stderr, flush=True)
+
+    # per-kernel timing pass (HIP events on the scene's stream)
+    sc.set_timing(True)
+    sc.Solution(args.profile_steps)
+    tm = sc.timing()
+    sc.set_timing(False)
+    kern = {k: (ms / cnt if cnt else 0.0, cnt) for k, (ms, cnt) in tm.items()}
+    for k, (avg, cnt) in kern.items():
+        if cnt:
+            print(f"bench: {k:40s} {avg * 1e3:9.1f} us x {cnt}", file=sys.stderr, flush=True)
+    dom = max(kern, key=lambda k: kern[k][0])
+    avg_ms = kern[dom][0]
+    bytes_alg = BYTES_3D.get(dom, 0) * N
+    achieved = bytes_alg / (avg_ms * 1e-3) / 1e9 if avg_ms else 0.0
+    traffic = None
+    tf_path = os.path.join(ROOT, "profiles", "traffic_dam_break.json")
+    if os.path.exists(tf_path):
+        traffic = json.load(open(tf_path)).get(dom)
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": bytes_alg}
+    if dom in FLOPS_PER_PAIR_3D and avg_ms:
+        tfl = FLOPS_PER_PAIR_3D[dom] * pairs / (avg_ms * 1e-3) / 1e12
+        roof["valu_fp64"] = {"achieved": round(tfl, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(tfl / FP64_PEAK_TFLOPS, 4),
+                             "flops_per_pair": FLOPS_PER_PAIR_3D[dom]}
+    kernels = {k: round(v[0], 4) for k, v in kern.items() if v[1]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        print(f"bench: cpu baseline: oracle, {args.cpu_steps} steps of the same scene",
+              file=sys.stderr, flush=True)
+        o = orc.Oracle(deck)
+        t0 = time.perf_counter()
+        o.run(args.cpu_steps)
+        tc = time.perf_counter() - t0
+        cpu = {"value": N * args.cpu_steps / tc, "unit": "particle-updates/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} full steps of the same {N}-particle dam break "
+                         f"(after its untimed initialisation), single thread, {tc:.1f} s"}
+        del o
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "dam_break_3d" + ("_small" if args.small else ""), "particles": N,
+                   "fluid_particles": deck.n_fluid, "dx": deck.meta["dx"], "h_over_dx": 1.3,
+                   "arm": "fp64", "monaghan_alpha": 0.1, "parallelism": "single-gpu"},
+        "neighbour_interactions_per_s": pairs * args.steps / elapsed,
+        "init_s": round(t_init, 3),
+        "kernels_avg_ms": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,16 @@ int  sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr);
 /* Blocks until all work queued on the context's stream has finished. */
 int  sph_synchronize(void *ctx);
 
+/* --- measurement (no reference counterpart: the reference only prints total
+   wall seconds, MSimulationData.cpp:110-122) --- */
+/* Turns HIP-event timing of the kernel groups of sph_solution on or off and
+   clears the totals. */
+int  sph_set_timing(void *ctx, int enable);
+/* slot < 0: returns the number of timed groups.  Otherwise fills the group's
+   name, total milliseconds and launch count. */
+int  sph_get_timing(void *ctx, int slot, char *name, int namelen, double *total_ms,
+                    long long *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,22 @@ namespace {
 
 constexpr double kPI = 3.141592654;   // global.h:12
 
+// Kernel groups timed with HIP events on the context's stream.
+enum KId { K_BIN, K_IMAGES, K_NBR, K_STRAIN, K_NORMALS, K_ADVANCE, K_MOMENTUM, K_INTEGRATE,
+           K_FINISH, K_COUNT };
+const char *const kKernelNames[K_COUNT] = {
+    "bin (grid, count, scan, scatter, rank)", "periodic images", "k_neighbours", "k_strain",
+    "k_normals+k_curvature", "k_advance", "k_momentum (+k_rigid_reduce)", "k_integrate", "k_finish"};
+
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    struct Pending { int id; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    double total_ms[K_COUNT] = {};
+    long long count[K_COUNT] = {};
+};
+
 struct Context {
     hipStream_t st = nullptr;
     Params P{};
@@ -33,7 +49,50 @@ struct Context {
     int N = 0, ndim = 0, maxnbr = 0, arm = 0;
     std::vector<void *> part_allocs, cell_allocs;
     long long *canon = nullptr;
+    Timing tm;
 };
+
+hipEvent_t take_event(Context *c)
+{
+    if (!c->tm.pool.empty()) {
+        hipEvent_t e = c->tm.pool.back();
+        c->tm.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+// Called after a stream synchronisation: folds finished event pairs into the
+// per-group totals.
+void harvest_timing(Context *c)
+{
+    for (auto &p : c->tm.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->tm.total_ms[p.id] += ms;
+            c->tm.count[p.id]++;
+        }
+        c->tm.pool.push_back(p.a);
+        c->tm.pool.push_back(p.b);
+    }
+    c->tm.pending.clear();
+}
+
+template <class F>
+inline void timed(Context *c, int id, F &&f)
+{
+    if (!c->tm.on) {
+        f();
+        return;
+    }
+    hipEvent_t a = take_event(c), b = take_event(c);
+    hipEventRecord(a, c->st);
+    f();
+    hipEventRecord(b, c->st);
+    c->tm.pending.push_back({id, a, b});
+}
 
 #define HIPCHK(ctx, expr)                                                          \
     do {                                                                           \
@@ -179,6 +238,7 @@ int sync_ctl(Context *c)
 {
     HIPCHK(c, hipMemcpyAsync(c->hctl, c->A.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    harvest_timing(c);
     return 0;
 }
 
@@ -253,15 +313,16 @@ void fused_step(Context *c)
 {
     const Params &P = c->P;
     const Arrays &A = c->A;
-    launch_setup_linked_list(P, A, c->st);
-    if (P.periodic) launch_ghost_setup(P, A, c->st);
-    launch_neighbours(P, A, 1, c->st);
-    launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, c->st);
-    if (P.st) launch_normals(P, A, c->st);
-    launch_advance(P, A, c->st);
-    launch_momentum(P, A, 1, c->st);
-    launch_integrate(P, A, M_BC | M_VEL | M_MOVE | M_RHO, c->st);
-    launch_finish(P, A, 1, 1, c->st);
+    hipStream_t st = c->st;
+    timed(c, K_BIN, [&] { launch_setup_linked_list(P, A, st); });
+    if (P.periodic) timed(c, K_IMAGES, [&] { launch_ghost_setup(P, A, st); });
+    timed(c, K_NBR, [&] { launch_neighbours(P, A, 1, st); });
+    timed(c, K_STRAIN, [&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st); });
+    if (P.st) timed(c, K_NORMALS, [&] { launch_normals(P, A, st); });
+    timed(c, K_ADVANCE, [&] { launch_advance(P, A, st); });
+    timed(c, K_MOMENTUM, [&] { launch_momentum(P, A, 1, st); });
+    timed(c, K_INTEGRATE, [&] { launch_integrate(P, A, M_BC | M_VEL | M_MOVE | M_RHO, st); });
+    timed(c, K_FINISH, [&] { launch_finish(P, A, 1, 1, st); });
 }
 
 // ---- host part of MSimulationInit::Initialize -------------------------------
@@ -309,6 +370,7 @@ int build_context(Context *c, const sph_scene_desc *d)
         }
         if ((c0 == 2) != (c1 == 2)) { c->err = "Error in definition of periodic boundary conditions."; return SPH_ERROR; }
         if (c0 == 2 && m >= D) { c->err = "periodic axis beyond ndim"; return SPH_ERROR; }
+        if (c0 == 2 && !(P.bx[1][m] > P.bx[0][m])) { c->err = "periodic planes must satisfy xmin < xmax"; return SPH_ERROR; }
         P.btype[m] = 1 + (c0 > 0 ? 1 : 0) + (c1 > 0 ? 2 : 0);
         if (c0 == 2) {
             P.periodic = 1;
@@ -575,6 +637,8 @@ void sph_destroy(void *ctx)
     if (c->A.rigid_idx) hipFree(c->A.rigid_idx);
     if (c->canon) hipFree(c->canon);
     if (c->hctl) hipHostFree(c->hctl);
+    harvest_timing(c);
+    for (hipEvent_t e : c->tm.pool) hipEventDestroy(e);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
 }
@@ -815,6 +879,37 @@ int sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr)
     HIPCHK(c, hipMemcpy(tmp.data(), c->canon, tmp.size() * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < N; i++)
         for (int k = 0; k < nnbr[i] && k < mn; k++) canon[(size_t)i * maxnbr + k] = tmp[(size_t)i * mn + k];
+    return SPH_OK;
+}
+
+}  // extern "C"
+
+// ---- kernel timing (HIP events on the context's stream) --------------------
+extern "C" {
+
+int sph_set_timing(void *ctx, int enable)
+{
+    Context *c = ctx_of(ctx);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    harvest_timing(c);
+    c->tm.on = enable != 0;
+    for (int k = 0; k < K_COUNT; k++) {
+        c->tm.total_ms[k] = 0.0;
+        c->tm.count[k] = 0;
+    }
+    return SPH_OK;
+}
+
+int sph_get_timing(void *ctx, int slot, char *name, int namelen, double *total_ms, long long *count)
+{
+    Context *c = ctx_of(ctx);
+    if (slot < 0) return K_COUNT;
+    if (slot >= K_COUNT) return SPH_ERROR;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    harvest_timing(c);
+    if (name && namelen > 0) snprintf(name, (size_t)namelen, "%s", kKernelNames[slot]);
+    if (total_ms) *total_ms = c->tm.total_ms[slot];
+    if (count) *count = c->tm.count[slot];
     return SPH_OK;
 }
 

@@ -78,6 +78,8 @@ def lib():
     L.sph_get_globals.argtypes = [vp, ctypes.POINTER(CGlobals)]
     L.sph_download.argtypes = [vp, ip, ctypes.c_void_p]
     L.sph_get_neighbours.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ip]
+    L.sph_set_timing.argtypes = [vp, ip]
+    L.sph_get_timing.argtypes = [vp, ip, ctypes.c_char_p, ip, dp, ctypes.POINTER(ctypes.c_longlong)]
     assert L.sph_desc_size() == ctypes.sizeof(CDeck), "sph_scene_desc layout mismatch"
     assert L.sph_material_desc_size() == ctypes.sizeof(CMaterial), "material layout mismatch"
     _lib = L
@@ -196,6 +198,21 @@ class Scene:
     def field(self, f: int) -> np.ndarray:
         out = np.zeros((self.np, 3) if f in _VEC else self.np)
         self._chk(lib().sph_download(self._h, f, out.ctypes.data))
+        return out
+
+    def set_timing(self, on: bool):
+        self._chk(lib().sph_set_timing(self._h, 1 if on else 0))
+
+    def timing(self) -> dict:
+        """{group name: (total ms, launches)} from HIP events on the scene's stream."""
+        L = lib()
+        out = {}
+        for k in range(L.sph_get_timing(self._h, -1, None, 0, None, None)):
+            name = ctypes.create_string_buffer(64)
+            ms = ctypes.c_double(0.0)
+            cnt = ctypes.c_longlong(0)
+            self._chk(L.sph_get_timing(self._h, k, name, 64, ctypes.byref(ms), ctypes.byref(cnt)))
+            out[name.value.decode()] = (ms.value, cnt.value)
         return out
 
     def neighbours(self):

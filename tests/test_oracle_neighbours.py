@@ -1,0 +1,155 @@
+"""Pins for the oracle's neighbour search (MSolution.cpp:116-215, 811-918).
+
+  * sets: brute-force O(N^2) search, with explicit periodic images for an
+    x-periodic scene -- the linked-list search must return exactly those sets;
+  * lattice counts: 20 neighbours in 2D at h = 1.2 dx, 80 in 3D at h = 1.3 dx;
+  * symmetry j in N(i) <=> i in N(j);
+  * order: cells ascending in (x, y, z) with z fastest, inside a cell images
+    first then real particles by descending number (the head-insertion order
+    of the reference's linked lists);
+  * the maxnbr overflow error of the reference.
+"""
+import numpy as np
+import pytest
+
+from sph07_amd import scenes
+from sph07_amd.deck import Deck, Material
+from oracle import oracle as orc
+
+
+def brute_force(x, h, ndim, period=None):
+    """Neighbour sets as canonical ids; period = (axis, lo, hi) adds images."""
+    n = x.shape[0]
+    cut2 = (2.0 * h) ** 2
+    sets = [set() for _ in range(n)]
+    shifts = [0]
+    if period is not None:
+        shifts = [-1, 0, 1]
+    for s in shifts:
+        xs = x.copy()
+        if s:
+            ax, lo, hi = period
+            xs[:, ax] += s * (hi - lo)
+        d2 = ((x[:, None, :ndim] - xs[None, :, :ndim]) ** 2).sum(-1)
+        ii, jj = np.nonzero(d2 < cut2)
+        for i, j in zip(ii, jj):
+            if s == 0:
+                if i != j:
+                    sets[i].add(int(j))
+            else:
+                sh = [0, 0, 0]
+                sh[period[0]] = s
+                sets[i].add(-(1 + int(j) * 27 + (sh[0] + 1) * 9 + (sh[1] + 1) * 3 + (sh[2] + 1)))
+    return sets
+
+
+def oracle_sets(o):
+    nn, canon = o.neighbours()
+    return [set(canon[i, :nn[i]].tolist()) for i in range(len(nn))], nn, canon
+
+
+def jittered_box(n=14, dx=1.0, jitter=0.3, seed=0):
+    rng = np.random.default_rng(seed)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x = np.zeros((n * n, 3))
+    x[:, 0] = (ii.ravel() + 0.5) * dx + rng.uniform(-jitter, jitter, n * n) * dx
+    x[:, 1] = (jj.ravel() + 0.5) * dx + rng.uniform(-jitter, jitter, n * n) * dx
+    perm = rng.permutation(n * n)
+    return x[perm]
+
+
+def deck_from_points(x, h, ndim=2, maxnbr=60, **kw):
+    n = x.shape[0]
+    mat = Material(eos=29, rho=1000.0, av_l=1e-6, mass=1000.0 * n, h=h, eosinput=(1.0, 0, 0, 0))
+    return Deck(ndim=ndim, x=x, mat_id=np.ones(n, np.int32), materials=[mat], maxnbr=maxnbr, **kw)
+
+
+def test_brute_force_free_box():
+    x = jittered_box()
+    h = 1.2
+    o = orc.Oracle(deck_from_points(x, h))
+    got, _, _ = oracle_sets(o)
+    want = brute_force(x, h, 2)
+    assert got == want
+
+
+def test_brute_force_periodic_images():
+    x = jittered_box(n=16, jitter=0.25, seed=3)
+    L = 16.0
+    d = deck_from_points(x, 1.2, boundary=1)
+    d.boundary_code[:, 0] = 2
+    d.boundary_x[0, 0], d.boundary_x[1, 0] = 0.0, L
+    o = orc.Oracle(d)
+    o.phase(orc.PH_NEIGHBOURS)         # the in-loop pass builds the images
+    got, _, _ = oracle_sets(o)
+    want = brute_force(x, 1.2, 2, period=(0, 0.0, L))
+    assert got == want
+    assert o.globals()["ngp"] > 0
+
+
+def test_lattice_counts():
+    d2 = scenes.poiseuille(nx=20, ny=20, dx=1.0, hfac=1.2)
+    o = orc.Oracle(d2)
+    o.phase(orc.PH_NEIGHBOURS)
+    nn, _ = o.neighbours()
+    x = o.field(orc.F_X)
+    interior = (x[:, 1] > 3) & (x[:, 1] < 17)
+    assert np.all(nn[interior] == 20)
+    d3 = scenes.dam_break(nx_fluid=8, ny=8, nz_fluid=8, nx_tank=8, nz_tank=8, dx=1.0, maxnbr=100)
+    o3 = orc.Oracle(d3)
+    nn3, _ = o3.neighbours()
+    x3 = o3.field(orc.F_X)
+    inner = np.all((x3 > 3) & (x3 < 5), axis=1)
+    assert inner.sum() > 0 and np.all(nn3[inner] == 80)
+
+
+def test_symmetry():
+    x = jittered_box(seed=7)
+    o = orc.Oracle(deck_from_points(x, 1.3))
+    sets, _, _ = oracle_sets(o)
+    for i, s in enumerate(sets):
+        for j in s:
+            assert i in sets[j]
+
+
+def test_list_order_follows_cells():
+    x = jittered_box(n=16, seed=5)
+    L = 16.0
+    h = 1.2
+    d = deck_from_points(x, h, boundary=1)
+    d.boundary_code[:, 0] = 2
+    d.boundary_x[0, 0], d.boundary_x[1, 0] = 0.0, L
+    o = orc.Oracle(d)
+    o.phase(orc.PH_NEIGHBOURS)
+    nn, canon = o.neighbours()
+    # grid of the periodic axis x and the free axis y (MNeighbourVars.cpp:30-55)
+    gs = 1.0 / (2.0 * h)
+    glx = int(gs * L)
+    gsx = glx / L
+    gminx = 0.0 - 1.0 / gsx
+    gminy = x[:, 1].min() - 0.01 * h
+
+    def cell(idx):
+        if idx >= 0:
+            return (int(gsx * (x[idx, 0] - gminx)), int(gs * (x[idx, 1] - gminy)))
+        src, code = divmod(-idx - 1, 27)
+        sx = code // 9 - 1
+        cx = int(gsx * (x[src, 0] - gminx))
+        return (cx + sx * glx, int(gs * (x[src, 1] - gminy)))
+
+    for i in range(len(nn)):
+        seq = canon[i, :nn[i]]
+        keys = []
+        for idx in seq:
+            c = cell(int(idx))
+            # images: newest first = ascending source number (they were
+            # created walking the source cell's list in descending order);
+            # reals: descending number.  Both are ascending in -idx.
+            keys.append((c[0], c[1], 0 if idx < 0 else 1, -idx))
+        assert keys == sorted(keys), f"particle {i}: order not cell-major / descending"
+
+
+def test_neighbour_overflow():
+    d = scenes.poiseuille(nx=10, ny=10, maxnbr=6)
+    with pytest.raises(orc.OracleError, match="exceeded maximum"):
+        orc.Oracle(d)
