@@ -57,40 +57,58 @@ def main():
     import numpy as np
     import torch
     from sph07_amd import scenes
-    from sph07_amd.native import Scene
+    from sph07_amd.native import Scene, ARM_FP64, ARM_FP32C, F_V
 
     assert torch.cuda.is_available(), "no GPU visible"
     kw = dict(nx_fluid=20, ny=34, nz_fluid=15, nx_tank=80, nz_tank=30) if args.small else {}
     deck = scenes.dam_break(**kw)
     N = deck.np
-    print(f"bench: dam_break {N} particles ({deck.n_fluid} fluid), FP64 arm", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    sc = Scene(deck)
-    t_init = time.perf_counter() - t0
-    sc.Solution(args.warmup)
-    sc.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    done = sc.Solution(args.steps)
-    sc.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    assert done == args.steps, (done, args.steps)
-    value = N * args.steps / elapsed
-    nn, _ = sc.neighbours()
-    pairs = float(nn.astype(np.int64).sum())
-    print(f"bench: {args.steps} steps in {elapsed:.3f} s -> {value:.4e} particle-updates/s",
-          file=sys.stderr, flush=True)
 
-    # per-kernel timing pass (HIP events on the scene's stream)
-    sc.set_timing(True)
-    sc.Solution(args.profile_steps)
-    tm = sc.timing()
-    sc.set_timing(False)
-    kern = {k: (ms / cnt if cnt else 0.0, cnt) for k, (ms, cnt) in tm.items()}
-    for k, (avg, cnt) in kern.items():
-        if cnt:
-            print(f"bench: {k:40s} {avg * 1e3:9.1f} us x {cnt}", file=sys.stderr, flush=True)
+    def run_arm(arm):
+        t0 = time.perf_counter()
+        sc = Scene(deck, arm)
+        t_init = time.perf_counter() - t0
+        sc.Solution(args.warmup)
+        sc.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = sc.Solution(args.steps)
+        sc.synchronize()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        assert done == args.steps, (done, args.steps)
+        sc.set_timing(True)
+        sc.Solution(args.profile_steps)
+        tm = sc.timing()
+        sc.set_timing(False)
+        kern = {k: (ms / cnt if cnt else 0.0, cnt) for k, (ms, cnt) in tm.items()}
+        return sc, elapsed, t_init, kern
+
+    arms = {}
+    for arm_name, arm in (("fp64", ARM_FP64), ("fp32c", ARM_FP32C)):
+        print(f"bench: dam_break {N} particles ({deck.n_fluid} fluid), {arm_name} arm",
+              file=sys.stderr, flush=True)
+        sc, elapsed, t_init, kern = run_arm(arm)
+        rate = N * args.steps / elapsed
+        print(f"bench: {arm_name}: {args.steps} steps in {elapsed:.3f} s -> {rate:.4e} particle-updates/s",
+              file=sys.stderr, flush=True)
+        for k, (avg, cnt) in kern.items():
+            if cnt:
+                print(f"bench:   {k:40s} {avg * 1e3:9.1f} us x {cnt}", file=sys.stderr, flush=True)
+        arms[arm_name] = dict(scene=sc, elapsed=elapsed, t_init=t_init, kern=kern, rate=rate)
+    # both arms advanced the same number of steps from the same deck
+    moving = deck.dispbc == 0
+    v64 = arms["fp64"]["scene"].field(F_V)[moving]
+    v32 = arms["fp32c"]["scene"].field(F_V)[moving]
+    vel_err = float(np.linalg.norm(v32 - v64) / max(np.linalg.norm(v64), 1e-300))
+    nn, _ = arms["fp64"]["scene"].neighbours()
+    pairs = float(nn.astype(np.int64).sum())
+    main_arm = "fp32c"
+    elapsed, kern, t_init = arms[main_arm]["elapsed"], arms[main_arm]["kern"], arms[main_arm]["t_init"]
+    value = arms[main_arm]["rate"]
+    for a in arms.values():
+        a["scene"].close()
+
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms = kern[dom][0]
     bytes_alg = BYTES_3D.get(dom, 0) * N
@@ -102,12 +120,7 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": bytes_alg}
-    if dom in FLOPS_PER_PAIR_3D and avg_ms:
-        tfl = FLOPS_PER_PAIR_3D[dom] * pairs / (avg_ms * 1e-3) / 1e12
-        roof["valu_fp64"] = {"achieved": round(tfl, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                             "frac": round(tfl / FP64_PEAK_TFLOPS, 4),
-                             "flops_per_pair": FLOPS_PER_PAIR_3D[dom]}
-    kernels = {k: round(v[0], 4) for k, v in kern.items() if v[1]}
+    kernels = {a: {k: round(v[0], 4) for k, v in arms[a]["kern"].items() if v[1]} for a in arms}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -127,11 +140,14 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "dam_break_3d" + ("_small" if args.small else ""), "particles": N,
                    "fluid_particles": deck.n_fluid, "dx": deck.meta["dx"], "h_over_dx": 1.3,
-                   "arm": "fp64", "monaghan_alpha": 0.1, "parallelism": "single-gpu"},
+                   "arm": main_arm, "monaghan_alpha": 0.1, "parallelism": "single-gpu"},
+        "fp64_value": arms["fp64"]["rate"],
+        "speedup_vs_fp64": value / arms["fp64"]["rate"],
+        "fp32c_rel_l2_velocity_error_vs_fp64": vel_err,
         "neighbour_interactions_per_s": pairs * args.steps / elapsed,
         "init_s": round(t_init, 3),
         "kernels_avg_ms": kernels,

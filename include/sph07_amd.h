@@ -83,6 +83,9 @@ typedef struct sph_globals {
 
 /* Arithmetic arm of the pair sweeps. */
 #define SPH_ARM_FP64        0   /* reference arithmetic, reference summation order */
+#define SPH_ARM_FP32C       1   /* FP32 pair sums with TwoSum compensation; FP64 state,
+                                   integration and rigid body.  Applies to sph_solution;
+                                   the phase-by-phase entry points run the FP64 sweeps. */
 
 /* Fields for sph_download (np entries; vector fields np*3). */
 enum {

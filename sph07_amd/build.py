@@ -16,11 +16,13 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsph07_amd.so")
 ARCH = os.environ.get("SPH07_ARCH", "gfx950")
 
-SOURCES = ["sph_kernels.hip", "sph_capi.cpp"]
+# (source, floating-point contraction): the FP64 arm and the host code keep
+# the reference's operation order; the FP32-compensated sweeps may fuse
+# multiply-adds (TwoSum itself is additions only, so contraction cannot touch it).
+SOURCES = [("sph_kernels.hip", "off"), ("sph_fp32c.hip", "fast"), ("sph_capi.cpp", "off")]
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
-    "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
@@ -38,9 +40,10 @@ def build(verbose: bool = False) -> str:
     out_dir = os.path.join(HERE, "_build")
     os.makedirs(out_dir, exist_ok=True)
     objs = []
-    for src in SOURCES:
+    for src, contract in SOURCES:
         obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc(), *COMMON, f"-ffp-contract={contract}", "-x", "hip", "-c",
+               os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -317,10 +317,20 @@ void fused_step(Context *c)
     timed(c, K_BIN, [&] { launch_setup_linked_list(P, A, st); });
     if (P.periodic) timed(c, K_IMAGES, [&] { launch_ghost_setup(P, A, st); });
     timed(c, K_NBR, [&] { launch_neighbours(P, A, 1, st); });
-    timed(c, K_STRAIN, [&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st); });
+    if (c->arm == SPH_ARM_FP32C)
+        timed(c, K_STRAIN, [&] { launch_strain32(P, A, st); });
+    else
+        timed(c, K_STRAIN, [&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st); });
     if (P.st) timed(c, K_NORMALS, [&] { launch_normals(P, A, st); });
     timed(c, K_ADVANCE, [&] { launch_advance(P, A, st); });
-    timed(c, K_MOMENTUM, [&] { launch_momentum(P, A, 1, st); });
+    timed(c, K_MOMENTUM, [&] {
+        if (c->arm == SPH_ARM_FP32C) {
+            launch_momentum32(P, A, st);
+            launch_rigid_reduce(P, A, 1, st);
+        } else {
+            launch_momentum(P, A, 1, st);
+        }
+    });
     timed(c, K_INTEGRATE, [&] { launch_integrate(P, A, M_BC | M_VEL | M_MOVE | M_RHO, st); });
     timed(c, K_FINISH, [&] { launch_finish(P, A, 1, 1, st); });
 }
@@ -615,8 +625,9 @@ int sph_create(const sph_scene_desc *desc, int arm, void **out, char *err, int e
     if (!desc || !out) return SPH_ERROR;
     Context *c = new Context();
     c->arm = arm;
-    int rc = arm == SPH_ARM_FP64 ? create_impl(c, desc) : SPH_ERR_UNSUPPORTED;
-    if (arm != SPH_ARM_FP64) c->err = "unknown arithmetic arm";
+    const bool arm_ok = arm == SPH_ARM_FP64 || arm == SPH_ARM_FP32C;
+    int rc = arm_ok ? create_impl(c, desc) : SPH_ERR_UNSUPPORTED;
+    if (!arm_ok) c->err = "unknown arithmetic arm";
     if (rc) {
         if (err && errlen > 0) snprintf(err, (size_t)errlen, "%s", c->err.c_str());
         sph_destroy(c);

@@ -1,0 +1,275 @@
+// sph_fp32c.hip -- FP32-compensated pair sweeps.
+//
+// Positions, velocities, densities, the time integration and every
+// per-particle update stay FP64 (the same k_integrate / k_advance / rigid
+// kernels as the FP64 arm).  Only the bulk pair sums run in FP32: the pair
+// separation and velocity difference are formed in FP64 (exact cancellation
+// of the large coordinates) and rounded once to FP32; the kernel gradient,
+// pressure, viscous and Monaghan terms are FP32 with hardware reciprocal and
+// reciprocal square root; each particle's sums are accumulated with the
+// TwoSum error-free transformation (Knuth 1969) into (sum, compensation)
+// FP32 pairs and converted to FP64 once at the end.
+//
+// Neighbour sets and their order are those of the FP64 arm (shared lists).
+#include "sph_kernels.hpp"
+
+#include <math.h>
+
+namespace sph {
+
+#define CTL_STOP32(ctl) ((ctl)->err != 0 || (ctl)->done != 0)
+
+// TwoSum: s + e == a + b exactly (Knuth, TAOCP vol. 2, 4.2.2)
+__device__ __forceinline__ void two_sum_acc(float &s, float &c, float x)
+{
+    const float t = s + x;
+    const float bp = t - s;
+    const float err = (s - (t - bp)) + (x - bp);
+    s = t;
+    c += err;
+}
+
+struct PairConst32 {
+    float ih, c1, c2, eps2, havg;
+};
+
+__device__ __forceinline__ PairConst32 pc32(const PairConst &pc)
+{
+    PairConst32 q;
+    q.havg = (float)pc.havg;
+    q.ih = (float)(1.0 / pc.havg);
+    q.c1 = (float)pc.const1;
+    q.c2 = (float)pc.const2;
+    q.eps2 = (float)pc.eps2;
+    return q;
+}
+
+// dW/dr / r (so that grad W = f * dx) in FP32 from r^2
+__device__ __forceinline__ float gradw_factor(float r2, const PairConst32 &q)
+{
+    const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    const float z = r2 * rinv * q.ih;
+    float dW;
+    if (z < 1.0f)
+        dW = q.c1 * ((2.25f * z - 3.0f) * z);
+    else if (z < 2.0f)
+        dW = -3.0f * q.c2 * ((2.0f - z) * (2.0f - z));
+    else
+        dW = 0.0f;
+    return dW * rinv * q.ih;
+}
+
+__device__ __forceinline__ double eos_p32(const Mat &mt, double rho)
+{
+    if (mt.eos == 29) return (mt.c * mt.c) * (rho - mt.rho0);
+    return mt.eos_b * (pow(rho / mt.rho0, mt.eos_gamma) - 1.0);
+}
+
+template <int D>
+__device__ __forceinline__ double crit_one32(const Params &P, const Arrays &A, const Ctl *ctl, int i,
+                                             const Mat &mt, double rho_i)
+{
+    double ct;
+    const double h = mt.h;
+    if (P.tcrit_opt == 3) {
+        ct = 0.25 * h / mt.c;
+        const double b = 0.125 * (h * h) / mt.nu;
+        ct = ct < b ? ct : b;
+        double aabs = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) aabs += A.a[d][i] * A.a[d][i];
+        aabs = sqrt(aabs);
+        const double c3 = 0.25 * sqrt(h / aabs);
+        ct = ct < c3 ? ct : c3;
+        if (P.st) {
+            const double c4 = 0.25 * sqrt(rho_i * ((h * h) * h) / 2 / 3.141592654 / mt.sigma);
+            ct = ct < c4 ? ct : c4;
+        }
+    } else {
+        double va = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) va += A.v[d][i] * A.v[d][i];
+        va = sqrt(va);
+        const double len = P.tcrit_opt == 1 ? h : A.mindist[i];
+        ct = len / (mt.c + va);
+        if (ctl->timestep == 1) ct = 1.e-09;
+    }
+    return ct;
+}
+
+__device__ __forceinline__ double wave_min32(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// continuity sweep + RhoUpdate/EOS + critical step (fused mode of k_strain)
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP32(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double crit = 1.0e20;
+    if (i < P.N) {
+        const uint32_t fi = A.info[i];
+        const bool act = fi & kActive;
+        const int mat_i = info_mat(fi);
+        const Mat &mt = P.mat[mat_i];
+        const double dt = ctl->dt;
+        double rho_i = A.rho[i];
+        if (act) {
+            double xi[D], vi[D];
+            float gs[D], gc[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                xi[d] = A.xs[d][i];
+                vi[d] = A.v[d][i];
+                gs[d] = 0.0f;
+                gc[d] = 0.0f;
+            }
+            const int nn = A.nnbr[i];
+            const size_t stride = (size_t)P.Ncap;
+            const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
+            for (int k = 0; k < nn; k++) {
+                const int j = A.nbr[(size_t)k * stride + i];
+                const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][info_mat(A.info[j])]);
+                float dx[D], r2 = 0.0f;
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    dx[d] = (float)(xi[d] - A.xs[d][j]);
+                    r2 += dx[d] * dx[d];
+                }
+                const float f = gradw_factor(r2, q) * (float)A.vol[j];
+#pragma unroll
+                for (int d = 0; d < D; d++)
+                    two_sum_acc(gs[d], gc[d], (float)(A.v[d][j] - vi[d]) * (f * dx[d]));
+            }
+            double tr = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; d++) tr += (double)gs[d] + (double)gc[d];
+            A.trd[i] = tr;
+#pragma unroll
+            for (int d = 0; d < D; d++) A.x[d][i] = xi[d] + 0.5 * vi[d] * dt;
+            rho_i = rho_i * (1.0 - tr * dt);
+            const double p = eos_p32(mt, rho_i);
+            A.p[i] = p;
+            A.prho2[i] = p / (rho_i * rho_i);
+        }
+        A.rho_new[i] = rho_i;
+        const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
+        if (act && !(fi & kRigid)) crit = ct;
+    }
+    __shared__ double red[kBlock / 64];
+    crit = wave_min32(crit);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = crit;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int k = 1; k < kBlock / 64; k++) m = fmin(m, red[k]);
+        atomicMin(&ctl->critts_key, dkey(m));
+    }
+}
+
+// momentum sweep: pressure gradient, Morris viscosity, Monaghan term
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP32(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const uint32_t fi = A.info[i];
+    double ai[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) ai[d] = 0.0;
+    if (fi & kActive) {
+        const int mat_i = info_mat(fi);
+        const Mat &mi = P.mat[mat_i];
+        const bool rig_i = fi & kRigid;
+        const double rho_i64 = A.rho_new[i];
+        const float rho_i = (float)rho_i64, pr_i = (float)A.prho2[i];
+        const float q_i = (float)(mi.nu * rho_i64);
+        const bool monaghan = mi.visc_type == 2;
+        double xi[D], vi[D];
+        float as[D], ac[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            xi[d] = A.x[d][i];
+            vi[d] = A.v[d][i];
+            as[d] = 0.0f;
+            ac[d] = 0.0f;
+        }
+        const int nn = A.nnbr[i];
+        const size_t stride = (size_t)P.Ncap;
+        const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
+        for (int k = 0; k < nn; k++) {
+            const int j = A.nbr[(size_t)k * stride + i];
+            const uint32_t fj = A.info[j];
+            const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
+            const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][mat_j]);
+            const float mj = (float)A.m[j];
+            const double rho_j64 = A.rho_new[j];
+            const float rho_j = (float)rho_j64;
+            float dx[D], dv[D], r2 = 0.0f, vr = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                dx[d] = (float)(xi[d] - A.x[d][j]);
+                dv[d] = (float)(vi[d] - A.v[d][j]);
+                r2 += dx[d] * dx[d];
+                vr += dv[d] * dx[d];
+            }
+            const float f = gradw_factor(r2, q);            // grad W = f * dx
+            float sdw = -mj * ((float)A.prho2[j] + pr_i);   // coefficient of grad W
+            float tv = 0.0f;                                // coefficient of (v_i - v_j)
+            if (!(rig_i && (fj & kRigid))) {
+                const float qj = (float)(P.mat[mat_j].nu * rho_j64);
+                const float inv = __builtin_amdgcn_rcpf(rho_i * rho_j * (r2 + q.eps2));
+                tv = mj * (q_i + qj) * (f * r2) * inv;
+                if (monaghan && vr < 0.0f) {
+                    const Mat &mj_ = P.mat[mat_j];
+                    const float alpha = (float)(0.5 * (mi.alpha + mj_.alpha));
+                    const float cbar = (float)(0.5 * (mi.c + mj_.c));
+                    const float mu = q.havg * vr * __builtin_amdgcn_rcpf(r2 + q.eps2);
+                    const float pij = -alpha * cbar * mu * __builtin_amdgcn_rcpf(0.5f * (rho_i + rho_j));
+                    sdw += -mj * pij;
+                }
+            }
+            const float sf = sdw * f;
+#pragma unroll
+            for (int d = 0; d < D; d++) two_sum_acc(as[d], ac[d], sf * dx[d] + tv * dv[d]);
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++) ai[d] = (double)as[d] + (double)ac[d];
+        if (P.st) {
+            const double fst = -mi.sigma / rho_i64 * A.kappa[i];
+#pragma unroll
+            for (int d = 0; d < D; d++) ai[d] = ai[d] + fst * A.nrm[d][i];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
+}
+
+static inline int nblk32(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_strain32(const Params &P, const Arrays &A, hipStream_t st)
+{
+    switch (P.ndim) {
+    case 1: hipLaunchKernelGGL(k_strain32<1>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL(k_strain32<2>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
+    default: hipLaunchKernelGGL(k_strain32<3>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
+    }
+}
+
+void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st)
+{
+    switch (P.ndim) {
+    case 1: hipLaunchKernelGGL(k_momentum32<1>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL(k_momentum32<2>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
+    default: hipLaunchKernelGGL(k_momentum32<3>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
+    }
+}
+
+}  // namespace sph

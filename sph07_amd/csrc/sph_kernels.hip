@@ -1114,6 +1114,12 @@ void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_
         hipLaunchKernelGGL(k_rigid_reduce, dim3(1), dim3(kRigidBlock), 0, st, P, A, rigid_kin);
 }
 
+void launch_rigid_reduce(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st)
+{
+    if (P.rigid)
+        hipLaunchKernelGGL(k_rigid_reduce, dim3(1), dim3(kRigidBlock), 0, st, P, A, rigid_kin);
+}
+
 void launch_rigid_kin(const Params &P, const Arrays &A, hipStream_t st)
 {
     if (P.rigid) hipLaunchKernelGGL(k_rigid_kin, dim3(1), dim3(1), 0, st, A.ctl);

@@ -19,8 +19,13 @@ void launch_normals(const Params &P, const Arrays &A, hipStream_t st);
 void launch_advance(const Params &P, const Arrays &A, hipStream_t st);
 void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st);
 void launch_rigid_kin(const Params &P, const Arrays &A, hipStream_t st);
+void launch_rigid_reduce(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st);
 void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st);
 void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st);
 void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st);
+
+// FP32-compensated pair sweeps (sph_fp32c.hip); fused-step forms only
+void launch_strain32(const Params &P, const Arrays &A, hipStream_t st);
+void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st);
 
 }  // namespace sph

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libsph07_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sph07_amd.h")
 
 ARM_FP64 = 0
+ARM_FP32C = 1
 
 F_X, F_V, F_A, F_RHO, F_P, F_C, F_MASS, F_H, F_TRD, F_MINDIST, F_NORMAL, F_NNORMAL, \
     F_N, F_KAPPA, F_CRITTS, F_RHOOLD, F_ACTIVE, F_VABS = range(18)

@@ -136,3 +136,26 @@ def test_neighbour_overflow_is_reported():
     with pytest.raises(SphError) as e:
         gpu_scene(deck)
     assert e.value.code == 2 and "exceeded maximum" in str(e.value)
+
+
+@pytest.mark.parametrize("name,steps", [("poiseuille", 200), ("dam_break", 60), ("cylinder", 60),
+                                        ("cavity", 200), ("droplet", 100)])
+def test_fp32c_arm_error(name, steps):
+    """FP32-compensated pair sums: relative L2 velocity error of the moving
+    particles <= 1e-5 against the FP64 arm (which matches the oracle above);
+    neighbour sets identical."""
+    from sph07_amd.native import Scene, ARM_FP32C, F_V, F_RHO
+    deck = small_scenes()[name]()
+    g64 = Scene(deck)
+    g32 = Scene(deck, ARM_FP32C)
+    assert g64.Solution(steps) == g32.Solution(steps) == steps
+    moving = deck.dispbc == 0
+    v64, v32 = g64.field(F_V)[moving], g32.field(F_V)[moving]
+    err = np.linalg.norm(v32 - v64) / np.linalg.norm(v64)
+    print(f"{name}: fp32c rel L2 velocity error {err:.3e}")
+    assert err <= 1e-5, err
+    r64, r32 = g64.field(F_RHO), g32.field(F_RHO)
+    assert np.abs(r32 - r64).max() / np.abs(r64).max() <= 1e-6
+    nn64, c64 = g64.neighbours()
+    nn32, c32 = g32.neighbours()
+    assert np.array_equal(nn64, nn32)
