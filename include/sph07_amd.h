@@ -152,6 +152,16 @@ int  sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr);
 /* Blocks until all work queued on the context's stream has finished. */
 int  sph_synchronize(void *ctx);
 
+/* --- .mcm deck reader (MSimulationData::GetInput, MSimulationData.cpp:186-1068) --- */
+/* Reads a deck into an owned descriptor; *deck is freed with sph_deck_free. */
+int  sph_deck_read(const char *path, void **deck, char *err, int errlen);
+/* The descriptor to pass to sph_create (valid until sph_deck_free). */
+const sph_scene_desc *sph_deck_desc(void *deck);
+/* Title card (:194-196) and status interval of control card 3 (:274-275). */
+const char *sph_deck_title(void *deck);
+int  sph_deck_status_interval(void *deck);
+void sph_deck_free(void *deck);
+
 /* --- measurement (no reference counterpart: the reference only prints total
    wall seconds, MSimulationData.cpp:110-122) --- */
 /* Turns HIP-event timing of the kernel groups of sph_solution on or off and

@@ -19,7 +19,9 @@ ARCH = os.environ.get("SPH07_ARCH", "gfx950")
 # (source, floating-point contraction): the FP64 arm and the host code keep
 # the reference's operation order; the FP32-compensated sweeps may fuse
 # multiply-adds (TwoSum itself is additions only, so contraction cannot touch it).
-SOURCES = [("sph_kernels.hip", "off"), ("sph_fp32c.hip", "fast"), ("sph_capi.cpp", "off")]
+SOURCES = [("sph_kernels.hip", "off"), ("sph_fp32c.hip", "fast"), ("sph_capi.cpp", "off"),
+           ("mcm_reader.cpp", "off")]
+EXE = os.path.join(HERE, "sph07_run")
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
@@ -60,6 +62,20 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    # command-line driver, linked against the library next to it
+    main_obj = os.path.join(out_dir, "sph07_main.o")
+    cmd = [hipcc(), "-O2", "-std=c++17", "-c", os.path.join(CSRC, "sph07_main.cpp"), "-o", main_obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed on sph07_main.cpp")
+    cmd = [hipcc(), main_obj, "-o", EXE + ".tmp", f"-L{HERE}", "-lsph07_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link of sph07_run failed")
+    os.replace(EXE + ".tmp", EXE)
     return LIB
 
 

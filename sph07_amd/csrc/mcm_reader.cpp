@@ -1,0 +1,320 @@
+// mcm_reader.cpp -- reader for sph07's fixed-format .mcm input decks.
+//
+// Reads the cards in the order of MSimulationData::GetInput
+// (MSimulationData.cpp:186-219) with the same fixed-width sscanf formats
+// (the deck format is the drop-in contract, so the card layouts cannot
+// differ): comment lines start with '*' or '$' and are skipped, blank lines
+// are consumed as cards (PassInputComments, :221-242).  The result is an
+// owned sph_scene_desc ready for sph_create.
+#include "../../include/sph07_amd.h"
+
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Deck {
+    sph_scene_desc d{};
+    std::vector<sph_material_desc> mats;
+    std::vector<double> x, v, node_mass, node_rho;
+    std::vector<int32_t> mat_id, color, dispbc;
+    std::string title, err;
+    int version = 0;
+    int status_interval = 100;
+    double stpltime = 0.0;
+};
+
+struct Reader {
+    FILE *f;
+    int lineno = 0;
+    char buf[300];
+    // MSimulationData::PassInputComments (:221-242)
+    bool next(std::string &out)
+    {
+        char *r;
+        do {
+            r = fgets(buf, sizeof buf, f);
+            if (r) lineno++;
+        } while (r && (buf[0] == '*' || buf[0] == '$'));
+        if (!r) return false;
+        out = buf;
+        return true;
+    }
+};
+
+int fail(Deck *dk, Reader &rd, const std::string &msg)
+{
+    dk->err = msg + " (line " + std::to_string(rd.lineno) + ")";
+    return SPH_ERROR;
+}
+
+int read_deck(Deck *dk, FILE *f)
+{
+    Reader rd{f};
+    std::string line;
+    sph_scene_desc &d = dk->d;
+    char tmp[300];
+    // title (:194-196)
+    if (!rd.next(line)) return fail(dk, rd, "empty deck");
+    memset(tmp, 0, sizeof tmp);
+    sscanf(line.c_str(), "%78c%2d", tmp, &dk->version);
+    tmp[78] = '\0';
+    dk->title = tmp;
+    // control cards (:244-323)
+    int axopt = 0, nummat = 0, np = 0, max_np = 0, nstressp = 0, nvelocp = 0;
+    if (!rd.next(line)) return fail(dk, rd, "missing control card 1");
+    sscanf(line.c_str(), "%5d%5d%10d%10d%10d%10d%10d", &axopt, &d.disctype, &nummat, &np, &max_np,
+           &nstressp, &nvelocp);
+    double drelax = 0.0;
+    if (!rd.next(line)) return fail(dk, rd, "missing control card 2");
+    sscanf(line.c_str(), "%10lf%10lf%10lf%10lf", &d.endtime, &d.tssf, &d.itss, &drelax);
+    double thpltime = 0.0;
+    int state_opt = 0, thnode = 0, ntrans = 0, restart = 0, run_restart = 0;
+    if (!rd.next(line)) return fail(dk, rd, "missing control card 3");
+    sscanf(line.c_str(), "%10lf%5d%10lf%10d%5d%10d%5d%5d", &dk->stpltime, &state_opt, &thpltime,
+           &thnode, &ntrans, &dk->status_interval, &restart, &run_restart);
+    int keep_files = 0, init_v_opt = 0;
+    if (!rd.next(line)) return fail(dk, rd, "missing control card 4");
+    sscanf(line.c_str(), "%5d%5d%5d%5d%5d", &keep_files, &init_v_opt, &d.massopt, &d.init_h_opt,
+           &d.init_rhoe);
+    int contact = 0, mirr = 0, nlcur = 0, nthp[3] = {0, 0, 0}, st = 0, rigid = 0;
+    if (!rd.next(line)) return fail(dk, rd, "missing control card 5");
+    sscanf(line.c_str(), "%5d%5d%5d%5d%5d%5d%5d%5d%5d%5d", &contact, &d.tcrit_opt, &d.veloc_opt, &mirr,
+           &nlcur, &nthp[0], &nthp[1], &nthp[2], &st, &rigid);
+    int krtype = 0;
+    if (!rd.next(line)) return fail(dk, rd, "missing control card 6");
+    sscanf(line.c_str(), "%5d%5d%5d", &d.h_opt, &krtype, &d.maxnbr);
+    if (!rd.next(line)) return fail(dk, rd, "missing control card 7");
+    d.surface_tension = st != 0;
+    d.rigid_body = rigid != 0;
+    if (d.tssf == 0.0) d.tssf = 0.8;
+    if (d.maxnbr == 0) d.maxnbr = 40;
+    if (dk->status_interval < 1) dk->status_interval = 100;
+    d.boundary = mirr == 1;
+    d.ndim = axopt;
+    d.nummat = nummat;
+    d.np = np;
+    d.baseaccel = (nthp[0] == 1 || nthp[1] == 1 || nthp[2] == 1);
+    if (d.ndim < 1 || d.ndim > 3) return fail(dk, rd, "bad number of dimensions");
+    if (np < 1 || nummat < 1) return fail(dk, rd, "bad particle or material count");
+    // materials (:441-541)
+    dk->mats.resize(nummat);
+    for (int m = 0; m < nummat; m++) {
+        sph_material_desc &mt = dk->mats[m];
+        memset(&mt, 0, sizeof mt);
+        int n = 0, ihq = 0;
+        double qh = 0.0;
+        if (!rd.next(line)) return fail(dk, rd, "missing material card");
+        sscanf(line.c_str(), "%5d%5d%10lf%5d%5d%10lf%5d%10lf%10lf%10lf", &n, &mt.model, &mt.rho, &mt.eos,
+               &ihq, &qh, &mt.visc_type, &mt.av_q, &mt.av_l, &mt.surface_tension);
+        double rho_min = 0, rho_max = 0;
+        if (!rd.next(line)) return fail(dk, rd, "missing material options card");
+        sscanf(line.c_str(), "%10lf%10lf%10lf%10lf", &mt.mass, &mt.h, &rho_min, &rho_max);
+        // MMaterial::CheckDefaults (MMaterial.cpp:71-81)
+        if (mt.visc_type == 0) mt.visc_type = 1;
+        if (mt.av_l == 0.0) mt.av_l = 0.06;
+        if (mt.av_q == 0.0) mt.av_q = 1.5;
+        if (n < 0 || n > nummat) return fail(dk, rd, "Error in input - material number is out of range.");
+        if (!rd.next(line)) return fail(dk, rd, "missing material title");
+        double strin[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (mt.model == 1 || mt.model == 3 || mt.model == 9 || mt.model == 10) {
+            for (int j = 0; j < 6; j++) {
+                if (!rd.next(line)) return fail(dk, rd, "missing constitutive card");
+                if (j == 0 || mt.model == 10)
+                    sscanf(line.c_str(), "%10lf%10lf%10lf%10lf%10lf%10lf%10lf%10lf", &strin[0], &strin[1],
+                           &strin[2], &strin[3], &strin[4], &strin[5], &strin[6], &strin[7]);
+            }
+        }
+        mt.strinput0 = strin[0];
+        if (mt.eos != 0) {
+            if (!(mt.model == 9 || mt.model == 10)) return fail(dk, rd, "EOS not permitted for this material");
+            if (!rd.next(line)) return fail(dk, rd, "missing EOS title");
+        }
+        if (mt.eos == 28) {
+            if (!rd.next(line)) return fail(dk, rd, "missing EOS card");
+            sscanf(line.c_str(), "%10lf%10lf", &mt.eosinput[0], &mt.eosinput[1]);
+        } else if (mt.eos == 29) {
+            if (!rd.next(line)) return fail(dk, rd, "missing EOS card");
+            sscanf(line.c_str(), "%10lf", &mt.eosinput[0]);
+        } else if (mt.eos == 13) {
+            if (!rd.next(line)) return fail(dk, rd, "missing EOS card");
+            sscanf(line.c_str(), "%10lf%10lf%10lf%10lf", &mt.eosinput[0], &mt.eosinput[1], &mt.eosinput[2],
+                   &mt.eosinput[3]);
+        }
+    }
+    // nodes (:655-763)
+    dk->x.assign(3 * (size_t)np, 0.0);
+    dk->v.assign(3 * (size_t)np, 0.0);
+    dk->mat_id.assign(np, 1);
+    dk->color.assign(np, 1);
+    dk->dispbc.assign(np, 0);
+    for (int j = 1; j <= np; j++) {
+        int n = 0, disp = 0, mat = 0, color = 1;
+        double xx[3] = {0, 0, 0};
+        if (!rd.next(line)) return fail(dk, rd, "missing node card");
+        switch (d.ndim) {
+        case 1: sscanf(line.c_str(), "%8d%5d%20lf%7d%7d", &n, &disp, &xx[0], &mat, &color); break;
+        case 2: sscanf(line.c_str(), "%8d%5d%20lf%20lf%7d%7d", &n, &disp, &xx[0], &xx[1], &mat, &color); break;
+        default:
+            sscanf(line.c_str(), "%8d%5d%20lf%20lf%20lf%7d%7d", &n, &disp, &xx[0], &xx[1], &xx[2], &mat, &color);
+        }
+        if (n != j) return fail(dk, rd, "Card for node " + std::to_string(n) + " is out of order.");
+        if (mat == -1) return fail(dk, rd, "Lennard-Jones particles are outside the GPU path");
+        if (mat < 1 || mat > nummat) return fail(dk, rd, "Node " + std::to_string(n) + " has incorrect material type.");
+        for (int k = 0; k < 3; k++) dk->x[3 * (size_t)(j - 1) + k] = xx[k];
+        dk->mat_id[j - 1] = mat;
+        dk->color[j - 1] = color;
+        dk->dispbc[j - 1] = disp;
+    }
+    if (d.massopt == 1 || d.init_h_opt == 1 || d.init_rhoe == 1) {
+        if (d.massopt == 1) dk->node_mass.assign(np, 0.0);
+        if (d.init_rhoe == 1) dk->node_rho.assign(np, 0.0);
+        for (int j = 1; j <= np; j++) {
+            int n = 0;
+            double pd[4] = {0, 0, 0, 0};
+            if (!rd.next(line)) return fail(dk, rd, "missing additional node card");
+            sscanf(line.c_str(), "%8d%20lf%20lf%20lf%20lf", &n, &pd[0], &pd[1], &pd[2], &pd[3]);
+            if (n != j) return fail(dk, rd, "Additional card for node out of order.");
+            if (d.massopt == 1) dk->node_mass[j - 1] = pd[0];
+            if (d.init_rhoe == 1) dk->node_rho[j - 1] = pd[2];
+        }
+    }
+    // velocities with range generation (:765-881)
+    if (init_v_opt == 1) {
+        int nid = 0;
+        double vv[3] = {0, 0, 0};
+        auto read_v = [&](int &id, double *v3) {
+            v3[0] = v3[1] = v3[2] = 0.0;
+            switch (d.ndim) {
+            case 1: sscanf(line.c_str(), "%8d%10lf", &id, &v3[0]); break;
+            case 2: sscanf(line.c_str(), "%8d%10lf%10lf", &id, &v3[0], &v3[1]); break;
+            default: sscanf(line.c_str(), "%8d%10lf%10lf%10lf", &id, &v3[0], &v3[1], &v3[2]);
+            }
+        };
+        if (!rd.next(line)) return fail(dk, rd, "missing velocity card");
+        read_v(nid, vv);
+        if (nid != 1) return fail(dk, rd, "first nodal velocity card is not for node 1");
+        for (int k = 0; k < 3; k++) dk->v[k] = vv[k];
+        bool gen = false;
+        for (int i = 2; i <= np; i++) {
+            double *vi = &dk->v[3 * (size_t)(i - 1)], *vp = vi - 3;
+            if (!gen) {
+                if (!rd.next(line)) return fail(dk, rd, "missing velocity card");
+                read_v(nid, vi);
+                if (i < nid) {
+                    if (nid > np) return fail(dk, rd, "Particle id in velocity input exceeds number of particles");
+                    if (vi[0] != vp[0] || vi[1] != vp[1] || vi[2] != vp[2])
+                        return fail(dk, rd, "Particle velocity for auto generation not constant.");
+                    gen = true;
+                } else if (i > nid) {
+                    return fail(dk, rd, "Velocities must be specified in increasing order.");
+                }
+            } else {
+                for (int k = 0; k < d.ndim; k++) vi[k] = vp[k];
+                if (i == nid) gen = false;
+            }
+        }
+    }
+    // boundary planes (:883-1003)
+    if (d.boundary) {
+        int c[6] = {0, 0, 0, 0, 0, 0};
+        double bx[6] = {0, 0, 0, 0, 0, 0};
+        if (!rd.next(line)) return fail(dk, rd, "missing boundary codes");
+        sscanf(line.c_str(), "%10d%10d%10d%10d%10d%10d", &c[0], &c[1], &c[2], &c[3], &c[4], &c[5]);
+        if (!rd.next(line)) return fail(dk, rd, "missing boundary planes");
+        sscanf(line.c_str(), "%10lf%10lf%10lf%10lf%10lf%10lf", &bx[0], &bx[1], &bx[2], &bx[3], &bx[4], &bx[5]);
+        bool found = false;
+        for (int m = 0; m < d.ndim; m++) {
+            d.boundary_code[0][m] = c[2 * m];
+            d.boundary_code[1][m] = c[2 * m + 1];
+            d.boundary_x[0][m] = bx[2 * m];
+            d.boundary_x[1][m] = bx[2 * m + 1];
+            if (c[2 * m] || c[2 * m + 1]) found = true;
+            if ((c[2 * m] == 2) != (c[2 * m + 1] == 2))
+                return fail(dk, rd, "Error in definition of periodic boundary conditions.");
+        }
+        if (!found) d.boundary = 0;
+    }
+    // base accelerations (:1005-1019)
+    if (d.baseaccel) {
+        if (!rd.next(line)) return fail(dk, rd, "missing base acceleration card");
+        sscanf(line.c_str(), "%20lf%20lf%20lf", &d.base_a[0], &d.base_a[1], &d.base_a[2]);
+    }
+    for (int m = 0; m < 3; m++)
+        if (nthp[m] != 1) d.base_a[m] = 0.0;
+    // plot card (:1021-1037)
+    double plottime = 0.0;
+    if (rd.next(line)) {
+        sscanf(line.c_str(), "%10lf", &plottime);
+        if (plottime != 0.0) {
+            rd.next(line);
+            rd.next(line);
+        }
+    }
+    // rigid body (:1058-1068)
+    if (d.rigid_body) {
+        double pm = 0, dp = 0;
+        if (!rd.next(line)) return fail(dk, rd, "missing rigid body card");
+        sscanf(line.c_str(), "%5d%10lf%10lf%10lf%10lf", &d.rb_id, &d.rb_total_mass, &d.rb_inertia, &pm, &dp);
+    }
+    // wire the pointers
+    d.mat = dk->mats.data();
+    d.x = dk->x.data();
+    d.v = dk->v.data();
+    d.mat_id = dk->mat_id.data();
+    d.color = dk->color.data();
+    d.dispbc = dk->dispbc.data();
+    d.node_mass = dk->node_mass.empty() ? nullptr : dk->node_mass.data();
+    d.node_h = nullptr;
+    d.node_rho = dk->node_rho.empty() ? nullptr : dk->node_rho.data();
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_deck_read(const char *path, void **out, char *err, int errlen)
+{
+    if (out) *out = nullptr;
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        if (err && errlen > 0) snprintf(err, (size_t)errlen, "Cannot open input file %s", path);
+        return SPH_ERROR;
+    }
+    Deck *dk = new Deck();
+    int rc = read_deck(dk, f);
+    fclose(f);
+    if (rc) {
+        if (err && errlen > 0) snprintf(err, (size_t)errlen, "%s", dk->err.c_str());
+        delete dk;
+        return rc;
+    }
+    *out = dk;
+    return SPH_OK;
+}
+
+const sph_scene_desc *sph_deck_desc(void *deck)
+{
+    return deck ? &((Deck *)deck)->d : nullptr;
+}
+
+const char *sph_deck_title(void *deck)
+{
+    return deck ? ((Deck *)deck)->title.c_str() : "";
+}
+
+int sph_deck_status_interval(void *deck)
+{
+    return deck ? ((Deck *)deck)->status_interval : 100;
+}
+
+void sph_deck_free(void *deck)
+{
+    delete (Deck *)deck;
+}
+
+}  // extern "C"

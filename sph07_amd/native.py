@@ -221,3 +221,60 @@ class Scene:
         canon = np.zeros((self.np, self.maxnbr), np.int64)
         self._chk(lib().sph_get_neighbours(self._h, nn.ctypes.data, canon.ctypes.data, self.maxnbr))
         return nn, canon
+
+
+def read_mcm(path: str) -> Deck:
+    """Reads a .mcm deck with the library's reader (sph_deck_read) into a Deck."""
+    from .deck import Material
+    L = lib()
+    L.sph_deck_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p,
+                                ctypes.c_int]
+    L.sph_deck_desc.argtypes = [ctypes.c_void_p]
+    L.sph_deck_desc.restype = ctypes.POINTER(CDeck)
+    L.sph_deck_title.argtypes = [ctypes.c_void_p]
+    L.sph_deck_title.restype = ctypes.c_char_p
+    L.sph_deck_free.argtypes = [ctypes.c_void_p]
+    L.sph_deck_free.restype = None
+    h = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(512)
+    rc = L.sph_deck_read(path.encode(), ctypes.byref(h), err, 512)
+    if rc != 0:
+        raise SphError(rc, err.value.decode())
+    try:
+        d = L.sph_deck_desc(h).contents
+        n = d.np
+
+        def arr(ptr, count, dtype):
+            if not ptr:
+                return None
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype).copy()
+
+        mats = []
+        for k in range(d.nummat):
+            m = d.mat[k]
+            mats.append(Material(model=m.model, eos=m.eos, visc_type=m.visc_type, rho=m.rho,
+                                 av_l=m.av_l, av_q=m.av_q, surface_tension=m.surface_tension,
+                                 mass=m.mass, h=m.h, eosinput=tuple(m.eosinput),
+                                 strinput0=m.strinput0, av_alpha=m.av_alpha))
+        deck = Deck(ndim=d.ndim, x=arr(d.x, 3 * n, np.float64).reshape(n, 3),
+                    v=arr(d.v, 3 * n, np.float64).reshape(n, 3),
+                    mat_id=arr(d.mat_id, n, np.int32), materials=mats,
+                    color=arr(d.color, n, np.int32), dispbc=arr(d.dispbc, n, np.int32),
+                    disctype=d.disctype, tcrit_opt=d.tcrit_opt, maxnbr=d.maxnbr,
+                    surface_tension=d.surface_tension, rigid_body=d.rigid_body,
+                    boundary=d.boundary, baseaccel=d.baseaccel, massopt=d.massopt,
+                    init_h_opt=d.init_h_opt, init_rhoe=d.init_rhoe, veloc_opt=d.veloc_opt,
+                    h_opt=d.h_opt, endtime=d.endtime, tssf=d.tssf, itss=d.itss, rb_id=d.rb_id,
+                    rb_total_mass=d.rb_total_mass, rb_inertia=d.rb_inertia,
+                    node_mass=arr(d.node_mass, n, np.float64),
+                    node_rho=arr(d.node_rho, n, np.float64),
+                    name=os.path.splitext(os.path.basename(path))[0])
+        deck.boundary_code = np.array([[d.boundary_code[s][m] for m in range(3)] for s in range(2)],
+                                      np.int32)
+        deck.boundary_x = np.array([[d.boundary_x[s][m] for m in range(3)] for s in range(2)])
+        deck.base_a = np.array([d.base_a[m] for m in range(3)])
+        deck.meta["title"] = L.sph_deck_title(h).decode().rstrip()
+        deck.n_fluid = int((deck.dispbc == 0).sum())
+        return deck
+    finally:
+        L.sph_deck_free(h)

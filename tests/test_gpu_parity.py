@@ -159,3 +159,16 @@ def test_fp32c_arm_error(name, steps):
     nn64, c64 = g64.neighbours()
     nn32, c32 = g32.neighbours()
     assert np.array_equal(nn64, nn32)
+
+
+@pytest.mark.parametrize("deckfile,steps", [("rigidbody.mcm", 100), ("kuglica.mcm", 100)])
+def test_reference_decks(deckfile, steps):
+    """Two of the reference's own input decks, read by the library's .mcm
+    reader: device FP64 arm against the oracle after `steps` iterations."""
+    import os
+    from sph07_amd.native import read_mcm
+    deck = read_mcm(os.path.join(os.path.dirname(__file__), "golden", deckfile))
+    o = orc.Oracle(deck)
+    g = gpu_scene(deck)
+    assert o.run(steps) == g.Solution(steps) == steps
+    compare_state(g, o, RTOL, exact=not deck.rigid_body, what=deckfile)
