@@ -50,7 +50,18 @@ struct Context {
     std::vector<void *> part_allocs, cell_allocs;
     long long *canon = nullptr;
     Timing tm;
+    // captured step sequence (hipGraph); rebuilt after any buffer change
+    hipGraphExec_t gexec = nullptr;
+    int gsteps = 0;
+    bool use_graph = true;
 };
+
+void drop_graph(Context *c)
+{
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gsteps = 0;
+}
 
 hipEvent_t take_event(Context *c)
 {
@@ -272,6 +283,7 @@ int check_device(Context *c)
             c->err = "ERROR: neighbour grid too large.";
             return SPH_ERROR;
         }
+        drop_graph(c);
         if (alloc_cells(c, (int)want)) return SPH_ERR_HIP;
         h.err = 0;
         if (push_ctl(c)) return SPH_ERR_HIP;
@@ -279,6 +291,7 @@ int check_device(Context *c)
     }
     if (h.err == kErrGhostCap) {
         int want = std::max(2 * c->P.gcap, h.err_particle + 1024);
+        drop_graph(c);
         if (grow_particles(c, want)) return SPH_ERR_HIP;
         h.err = 0;
         if (push_ctl(c)) return SPH_ERR_HIP;
@@ -642,6 +655,7 @@ void sph_destroy(void *ctx)
     Context *c = ctx_of(ctx);
     if (!c) return;
     if (c->st) hipStreamSynchronize(c->st);
+    drop_graph(c);
     free_list(c->part_allocs);
     free_list(c->cell_allocs);
     if (c->A.ctl) hipFree(c->A.ctl);
@@ -751,6 +765,26 @@ int sph_end_step(void *ctx)
     return run_phase(c, [&] { launch_finish(c->P, c->A, 0, 1, c->st); });
 }
 
+// Captures kGraphSteps fused steps on the context's stream into one graph.
+constexpr int kGraphSteps = 16;
+
+int build_graph(Context *c)
+{
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kGraphSteps; k++) fused_step(c);
+    HIPCHK(c, hipStreamEndCapture(c->st, &g));
+    hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->gexec = nullptr;
+        c->use_graph = false;   // fall back to direct launches of the same kernels
+        return 0;
+    }
+    c->gsteps = kGraphSteps;
+    return 0;
+}
+
 int sph_solution(void *ctx, int max_steps, int *steps_done)
 {
     Context *c = ctx_of(ctx);
@@ -761,7 +795,14 @@ int sph_solution(void *ctx, int max_steps, int *steps_done)
     int guard = 0;
     while (remaining > 0 && !c->hctl->done) {
         const int chunk = std::min(remaining, 64);
-        for (int k = 0; k < chunk; k++) fused_step(c);
+        int k = 0;
+        if (c->use_graph && !c->tm.on && chunk >= kGraphSteps) {
+            if (!c->gexec && build_graph(c)) return SPH_ERR_HIP;
+            if (c->gexec)
+                for (; k + c->gsteps <= chunk; k += c->gsteps)
+                    HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+        }
+        for (; k < chunk; k++) fused_step(c);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             c->err = std::string("HIP launch error: ") + hipGetErrorString(e);

@@ -125,8 +125,13 @@ def test_fused_steps(name, steps):
     for k in ("dt", "ptime", "cmm_min0", "cmm_max0", "cmm_min1", "cmm_max1"):
         assert abs(gg[k] - go[k]) <= RTOL * max(abs(go[k]), 1e-300), (k, gg[k], go[k])
     if deck.rigid_body:
-        for k in ("rbR0", "rbR1", "rbV0", "rbV1", "rbW2"):
-            assert abs(gg[k] - go[k]) <= 1e-9 * max(abs(go[k]), 1e-12), (k, gg[k], go[k])
+        # scales: |R| for the centre, |V| for velocities, |V|/radius for the
+        # spin (zero up to rounding for this symmetric body)
+        R = np.hypot(go["rbR0"], go["rbR1"])
+        V = max(np.hypot(go["rbV0"], go["rbV1"]), 1e-300)
+        scale = {"rbR0": R, "rbR1": R, "rbV0": V, "rbV1": V, "rbW2": V / deck.meta["radius"]}
+        for k, sc in scale.items():
+            assert abs(gg[k] - go[k]) <= 1e-9 * sc, (k, gg[k], go[k])
 
 
 def test_neighbour_overflow_is_reported():
