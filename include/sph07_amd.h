@@ -152,6 +152,33 @@ int  sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr);
 /* Blocks until all work queued on the context's stream has finished. */
 int  sph_synchronize(void *ctx);
 
+/* --- batched scenes (no reference counterpart: the reference runs one scene
+   per process, MSolution.h:16-22 singletons).  Many independent scenes of one
+   ndim and arm are advanced together: every step is one sequence of kernel
+   launches over all live scenes (per-scene parameter tables and block maps),
+   captured in a HIP graph.  Each scene keeps its own dt, step counter and
+   state, so its results do not depend on which other scenes share the batch. --- */
+int  sph_batch_create(int ndim, int arm, void **batch);
+void sph_batch_destroy(void *batch);
+const char *sph_batch_last_error(void *batch);
+/* Initialises a scene (as sph_create) and makes it live; step_limit > 0 stops
+   it after that many steps.  *slot identifies it until sph_batch_remove. */
+int  sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int *slot,
+                   char *err, int errlen);
+int  sph_batch_remove(void *batch, int slot);
+/* Advances every live scene by up to nsteps steps; *live receives the number
+   of scenes still running. */
+int  sph_batch_step(void *batch, int nsteps, int *live);
+int  sph_batch_status(void *batch, int slot, sph_globals *out);
+/* The scene context of a slot, for sph_download / sph_get_neighbours. */
+void *sph_batch_scene(void *batch, int slot);
+int  sph_batch_live(void *batch);
+/* Writes up to max slots that are resident but stopped (finished or failed);
+   returns the count. */
+int  sph_batch_finished(void *batch, int *slots, int max);
+/* Device memory a scene of this deck will hold, in bytes (admission control). */
+long long sph_scene_footprint(const sph_scene_desc *desc);
+
 /* --- .mcm deck reader (MSimulationData::GetInput, MSimulationData.cpp:186-1068) --- */
 /* Reads a deck into an owned descriptor; *deck is freed with sph_deck_free. */
 int  sph_deck_read(const char *path, void **deck, char *err, int errlen);

@@ -966,3 +966,358 @@ int sph_get_timing(void *ctx, int slot, char *name, int namelen, double *total_m
 }
 
 }  // extern "C"
+
+// ============================ batched scenes =================================
+// Many independent scenes advanced by one launch sequence per step (the
+// ensemble and serving workloads).  Each scene keeps its own device state
+// and control block; the batch holds device tables of their parameter and
+// array blocks and the block maps of the batched kernels (launch_batch_step).
+namespace {
+
+struct Batch {
+    hipStream_t st = nullptr;
+    int ndim = 2, arm = SPH_ARM_FP64;
+    std::vector<Context *> slots;          // nullptr = free slot
+    std::vector<int> live;                 // resident, not finished, no error
+    Params *dP = nullptr;
+    Arrays *dA = nullptr;
+    int cap_slots = 0;
+    int *dscenes = nullptr;
+    int2 *dpmap = nullptr, *dtmap = nullptr;
+    int cap_scenes = 0, cap_pmap = 0, cap_tmap = 0;
+    Ctl *dctl = nullptr, *hctl = nullptr; // gathered control blocks
+    int cap_ctl = 0;
+    BatchView view{};
+    bool dirty = true;
+    hipGraphExec_t gexec = nullptr;
+    int gsteps = 0;
+    std::string err;
+};
+
+#define BCHK(b, expr)                                                               \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            (b)->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr; \
+            return SPH_ERR_HIP;                                                     \
+        }                                                                           \
+    } while (0)
+
+void batch_drop_graph(Batch *b)
+{
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    b->gsteps = 0;
+}
+
+template <class T>
+int grow_dev(Batch *b, T **p, int *cap, int need)
+{
+    if (need <= *cap) return 0;
+    int nc = std::max(need, 2 * *cap + 16);
+    if (*p) BCHK(b, hipFree(*p));
+    BCHK(b, hipMalloc((void **)p, (size_t)nc * sizeof(T)));
+    *cap = nc;
+    return 0;
+}
+
+// Rebuilds the scene list, the block maps and the per-slot tables.
+int batch_rebuild(Batch *b)
+{
+    BCHK(b, hipStreamSynchronize(b->st));
+    batch_drop_graph(b);
+    const int nslot = (int)b->slots.size();
+    if (grow_dev(b, &b->dP, &b->cap_slots, nslot)) return SPH_ERR_HIP;
+    {
+        int cap2 = b->cap_slots - 1;
+        (void)cap2;
+    }
+    // Arrays table has the same capacity as the Params table
+    static_assert(sizeof(Arrays) > 0, "");
+    std::vector<int> scenes;
+    std::vector<int2> pmap, tmap;
+    int any_p = 0, any_st = 0, any_r = 0, mask = 0;
+    for (int s : b->live) {
+        Context *c = b->slots[s];
+        scenes.push_back(s);
+        for (int lb = 0; lb < (c->N + kBlock - 1) / kBlock; lb++) pmap.push_back(make_int2(s, lb));
+        for (int t = 0; t < (c->P.ncell_cap + kScanTileItems - 1) / kScanTileItems; t++) tmap.push_back(make_int2(s, t));
+        any_p |= c->P.periodic;
+        any_st |= c->P.st;
+        any_r |= c->P.rigid;
+        for (int m = 0; m < 3; m++)
+            for (int side = 0; side < 2; side++) {
+                const int code = (m == 0) ? c->P.bcode[0][0] : c->P.bcode[side][m];
+                if (c->P.periodic && code == 2) mask |= 1 << (2 * m + side);
+            }
+    }
+    if (grow_dev(b, &b->dscenes, &b->cap_scenes, (int)scenes.size() + 1)) return SPH_ERR_HIP;
+    if (grow_dev(b, &b->dpmap, &b->cap_pmap, (int)pmap.size() + 1)) return SPH_ERR_HIP;
+    if (grow_dev(b, &b->dtmap, &b->cap_tmap, (int)tmap.size() + 1)) return SPH_ERR_HIP;
+    if (grow_dev(b, &b->dctl, &b->cap_ctl, nslot + 1)) return SPH_ERR_HIP;
+    if (b->hctl) hipHostFree(b->hctl);
+    BCHK(b, hipHostMalloc((void **)&b->hctl, (size_t)b->cap_ctl * sizeof(Ctl), hipHostMallocDefault));
+    // parameter / array tables indexed by slot
+    static thread_local std::vector<Params> hp;
+    static thread_local std::vector<Arrays> ha;
+    hp.assign(nslot, Params{});
+    ha.assign(nslot, Arrays{});
+    for (int s = 0; s < nslot; s++)
+        if (b->slots[s]) {
+            hp[s] = b->slots[s]->P;
+            ha[s] = b->slots[s]->A;
+        }
+    if (b->dA) BCHK(b, hipFree(b->dA));
+    BCHK(b, hipMalloc((void **)&b->dA, (size_t)std::max(nslot, 1) * sizeof(Arrays)));
+    if (nslot) {
+        BCHK(b, hipMemcpy(b->dP, hp.data(), nslot * sizeof(Params), hipMemcpyHostToDevice));
+        BCHK(b, hipMemcpy(b->dA, ha.data(), nslot * sizeof(Arrays), hipMemcpyHostToDevice));
+    }
+    if (!scenes.empty()) {
+        BCHK(b, hipMemcpy(b->dscenes, scenes.data(), scenes.size() * sizeof(int), hipMemcpyHostToDevice));
+        BCHK(b, hipMemcpy(b->dpmap, pmap.data(), pmap.size() * sizeof(int2), hipMemcpyHostToDevice));
+        BCHK(b, hipMemcpy(b->dtmap, tmap.data(), tmap.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    BatchView &v = b->view;
+    v.P = b->dP;
+    v.A = b->dA;
+    v.scenes = b->dscenes;
+    v.pmap = b->dpmap;
+    v.tmap = b->dtmap;
+    v.nscene = (int)scenes.size();
+    v.npart_blocks = (int)pmap.size();
+    v.ntile_blocks = (int)tmap.size();
+    v.zero_nb = 64;
+    v.slab_nb = 8;
+    v.ndim = b->ndim;
+    v.arm = b->arm;
+    v.any_periodic = any_p;
+    v.any_st = any_st;
+    v.any_rigid = any_r;
+    v.pass_mask = mask;
+    b->dirty = false;
+    return 0;
+}
+
+// Gathers every live scene's control block, repairs capacity errors and
+// drops finished or failed scenes from the live set.
+int batch_check(Batch *b)
+{
+    const int ns = (int)b->live.size();
+    if (ns == 0) return 0;
+    launch_gather_ctl(b->dA, b->dscenes, ns, b->dctl, b->st);
+    BCHK(b, hipMemcpyAsync(b->hctl, b->dctl, ns * sizeof(Ctl), hipMemcpyDeviceToHost, b->st));
+    BCHK(b, hipStreamSynchronize(b->st));
+    std::vector<int> keep;
+    for (int k = 0; k < ns; k++) {
+        const int s = b->live[k];
+        Context *c = b->slots[s];
+        *c->hctl = b->hctl[k];
+        Ctl &h = *c->hctl;
+        if (h.err == kErrGridCap || h.err == kErrGhostCap) {
+            int rc = 0;
+            if (h.err == kErrGridCap) {
+                long long want = (long long)h.ncell * 3 / 2 + 1024;
+                if (want > 0x7fffffffLL - 1) rc = SPH_ERROR;
+                else rc = alloc_cells(c, (int)want);
+            } else {
+                rc = grow_particles(c, std::max(2 * c->P.gcap, h.err_particle + 1024));
+            }
+            if (rc == 0) {
+                h.err = 0;
+                rc = push_ctl(c);
+            }
+            b->dirty = true;
+            if (rc) {
+                c->err = "capacity repair failed";
+                continue;
+            }
+            keep.push_back(s);
+            continue;
+        }
+        if (h.err) {
+            c->err = err_text(h.err);
+            b->dirty = true;
+            continue;
+        }
+        if (h.done) {
+            b->dirty = true;
+            continue;
+        }
+        keep.push_back(s);
+    }
+    b->live.swap(keep);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_batch_create(int ndim, int arm, void **out)
+{
+    if (out) *out = nullptr;
+    if (ndim < 1 || ndim > 3 || (arm != SPH_ARM_FP64 && arm != SPH_ARM_FP32C)) return SPH_ERROR;
+    Batch *b = new Batch();
+    b->ndim = ndim;
+    b->arm = arm;
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return SPH_ERR_HIP;
+    }
+    *out = b;
+    return SPH_OK;
+}
+
+const char *sph_batch_last_error(void *batch)
+{
+    return batch ? ((Batch *)batch)->err.c_str() : "null batch";
+}
+
+// Bytes of device memory a scene of this deck will hold (admission control).
+long long sph_scene_footprint(const sph_scene_desc *d)
+{
+    const long long n = d->np;
+    const long long D = d->ndim;
+    const long long maxnbr = d->maxnbr > 0 ? d->maxnbr : 40;
+    const long long per = 8 * (4 * D) + 8 * 6 + 4 + 8 * 4 + 4 * maxnbr + 4 * 6 + (d->surface_tension ? 8 * (2 * D + 2) : 0);
+    const long long img = d->boundary ? n / 4 + 4096 : 0;
+    return (n + img) * per + n * 8 /* cells, upper bound */ + 65536;
+}
+
+int sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int *slot, char *err, int errlen)
+{
+    Batch *b = (Batch *)batch;
+    if (slot) *slot = -1;
+    if (desc->ndim != b->ndim) {
+        if (err && errlen > 0) snprintf(err, (size_t)errlen, "scene ndim differs from the batch");
+        return SPH_ERROR;
+    }
+    BCHK(b, hipStreamSynchronize(b->st));
+    void *ctx = nullptr;
+    int rc = sph_create(desc, b->arm, &ctx, err, errlen);
+    if (rc) return rc;
+    Context *c = ctx_of(ctx);
+    c->hctl->step_limit = step_limit;
+    if (push_ctl(c)) {
+        if (err && errlen > 0) snprintf(err, (size_t)errlen, "%s", c->err.c_str());
+        sph_destroy(c);
+        return SPH_ERR_HIP;
+    }
+    int s = -1;
+    for (int k = 0; k < (int)b->slots.size(); k++)
+        if (!b->slots[k]) { s = k; break; }
+    if (s < 0) {
+        s = (int)b->slots.size();
+        b->slots.push_back(nullptr);
+    }
+    b->slots[s] = c;
+    b->live.push_back(s);
+    b->dirty = true;
+    if (slot) *slot = s;
+    return SPH_OK;
+}
+
+int sph_batch_remove(void *batch, int slot)
+{
+    Batch *b = (Batch *)batch;
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return SPH_ERROR;
+    BCHK(b, hipStreamSynchronize(b->st));
+    sph_destroy(b->slots[slot]);
+    b->slots[slot] = nullptr;
+    b->live.erase(std::remove(b->live.begin(), b->live.end(), slot), b->live.end());
+    b->dirty = true;
+    return SPH_OK;
+}
+
+// Advances every live scene by up to nsteps steps; scenes stop on their own
+// step limit or stop test.  Returns the number of scenes still live in *live.
+int sph_batch_step(void *batch, int nsteps, int *live)
+{
+    Batch *b = (Batch *)batch;
+    constexpr int kBatchGraphSteps = 8;
+    int done_steps = 0;
+    while (done_steps < nsteps && !b->live.empty()) {
+        if (b->dirty && batch_rebuild(b)) return SPH_ERR_HIP;
+        const int chunk = std::min(nsteps - done_steps, 32);
+        int k = 0;
+        if (chunk >= kBatchGraphSteps) {
+            if (!b->gexec) {
+                hipGraph_t g = nullptr;
+                BCHK(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+                for (int q = 0; q < kBatchGraphSteps; q++) launch_batch_step(b->view, b->st);
+                BCHK(b, hipStreamEndCapture(b->st, &g));
+                if (hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) != hipSuccess) b->gexec = nullptr;
+                hipGraphDestroy(g);
+                b->gsteps = b->gexec ? kBatchGraphSteps : 0;
+            }
+            if (b->gexec)
+                for (; k + b->gsteps <= chunk; k += b->gsteps) BCHK(b, hipGraphLaunch(b->gexec, b->st));
+        }
+        for (; k < chunk; k++) launch_batch_step(b->view, b->st);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            b->err = std::string("HIP launch error: ") + hipGetErrorString(e);
+            return SPH_ERR_HIP;
+        }
+        done_steps += chunk;
+        if (batch_check(b)) return SPH_ERR_HIP;
+    }
+    if (live) *live = (int)b->live.size();
+    return SPH_OK;
+}
+
+// Status of one slot: its globals (timestep, ptime, done, error).
+int sph_batch_status(void *batch, int slot, sph_globals *out)
+{
+    Batch *b = (Batch *)batch;
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return SPH_ERROR;
+    BCHK(b, hipStreamSynchronize(b->st));
+    return sph_get_globals(b->slots[slot], out);
+}
+
+// The scene context of a slot, for sph_download / sph_get_neighbours.
+void *sph_batch_scene(void *batch, int slot)
+{
+    Batch *b = (Batch *)batch;
+    if (slot < 0 || slot >= (int)b->slots.size()) return nullptr;
+    hipStreamSynchronize(b->st);
+    return b->slots[slot];
+}
+
+// Slots that are resident but no longer live (finished or failed), so the
+// caller can retire them; returns how many were written.
+int sph_batch_finished(void *batch, int *slots, int max)
+{
+    Batch *b = (Batch *)batch;
+    int n = 0;
+    for (int s = 0; s < (int)b->slots.size() && n < max; s++)
+        if (b->slots[s] && std::find(b->live.begin(), b->live.end(), s) == b->live.end()) slots[n++] = s;
+    return n;
+}
+
+int sph_batch_live(void *batch)
+{
+    return (int)((Batch *)batch)->live.size();
+}
+
+void sph_batch_destroy(void *batch)
+{
+    Batch *b = (Batch *)batch;
+    if (!b) return;
+    hipStreamSynchronize(b->st);
+    batch_drop_graph(b);
+    for (Context *c : b->slots)
+        if (c) sph_destroy(c);
+    hipFree(b->dP);
+    hipFree(b->dA);
+    hipFree(b->dscenes);
+    hipFree(b->dpmap);
+    hipFree(b->dtmap);
+    hipFree(b->dctl);
+    if (b->hctl) hipHostFree(b->hctl);
+    hipStreamDestroy(b->st);
+    delete b;
+}
+
+}  // extern "C"

@@ -106,11 +106,11 @@ __device__ __forceinline__ double wave_min32(double v)
 
 // continuity sweep + RhoUpdate/EOS + critical step (fused mode of k_strain)
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
+__device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, const int bid)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP32(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     double crit = 1.0e20;
     if (i < P.N) {
         const uint32_t fi = A.info[i];
@@ -174,11 +174,11 @@ __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
 
 // momentum sweep: pressure gradient, Morris viscosity, Monaghan term
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
+__device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP32(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const uint32_t fi = A.info[i];
     double ai[D];
@@ -250,6 +250,37 @@ __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
     }
 #pragma unroll
     for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
+}
+
+template <int D> __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A) { strain32_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Params *Pt, const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    strain32_body<D>(Pt[e.x], At[e.x], e.y);
+}
+template <int D> __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A) { momentum32_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *Pt, const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    momentum32_body<D>(Pt[e.x], At[e.x], e.y);
+}
+
+void launch_batch_strain32(const BatchView &B, hipStream_t st)
+{
+    switch (B.ndim) {
+    case 1: hipLaunchKernelGGL(kb_strain32<1>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
+    case 2: hipLaunchKernelGGL(kb_strain32<2>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
+    default: hipLaunchKernelGGL(kb_strain32<3>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
+    }
+}
+
+void launch_batch_momentum32(const BatchView &B, hipStream_t st)
+{
+    switch (B.ndim) {
+    case 1: hipLaunchKernelGGL(kb_momentum32<1>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
+    case 2: hipLaunchKernelGGL(kb_momentum32<2>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
+    default: hipLaunchKernelGGL(kb_momentum32<3>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
+    }
 }
 
 static inline int nblk32(int n) { return (n + kBlock - 1) / kBlock; }

@@ -105,7 +105,7 @@ __device__ __forceinline__ void gradw64(double dw[D], const double *xi, const do
 // ---------------------------------------------------------------------------
 // grid: MNeighbourVars::InitGrid (MNeighbourVars.cpp:18-66)
 // ---------------------------------------------------------------------------
-__global__ void k_grid_setup(Params P, Ctl *ctl)
+__device__ __forceinline__ void grid_setup_body(const Params &P, Ctl *ctl)
 {
     if (CTL_STOP(ctl)) return;
     long long ncell = 1;
@@ -136,12 +136,12 @@ __global__ void k_grid_setup(Params P, Ctl *ctl)
 }
 
 // cell_count[0..ncell] and image counts cleared
-__global__ void k_zero_cells(Params P, Arrays A)
+__device__ __forceinline__ void zero_cells_body(const Params &P, const Arrays &A, const int bid, const int nb)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
     const int n = ctl->ncell;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= n; c += gridDim.x * blockDim.x) {
+    for (int c = bid * kBlock + threadIdx.x; c <= n; c += nb * kBlock) {
         A.cell_count[c] = 0;
         if (P.periodic && c < n) A.gcount[c] = 0;
     }
@@ -151,11 +151,11 @@ __global__ void k_zero_cells(Params P, Arrays A)
 // particle (MNeighbourVars::GetBoxCoords :83-89, truncation, no clamp) and an
 // arbitrary slot inside the cell; the order inside a cell is fixed by k_bin_rank.
 template <int D>
-__global__ void k_bin_count(Params P, Arrays A)
+__device__ __forceinline__ void bin_count_body(const Params &P, const Arrays &A, const int bid)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     uint32_t f = A.info[i];
     if (!(f & kActive)) { A.cell_of[i] = -1; return; }
@@ -179,10 +179,10 @@ __global__ void k_bin_count(Params P, Arrays A)
     A.slot[i] = atomicAdd(&A.cell_count[c], 1);
 }
 
-__global__ void k_bin_scatter(Params P, Arrays A)
+__device__ __forceinline__ void bin_scatter_body(const Params &P, const Arrays &A, const int bid)
 {
     if (CTL_STOP(A.ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const int c = A.cell_of[i];
     if (c < 0) return;
@@ -191,10 +191,10 @@ __global__ void k_bin_scatter(Params P, Arrays A)
 
 // Order inside a cell: descending particle number, the order head insertion
 // leaves in the reference's linked lists (MSolution.cpp:143-144).
-__global__ void k_bin_rank(Params P, Arrays A)
+__device__ __forceinline__ void bin_rank_body(const Params &P, const Arrays &A, const int bid)
 {
     if (CTL_STOP(A.ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const int c = A.cell_of[i];
     if (c < 0) return;
@@ -210,21 +210,21 @@ __global__ void k_bin_rank(Params P, Arrays A)
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
-__global__ void k_scan_reduce(const int *in, const int *n_ptr, int *aux, const Ctl *ctl)
+__device__ __forceinline__ void scan_reduce_body(const int *in, const int *n_ptr, int *aux, const Ctl *ctl, const int bid)
 {
     if (CTL_STOP(ctl)) return;
     const int n = *n_ptr;
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const int base = bid * kScanTile + threadIdx.x * kScanItems;
     int s = 0;
 #pragma unroll
     for (int q = 0; q < kScanItems; q++) s += (base + q < n) ? in[base + q] : 0;
     int total;
     block_excl_scan(s, &total);
-    if (threadIdx.x == 0) aux[blockIdx.x] = total;
+    if (threadIdx.x == 0) aux[bid] = total;
 }
 
 // single block: exclusive scan of the tile sums, total written to out[n]
-__global__ void k_scan_aux(int *aux, int nblocks, const int *n_ptr, int *out, const Ctl *ctl)
+__device__ __forceinline__ void scan_aux_body(int *aux, int nblocks, const int *n_ptr, int *out, const Ctl *ctl)
 {
     if (CTL_STOP(ctl)) return;
     int carry = 0;
@@ -240,11 +240,11 @@ __global__ void k_scan_aux(int *aux, int nblocks, const int *n_ptr, int *out, co
     if (threadIdx.x == 0) out[*n_ptr] = carry;
 }
 
-__global__ void k_scan_down(const int *in, int *out, const int *n_ptr, const int *aux, const Ctl *ctl)
+__device__ __forceinline__ void scan_down_body(const int *in, int *out, const int *n_ptr, const int *aux, const Ctl *ctl, const int bid)
 {
     if (CTL_STOP(ctl)) return;
     const int n = *n_ptr;
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const int base = bid * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems], s = 0;
 #pragma unroll
     for (int q = 0; q < kScanItems; q++) {
@@ -252,7 +252,7 @@ __global__ void k_scan_down(const int *in, int *out, const int *n_ptr, const int
         s += v[q];
     }
     int total;
-    int ex = block_excl_scan(s, &total) + aux[blockIdx.x];
+    int ex = block_excl_scan(s, &total) + aux[bid];
 #pragma unroll
     for (int q = 0; q < kScanItems; q++) {
         if (base + q < n) out[base + q] = ex;
@@ -294,20 +294,20 @@ __device__ __forceinline__ void slab_coords(const PassGeom &g, int s, int &i, in
 
 // A source cell contributes only when its list head is a real particle:
 // it holds particles and no image yet (`if (id<0) continue`, :906-909).
-__global__ void k_reset_ngp(Ctl *ctl)
+__device__ __forceinline__ void reset_ngp_body(Ctl *ctl)
 {
     if (CTL_STOP(ctl)) return;
     ctl->ngp = 0;   // MGhostParticleData::DeleteContents (MGhostParticleData.cpp:48-53)
 }
 
-__global__ void k_ghost_count(Params P, Arrays A, int m, int side)
+__device__ __forceinline__ void ghost_count_body(const Params &P, const Arrays &A, int m, int side, const int bid, const int nb)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
     const double *add = side == 0 ? &P.minadd[0][m] : &P.maxadd[0][m];
     double addv[3] = {add[0], add[3], add[6]};
     PassGeom g = pass_geom(ctl, m, side, addv);
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.n; s += gridDim.x * blockDim.x) {
+    for (int s = bid * kBlock + threadIdx.x; s < g.n; s += nb * kBlock) {
         int i, j, k;
         slab_coords(g, s, i, j, k);
         const int c = ctl->glim[2] * (ctl->glim[1] * i + j) + k;
@@ -316,7 +316,7 @@ __global__ void k_ghost_count(Params P, Arrays A, int m, int side)
     }
 }
 
-__global__ void k_ghost_scan(Params P, Arrays A, int m, int side)
+__device__ __forceinline__ void ghost_scan_body(const Params &P, const Arrays &A, int m, int side)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
@@ -339,7 +339,7 @@ __global__ void k_ghost_scan(Params P, Arrays A, int m, int side)
     }
 }
 
-__global__ void k_ghost_emit(Params P, Arrays A, int m, int side)
+__device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A, int m, int side, const int bid, const int nb)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
@@ -349,7 +349,7 @@ __global__ void k_ghost_emit(Params P, Arrays A, int m, int side)
     int shift[3];
     for (int d = 0; d < 3; d++) shift[d] = addv[d] != 0.0 ? (addv[d] >= 0 ? 1 : -1) : 0;
     const int shift_code = (shift[0] + 1) * 9 + (shift[1] + 1) * 3 + (shift[2] + 1);
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.n; s += gridDim.x * blockDim.x) {
+    for (int s = bid * kBlock + threadIdx.x; s < g.n; s += nb * kBlock) {
         const int cnt = A.slab_cnt[s];
         if (cnt == 0) continue;
         int i, j, k;
@@ -396,11 +396,11 @@ __global__ void k_ghost_emit(Params P, Arrays A, int m, int side)
 // MSolution::Strain (:334-338) for the fused step.
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void k_neighbours(Params P, Arrays A, int shift)
+__device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A, int shift, const int bid)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const uint32_t fi = A.info[i];
     const double mi = A.m[i];
@@ -470,11 +470,11 @@ __global__ void k_neighbours(Params P, Arrays A, int shift)
 
 // MSolution::Strain half-step back (MSolution.cpp:334-338), unfused path
 template <int D>
-__global__ void k_shift(Params P, Arrays A)
+__device__ __forceinline__ void shift_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const bool act = A.info[i] & kActive;
     const double dt = ctl->dt;
@@ -546,11 +546,11 @@ __device__ __forceinline__ void block_min_to(unsigned long long *key, double v)
 enum { S_STRAIN = 1, S_CONST = 2, S_CRIT = 4, S_INPLACE = 8 };
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask)
+__device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, int mask, const int bid)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     double crit = 1.0e20;
     if (i < P.N) {
         const uint32_t fi = A.info[i];
@@ -614,11 +614,11 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask)
 // and its normalised form, then the divergence of the normalised normal.
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_normals(Params P, Arrays A)
+__device__ __forceinline__ void normals_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const uint32_t fi = A.info[i];
     if (!(fi & kActive)) return;
@@ -658,11 +658,11 @@ __global__ __launch_bounds__(kBlock) void k_normals(Params P, Arrays A)
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_curvature(Params P, Arrays A)
+__device__ __forceinline__ void curvature_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const uint32_t fi = A.info[i];
     if (!(fi & kActive)) return;
@@ -697,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_curvature(Params P, Arrays A)
 }
 
 // dt rule of MSolution::Solution (:275-277); resets the min key
-__global__ void k_advance(Params P, Ctl *ctl)
+__device__ __forceinline__ void advance_body(const Params &P, Ctl *ctl)
 {
     if (CTL_STOP(ctl)) return;
     const double critts = dunkey(ctl->critts_key);
@@ -715,11 +715,11 @@ __global__ void k_advance(Params P, Ctl *ctl)
 // approaching pairs (extension; the reference's visc_type field is unused).
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A)
+__device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const uint32_t fi = A.info[i];
     double ai[D];
@@ -810,7 +810,7 @@ __device__ __forceinline__ double block_sum_big(double v, double *sh)
     return s;   // valid in thread 0
 }
 
-__global__ __launch_bounds__(kRigidBlock) void k_rigid_reduce(Params P, Arrays A, int kin)
+__device__ __forceinline__ void rigid_reduce_body(const Params &P, const Arrays &A, int kin)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
@@ -864,7 +864,7 @@ __global__ __launch_bounds__(kRigidBlock) void k_rigid_reduce(Params P, Arrays A
 }
 
 // first half of MSolution::RigidBodyVelocity (:1567-1577), unfused path
-__global__ void k_rigid_kin(Ctl *ctl)
+__device__ __forceinline__ void rigid_kin_body(Ctl *ctl)
 {
     if (CTL_STOP(ctl)) return;
     const double dtn = 0.5 * (ctl->dt + ctl->dtold);
@@ -896,11 +896,11 @@ __device__ __forceinline__ void constrain(double *q, int code, int D)
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_integrate(Params P, Arrays A, int mask)
+__device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A, int mask, const int bid)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * kBlock + threadIdx.x;
     double lo[D], hi[D];
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -1000,7 +1000,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Params P, Arrays A, int ma
 
 // bounding box of MoveParticles into coord_minmax; with end != 0 also the
 // step counter and stop test of MSolution::Solution (:317-319)
-__global__ void k_finish(Params P, Ctl *ctl, int bbox, int end)
+__device__ __forceinline__ void finish_body(const Params &P, Ctl *ctl, int bbox, int end)
 {
     if (CTL_STOP(ctl)) return;
     if (bbox) {
@@ -1014,7 +1014,179 @@ __global__ void k_finish(Params P, Ctl *ctl, int bbox, int end)
     if (end) {
         ctl->timestep++;
         if (ctl->ptime - ctl->dt >= P.endtime) ctl->done = 1;
+        if (ctl->step_limit > 0 && ctl->timestep - 1 >= ctl->step_limit) ctl->done = 1;
     }
+}
+
+// ---------------------------------------------------------------------------
+// kernels: single-scene form (parameters by value) and batched form (one
+// launch over many scenes; block b works for scene map[b].x on its local
+// block map[b].y, so every block belongs to one scene and per-scene
+// parameters are wave-uniform loads from the scene tables).
+// ---------------------------------------------------------------------------
+#define SCENE_OF(e) const Params &Pb = Pt[(e)]; const Arrays &Ab = At[(e)]
+
+__global__ void k_grid_setup(Params P, Ctl *ctl) { grid_setup_body(P, ctl); }
+__global__ void kb_grid_setup(const Params *Pt, const Arrays *At, const int *sc, int ns)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ns) { SCENE_OF(sc[t]); grid_setup_body(Pb, Ab.ctl); }
+}
+__global__ void k_zero_cells(Params P, Arrays A) { zero_cells_body(P, A, blockIdx.x, gridDim.x); }
+__global__ void kb_zero_cells(const Params *Pt, const Arrays *At, const int *sc, int nb)
+{
+    SCENE_OF(sc[blockIdx.x / nb]);
+    zero_cells_body(Pb, Ab, blockIdx.x % nb, nb);
+}
+template <int D> __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, Arrays A) { bin_count_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_bin_count(const Params *Pt, const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    bin_count_body<D>(Pb, Ab, e.y);
+}
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(Params P, Arrays A) { bin_scatter_body(P, A, blockIdx.x); }
+__global__ __launch_bounds__(kBlock) void kb_bin_scatter(const Params *Pt, const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    bin_scatter_body(Pb, Ab, e.y);
+}
+__global__ __launch_bounds__(kBlock) void k_bin_rank(Params P, Arrays A) { bin_rank_body(P, A, blockIdx.x); }
+__global__ __launch_bounds__(kBlock) void kb_bin_rank(const Params *Pt, const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    bin_rank_body(Pb, Ab, e.y);
+}
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *in, const int *n_ptr, int *aux, const Ctl *ctl)
+{
+    scan_reduce_body(in, n_ptr, aux, ctl, blockIdx.x);
+}
+__global__ __launch_bounds__(kBlock) void kb_scan_reduce(const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    const Arrays &Ab = At[e.x];
+    scan_reduce_body(Ab.cell_count, &Ab.ctl->ncell, Ab.scan_aux, Ab.ctl, e.y);
+}
+__global__ __launch_bounds__(kBlock) void k_scan_aux(int *aux, int nblocks, const int *n_ptr, int *out, const Ctl *ctl)
+{
+    scan_aux_body(aux, nblocks, n_ptr, out, ctl);
+}
+__global__ __launch_bounds__(kBlock) void kb_scan_aux(const Params *Pt, const Arrays *At, const int *sc)
+{
+    SCENE_OF(sc[blockIdx.x]);
+    const int nb = (Pb.ncell_cap + kScanTile - 1) / kScanTile;
+    scan_aux_body(Ab.scan_aux, nb, &Ab.ctl->ncell, Ab.cell_start, Ab.ctl);
+}
+__global__ __launch_bounds__(kBlock) void k_scan_down(const int *in, int *out, const int *n_ptr, const int *aux, const Ctl *ctl)
+{
+    scan_down_body(in, out, n_ptr, aux, ctl, blockIdx.x);
+}
+__global__ __launch_bounds__(kBlock) void kb_scan_down(const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    const Arrays &Ab = At[e.x];
+    scan_down_body(Ab.cell_count, Ab.cell_start, &Ab.ctl->ncell, Ab.scan_aux, Ab.ctl, e.y);
+}
+__global__ void k_reset_ngp(Ctl *ctl) { reset_ngp_body(ctl); }
+__global__ void kb_reset_ngp(const Arrays *At, const int *sc, int ns)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ns) reset_ngp_body(At[sc[t]].ctl);
+}
+__global__ __launch_bounds__(kBlock) void k_ghost_count(Params P, Arrays A, int m, int side)
+{
+    ghost_count_body(P, A, m, side, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_ghost_scan(Params P, Arrays A, int m, int side) { ghost_scan_body(P, A, m, side); }
+__global__ __launch_bounds__(kBlock) void k_ghost_emit(Params P, Arrays A, int m, int side)
+{
+    ghost_emit_body(P, A, m, side, blockIdx.x, gridDim.x);
+}
+// batched image passes: scenes without this periodic side skip the pass
+__device__ __forceinline__ bool has_pass(const Params &P, int m, int side)
+{
+    if (!P.periodic) return false;
+    const int code = (m == 0) ? P.bcode[0][0] : P.bcode[side][m];
+    return code == 2;
+}
+__global__ __launch_bounds__(kBlock) void kb_ghost_count(const Params *Pt, const Arrays *At, const int *sc, int nb, int m, int side)
+{
+    SCENE_OF(sc[blockIdx.x / nb]);
+    if (has_pass(Pb, m, side)) ghost_count_body(Pb, Ab, m, side, blockIdx.x % nb, nb);
+}
+__global__ __launch_bounds__(kBlock) void kb_ghost_scan(const Params *Pt, const Arrays *At, const int *sc, int m, int side)
+{
+    SCENE_OF(sc[blockIdx.x]);
+    if (has_pass(Pb, m, side)) ghost_scan_body(Pb, Ab, m, side);
+}
+__global__ __launch_bounds__(kBlock) void kb_ghost_emit(const Params *Pt, const Arrays *At, const int *sc, int nb, int m, int side)
+{
+    SCENE_OF(sc[blockIdx.x / nb]);
+    if (has_pass(Pb, m, side)) ghost_emit_body(Pb, Ab, m, side, blockIdx.x % nb, nb);
+}
+template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { neighbours_body<D>(P, A, shift, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *Pt, const Arrays *At, const int2 *map, int shift)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    neighbours_body<D>(Pb, Ab, shift, e.y);
+}
+template <int D> __global__ __launch_bounds__(kBlock) void k_shift(Params P, Arrays A) { shift_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask) { strain_body<D>(P, A, mask, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_strain(const Params *Pt, const Arrays *At, const int2 *map, int mask)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    strain_body<D>(Pb, Ab, mask, e.y);
+}
+template <int D> __global__ __launch_bounds__(kBlock) void k_normals(Params P, Arrays A) { normals_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_normals(const Params *Pt, const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    if (Pb.st) normals_body<D>(Pb, Ab, e.y);
+}
+template <int D> __global__ __launch_bounds__(kBlock) void k_curvature(Params P, Arrays A) { curvature_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_curvature(const Params *Pt, const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    if (Pb.st) curvature_body<D>(Pb, Ab, e.y);
+}
+__global__ void k_advance(Params P, Ctl *ctl) { advance_body(P, ctl); }
+__global__ void kb_advance(const Params *Pt, const Arrays *At, const int *sc, int ns)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ns) { SCENE_OF(sc[t]); advance_body(Pb, Ab.ctl); }
+}
+template <int D> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { momentum_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Params *Pt, const Arrays *At, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    momentum_body<D>(Pb, Ab, e.y);
+}
+__global__ __launch_bounds__(kRigidBlock) void k_rigid_reduce(Params P, Arrays A, int kin) { rigid_reduce_body(P, A, kin); }
+__global__ __launch_bounds__(kRigidBlock) void kb_rigid_reduce(const Params *Pt, const Arrays *At, const int *sc, int kin)
+{
+    SCENE_OF(sc[blockIdx.x]);
+    if (Pb.rigid) rigid_reduce_body(Pb, Ab, kin);
+}
+__global__ void k_rigid_kin(Ctl *ctl) { rigid_kin_body(ctl); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_integrate(Params P, Arrays A, int mask) { integrate_body<D>(P, A, mask, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_integrate(const Params *Pt, const Arrays *At, const int2 *map, int mask)
+{
+    const int2 e = map[blockIdx.x];
+    SCENE_OF(e.x);
+    integrate_body<D>(Pb, Ab, mask, e.y);
+}
+__global__ void k_finish(Params P, Ctl *ctl, int bbox, int end) { finish_body(P, ctl, bbox, end); }
+__global__ void kb_finish(const Params *Pt, const Arrays *At, const int *sc, int ns, int bbox, int end)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ns) { SCENE_OF(sc[t]); finish_body(Pb, Ab.ctl, bbox, end); }
 }
 
 // canonical neighbour ids (tests and the sph_get_neighbours entry point)
@@ -1138,6 +1310,55 @@ void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStrea
 void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st)
 {
     hipLaunchKernelGGL(k_canon, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, out, maxnbr);
+}
+
+// One batched step over the scenes of a BatchView (all of one ndim and arm).
+void launch_batch_step(const BatchView &B, hipStream_t st)
+{
+    const int ns = B.nscene, sb = (ns + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(kb_grid_setup, dim3(sb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, ns);
+    hipLaunchKernelGGL(kb_zero_cells, dim3(ns * B.zero_nb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, B.zero_nb);
+    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_bin_count<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap));
+    hipLaunchKernelGGL(kb_scan_reduce, dim3(B.ntile_blocks), dim3(kBlock), 0, st, B.A, B.tmap);
+    hipLaunchKernelGGL(kb_scan_aux, dim3(ns), dim3(kBlock), 0, st, B.P, B.A, B.scenes);
+    hipLaunchKernelGGL(kb_scan_down, dim3(B.ntile_blocks), dim3(kBlock), 0, st, B.A, B.tmap);
+    hipLaunchKernelGGL(kb_bin_scatter, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap);
+    hipLaunchKernelGGL(kb_bin_rank, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap);
+    if (B.any_periodic) {
+        hipLaunchKernelGGL(kb_reset_ngp, dim3(sb), dim3(kBlock), 0, st, B.A, B.scenes, ns);
+        for (int m = 0; m < 3; m++)
+            for (int side = 0; side < 2; side++) {
+                if (!(B.pass_mask & (1 << (2 * m + side)))) continue;
+                hipLaunchKernelGGL(kb_ghost_count, dim3(ns * B.slab_nb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, B.slab_nb, m, side);
+                hipLaunchKernelGGL(kb_ghost_scan, dim3(ns), dim3(kBlock), 0, st, B.P, B.A, B.scenes, m, side);
+                hipLaunchKernelGGL(kb_ghost_emit, dim3(ns * B.slab_nb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, B.slab_nb, m, side);
+            }
+    }
+    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_neighbours<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap, 1));
+    if (B.arm == 1) launch_batch_strain32(B, st);
+    else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_strain<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap, M_STRAIN | M_CONST | M_CRIT));
+    if (B.any_st) {
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_normals<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap));
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_curvature<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap));
+    }
+    hipLaunchKernelGGL(kb_advance, dim3(sb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, ns);
+    if (B.arm == 1) launch_batch_momentum32(B, st);
+    else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_momentum<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap));
+    if (B.any_rigid) hipLaunchKernelGGL(kb_rigid_reduce, dim3(ns), dim3(kRigidBlock), 0, st, B.P, B.A, B.scenes, 1);
+    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_integrate<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap, M_BC | M_VEL | M_MOVE | M_RHO));
+    hipLaunchKernelGGL(kb_finish, dim3(sb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, ns, 1, 1);
+}
+
+// gathers the control blocks of a batch into one array for a single copy
+__global__ void kb_gather_ctl(const Arrays *At, const int *sc, int ns, Ctl *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ns) out[t] = *At[sc[t]].ctl;
+}
+
+void launch_gather_ctl(const Arrays *At, const int *sc, int ns, Ctl *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(kb_gather_ctl, dim3((ns + kBlock - 1) / kBlock), dim3(kBlock), 0, st, At, sc, ns, out);
 }
 
 }  // namespace sph

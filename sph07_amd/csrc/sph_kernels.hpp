@@ -24,6 +24,27 @@ void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st
 void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st);
 void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st);
 
+// Device tables of one batched launch: per-scene parameter and array blocks,
+// the scene list, and block maps (scene, local block) over particles and
+// scan tiles.  All scenes share ndim and arithmetic arm.
+struct BatchView {
+    const Params *P;
+    const Arrays *A;
+    const int *scenes;       // scene indices into P/A, nscene entries
+    const int2 *pmap;        // npart_blocks entries
+    const int2 *tmap;        // ntile_blocks entries (scan tiles over cells)
+    int nscene, npart_blocks, ntile_blocks;
+    int zero_nb, slab_nb;    // fixed blocks per scene for grid-stride kernels
+    int ndim, arm;
+    int any_periodic, any_st, any_rigid, pass_mask;
+};
+constexpr int kScanTileItems = kBlock * 8;
+
+void launch_batch_step(const BatchView &B, hipStream_t st);
+void launch_gather_ctl(const Arrays *At, const int *sc, int ns, Ctl *out, hipStream_t st);
+void launch_batch_strain32(const BatchView &B, hipStream_t st);
+void launch_batch_momentum32(const BatchView &B, hipStream_t st);
+
 // FP32-compensated pair sweeps (sph_fp32c.hip); fused-step forms only
 void launch_strain32(const Params &P, const Arrays &A, hipStream_t st);
 void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st);

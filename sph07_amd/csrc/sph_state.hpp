@@ -48,7 +48,8 @@ struct Ctl {
     int ncell;
     int timestep, done, err, ngp;
     int err_particle;
-    int pad_[3];
+    int step_limit;       // > 0: the scene stops after this many steps (batched runs)
+    int pad_[2];
 };
 
 // Per-material constants (MMaterial.h:17-40 after CheckDefaults).

@@ -52,7 +52,9 @@ _lib = None
 def header_symbols():
     """Names of the functions declared in include/sph07_amd.h."""
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s+\**(sph_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(
+        r"^\s*(?:int|void|long long|const char|const sph_scene_desc)\s*\**\s*(sph_\w+)\s*\(",
+        txt, re.M)))
 
 
 def lib():
@@ -278,3 +280,98 @@ def read_mcm(path: str) -> Deck:
         return deck
     finally:
         L.sph_deck_free(h)
+
+
+class Batch:
+    """Many independent scenes advanced together (sph_batch_* entry points)."""
+
+    def __init__(self, ndim: int, arm: int = ARM_FP64):
+        L = lib()
+        L.sph_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.sph_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.sph_batch_destroy.restype = None
+        L.sph_batch_last_error.argtypes = [ctypes.c_void_p]
+        L.sph_batch_last_error.restype = ctypes.c_char_p
+        L.sph_batch_add.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]
+        L.sph_batch_remove.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.sph_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.sph_batch_status.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(CGlobals)]
+        L.sph_batch_scene.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.sph_batch_scene.restype = ctypes.c_void_p
+        L.sph_batch_live.argtypes = [ctypes.c_void_p]
+        L.sph_scene_footprint.argtypes = [ctypes.c_void_p]
+        L.sph_scene_footprint.restype = ctypes.c_longlong
+        h = ctypes.c_void_p()
+        rc = L.sph_batch_create(ndim, arm, ctypes.byref(h))
+        if rc != 0:
+            raise SphError(rc, "sph_batch_create failed")
+        self._h = h
+        self.ndim = ndim
+        self.arm = arm
+        self.decks = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sph_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SphError(rc, lib().sph_batch_last_error(self._h).decode())
+
+    def add(self, deck: Deck, step_limit: int = 0) -> int:
+        c, keep = deck.to_c()
+        slot = ctypes.c_int(-1)
+        err = ctypes.create_string_buffer(512)
+        rc = lib().sph_batch_add(self._h, ctypes.byref(c), step_limit, ctypes.byref(slot), err, 512)
+        if rc != 0:
+            raise SphError(rc, err.value.decode())
+        self.decks[slot.value] = deck
+        return slot.value
+
+    def remove(self, slot: int):
+        self._chk(lib().sph_batch_remove(self._h, slot))
+        self.decks.pop(slot, None)
+
+    def step(self, nsteps: int) -> int:
+        live = ctypes.c_int(0)
+        self._chk(lib().sph_batch_step(self._h, nsteps, ctypes.byref(live)))
+        return live.value
+
+    def status(self, slot: int) -> dict:
+        g = CGlobals()
+        self._chk(lib().sph_batch_status(self._h, slot, ctypes.byref(g)))
+        return {k: getattr(g, k) for k in ("dt", "ptime", "timestep", "done", "error", "np")}
+
+    def field(self, slot: int, f: int) -> np.ndarray:
+        h = lib().sph_batch_scene(self._h, slot)
+        n = self.decks[slot].np
+        out = np.zeros((n, 3) if f in _VEC else n)
+        rc = lib().sph_download(h, f, out.ctypes.data)
+        if rc != 0:
+            raise SphError(rc, lib().sph_last_error(h).decode())
+        return out
+
+    @property
+    def live(self) -> int:
+        return lib().sph_batch_live(self._h)
+
+    def finished(self):
+        L = lib()
+        L.sph_batch_finished.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        buf = np.zeros(max(len(self.decks), 1), np.int32)
+        n = L.sph_batch_finished(self._h, buf.ctypes.data, len(buf))
+        return buf[:n].tolist()
+
+
+def footprint(deck: Deck) -> int:
+    """Device bytes a scene of this deck will hold (sph_scene_footprint)."""
+    L = lib()
+    L.sph_scene_footprint.argtypes = [ctypes.c_void_p]
+    L.sph_scene_footprint.restype = ctypes.c_longlong
+    c, keep = deck.to_c()
+    return int(L.sph_scene_footprint(ctypes.byref(c)))
