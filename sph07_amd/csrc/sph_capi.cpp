@@ -43,7 +43,10 @@ struct Timing {
 struct Context {
     hipStream_t st = nullptr;
     Params P{};
-    Arrays A{};
+    // two buffer sets of the persistent particle fields: the per-step sort
+    // moves them from AA[cur] to AA[cur ^ 1]; everything else is shared
+    Arrays AA[2]{};
+    int cur = 0;
     Ctl *hctl = nullptr;          // pinned mirror of the control block
     std::string err;
     int N = 0, ndim = 0, maxnbr = 0, arm = 0;
@@ -55,6 +58,8 @@ struct Context {
     int gsteps = 0;
     bool use_graph = true;
 };
+
+inline Arrays &cur(Context *c) { return c->AA[c->cur]; }
 
 void drop_graph(Context *c)
 {
@@ -131,11 +136,11 @@ void free_list(std::vector<void *> &list)
     list.clear();
 }
 
-// per-particle arrays of capacity Ncap = N + gcap
+// per-particle arrays of capacity Ncap = N + gcap; persistent fields twice
 int alloc_particles(Context *c, int gcap)
 {
     Params &P = c->P;
-    Arrays &A = c->A;
+    Arrays &A = c->AA[0], &B = c->AA[1];
     const size_t ncap = (size_t)c->N + gcap;
     std::vector<void *> &L = c->part_allocs;
     P.gcap = gcap;
@@ -143,23 +148,31 @@ int alloc_particles(Context *c, int gcap)
     int rc = 0;
     for (int d = 0; d < 3; d++) {
         A.x[d] = A.xs[d] = A.v[d] = A.a[d] = A.nrm[d] = A.nn[d] = nullptr;
+        B.x[d] = B.v[d] = B.a[d] = nullptr;
         if (d >= c->ndim) continue;
-        rc |= dalloc(c, L, &A.x[d], ncap);
+        for (Arrays *S : {&A, &B}) {
+            rc |= dalloc(c, L, &S->x[d], ncap);
+            rc |= dalloc(c, L, &S->v[d], ncap);
+            rc |= dalloc(c, L, &S->a[d], c->N);
+        }
         rc |= dalloc(c, L, &A.xs[d], ncap);
-        rc |= dalloc(c, L, &A.v[d], ncap);
-        rc |= dalloc(c, L, &A.a[d], c->N);
         if (P.st) {
             rc |= dalloc(c, L, &A.nrm[d], c->N);
             rc |= dalloc(c, L, &A.nn[d], c->N);
         }
     }
-    rc |= dalloc(c, L, &A.rho, ncap);
+    for (Arrays *S : {&A, &B}) {
+        rc |= dalloc(c, L, &S->rho, ncap);
+        rc |= dalloc(c, L, &S->m, ncap);
+        rc |= dalloc(c, L, &S->info, ncap);
+        rc |= dalloc(c, L, &S->orig, c->N);
+        if (P.rigid) rc |= dalloc(c, L, &S->rrank, c->N);
+        else S->rrank = nullptr;
+    }
     rc |= dalloc(c, L, &A.rho_new, ncap);
     rc |= dalloc(c, L, &A.p, ncap);
     rc |= dalloc(c, L, &A.prho2, ncap);
     rc |= dalloc(c, L, &A.vol, ncap);
-    rc |= dalloc(c, L, &A.m, ncap);
-    rc |= dalloc(c, L, &A.info, ncap);
     rc |= dalloc(c, L, &A.trd, c->N);
     rc |= dalloc(c, L, &A.mindist, c->N);
     rc |= dalloc(c, L, &A.critts_p, c->N);
@@ -175,15 +188,36 @@ int alloc_particles(Context *c, int gcap)
     rc |= dalloc(c, L, &A.cell_of, c->N);
     rc |= dalloc(c, L, &A.slot, c->N);
     rc |= dalloc(c, L, &A.tmp, c->N);
-    rc |= dalloc(c, L, &A.entries, c->N);
     rc |= dalloc(c, L, &A.gsrc, gcap);
     rc |= dalloc(c, L, &A.gshift, gcap);
+    // shared fields: the second set points at the same arrays
+    for (int d = 0; d < 3; d++) {
+        B.xs[d] = A.xs[d];
+        B.nrm[d] = A.nrm[d];
+        B.nn[d] = A.nn[d];
+    }
+    B.rho_new = A.rho_new; B.p = A.p; B.prho2 = A.prho2; B.vol = A.vol;
+    B.trd = A.trd; B.mindist = A.mindist; B.critts_p = A.critts_p; B.rhoold = A.rhoold;
+    B.Nf = A.Nf; B.kappa = A.kappa; B.nnbr = A.nnbr; B.nbr = A.nbr;
+    B.cell_of = A.cell_of; B.slot = A.slot; B.tmp = A.tmp;
+    B.gsrc = A.gsrc; B.gshift = A.gshift;
     return rc;
+}
+
+// cell arrays, control block and rigid list are shared by both sets
+void share_cells(Context *c)
+{
+    Arrays &A = c->AA[0], &B = c->AA[1];
+    B.cell_count = A.cell_count; B.cell_start = A.cell_start;
+    B.gcount = A.gcount; B.gfirst = A.gfirst;
+    B.slab_cnt = A.slab_cnt; B.slab_off = A.slab_off; B.scan_aux = A.scan_aux;
+    B.rigid_idx = A.rigid_idx;
+    B.ctl = A.ctl;
 }
 
 int alloc_cells(Context *c, int cap)
 {
-    Arrays &A = c->A;
+    Arrays &A = c->AA[0];
     std::vector<void *> &L = c->cell_allocs;
     free_list(L);
     c->P.ncell_cap = cap;
@@ -195,20 +229,21 @@ int alloc_cells(Context *c, int cap)
     rc |= dalloc(c, L, &A.slab_cnt, (size_t)cap);
     rc |= dalloc(c, L, &A.slab_off, (size_t)cap);
     rc |= dalloc(c, L, &A.scan_aux, (size_t)cap / 2048 + 4);
+    share_cells(c);
     return rc;
 }
 
-// Grows the image capacity: per-particle arrays re-allocated, the real
-// particles' persistent fields copied over.
+// Grows the image capacity: per-particle arrays re-allocated; the current
+// set's persistent fields move to set 0 of the new allocation.
 int grow_particles(Context *c, int new_gcap)
 {
-    Arrays old = c->A;
+    Arrays old = cur(c);
     std::vector<void *> old_list;
     old_list.swap(c->part_allocs);
-    const int rigid_keep = 0;
-    (void)rigid_keep;
     if (alloc_particles(c, new_gcap)) return SPH_ERR_HIP;
-    Arrays &A = c->A;
+    share_cells(c);
+    c->cur = 0;
+    Arrays &A = c->AA[0];
     const size_t n = (size_t)c->N;
     auto cp = [&](void *dst, const void *src, size_t bytes) -> int {
         if (!dst || !src) return 0;
@@ -233,9 +268,12 @@ int grow_particles(Context *c, int new_gcap)
     rc |= cp(A.vol, old.vol, n * 8);
     rc |= cp(A.m, old.m, n * 8);
     rc |= cp(A.info, old.info, n * 4);
+    rc |= cp(A.orig, old.orig, n * 4);
+    if (c->P.rigid) rc |= cp(A.rrank, old.rrank, n * 4);
     rc |= cp(A.trd, old.trd, n * 8);
     rc |= cp(A.mindist, old.mindist, n * 8);
     rc |= cp(A.critts_p, old.critts_p, n * 8);
+    rc |= cp(A.nnbr, old.nnbr, n * 4);
     if (c->P.st) {
         rc |= cp(A.Nf, old.Nf, n * 8);
         rc |= cp(A.kappa, old.kappa, n * 8);
@@ -247,7 +285,7 @@ int grow_particles(Context *c, int new_gcap)
 
 int sync_ctl(Context *c)
 {
-    HIPCHK(c, hipMemcpyAsync(c->hctl, c->A.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->hctl, cur(c).ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     harvest_timing(c);
     return 0;
@@ -255,7 +293,7 @@ int sync_ctl(Context *c)
 
 int push_ctl(Context *c)
 {
-    HIPCHK(c, hipMemcpyAsync(c->A.ctl, c->hctl, sizeof(Ctl), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(cur(c).ctl, c->hctl, sizeof(Ctl), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
@@ -322,12 +360,21 @@ int run_phase(Context *c, F &&f)
     return SPH_ERROR;
 }
 
+// MSolution::SetupLinkedList: binning, then the move of the persistent
+// fields into cell order in the other buffer set, which becomes current.
+void sort_step(Context *c)
+{
+    launch_setup_linked_list(c->P, cur(c), c->st);
+    launch_permute(c->P, cur(c), c->AA[c->cur ^ 1], c->st);
+    c->cur ^= 1;
+}
+
 void fused_step(Context *c)
 {
     const Params &P = c->P;
-    const Arrays &A = c->A;
+    timed(c, K_BIN, [&] { sort_step(c); });
+    const Arrays &A = cur(c);
     hipStream_t st = c->st;
-    timed(c, K_BIN, [&] { launch_setup_linked_list(P, A, st); });
     if (P.periodic) timed(c, K_IMAGES, [&] { launch_ghost_setup(P, A, st); });
     timed(c, K_NBR, [&] { launch_neighbours(P, A, 1, st); });
     if (c->arm == SPH_ARM_FP32C)
@@ -545,19 +592,28 @@ int create_impl(Context *c, const sph_scene_desc *d)
     HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIPCHK(c, hipHostMalloc((void **)&c->hctl, sizeof(Ctl), hipHostMallocDefault));
     memset(c->hctl, 0, sizeof(Ctl));
-    if (dalloc(c, c->part_allocs, &c->A.ctl, 1)) return SPH_ERR_HIP;
+    if (dalloc(c, c->part_allocs, &c->AA[0].ctl, 1)) return SPH_ERR_HIP;
     // the control block is not part of part_allocs' lifetime: keep it separate
     c->part_allocs.pop_back();
     if (alloc_particles(c, gcap)) return SPH_ERR_HIP;
     if (alloc_cells(c, (int)cap)) return SPH_ERR_HIP;
     if (P.rigid) {
-        if (dalloc(c, c->cell_allocs, &c->A.rigid_idx, rigid_idx.size())) return SPH_ERR_HIP;
+        if (dalloc(c, c->cell_allocs, &c->AA[0].rigid_idx, rigid_idx.size())) return SPH_ERR_HIP;
         // rigid index list must survive cell re-allocation: move to its own slot
         c->cell_allocs.pop_back();
-        HIPCHK(c, hipMemcpy(c->A.rigid_idx, rigid_idx.data(), rigid_idx.size() * sizeof(int),
+        HIPCHK(c, hipMemcpy(c->AA[0].rigid_idx, rigid_idx.data(), rigid_idx.size() * sizeof(int),
                             hipMemcpyHostToDevice));
     }
-    Arrays &A = c->A;
+    share_cells(c);
+    c->cur = 0;
+    Arrays &A = c->AA[0];
+    {
+        std::vector<int> orig(N), rrank(N, -1);
+        for (int i = 0; i < N; i++) orig[i] = i;
+        for (size_t k = 0; k < rigid_idx.size(); k++) rrank[rigid_idx[k]] = (int)k;
+        HIPCHK(c, hipMemcpy(A.orig, orig.data(), N * 4, hipMemcpyHostToDevice));
+        if (P.rigid) HIPCHK(c, hipMemcpy(A.rrank, rrank.data(), N * 4, hipMemcpyHostToDevice));
+    }
     for (int dd = 0; dd < D; dd++) {
         HIPCHK(c, hipMemcpy(A.x[dd], hx[dd].data(), N * 8, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(A.xs[dd], hx[dd].data(), N * 8, hipMemcpyHostToDevice));
@@ -594,9 +650,9 @@ int create_impl(Context *c, const sph_scene_desc *d)
     if (push_ctl(c)) return SPH_ERR_HIP;
     // ---- first neighbours (no images), critical step, first dt (:71-82) ----
     rc = run_phase(c, [&] {
-        launch_setup_linked_list(c->P, c->A, c->st);
-        launch_neighbours(c->P, c->A, 0, c->st);
-        launch_strain(c->P, c->A, M_CRIT, c->st);
+        sort_step(c);
+        launch_neighbours(c->P, cur(c), 0, c->st);
+        launch_strain(c->P, cur(c), M_CRIT, c->st);
     });
     if (rc) return rc;
     const double critts = dunkey(h.critts_key);
@@ -609,14 +665,24 @@ int create_impl(Context *c, const sph_scene_desc *d)
 
 Context *ctx_of(void *p) { return (Context *)p; }
 
+// deck number of every storage slot of the current order
+int slot_orig(Context *c, std::vector<int> &orig)
+{
+    orig.resize(c->N);
+    HIPCHK(c, hipMemcpy(orig.data(), cur(c).orig, c->N * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int download_vec(Context *c, double *const *src, double *out)
 {
     const int N = c->N;
     std::vector<double> tmp(N);
+    std::vector<int> orig;
+    if (slot_orig(c, orig)) return SPH_ERR_HIP;
     for (int d = 0; d < 3; d++) {
         if (d < c->ndim && src[d]) {
             HIPCHK(c, hipMemcpy(tmp.data(), src[d], N * 8, hipMemcpyDeviceToHost));
-            for (int i = 0; i < N; i++) out[3 * (size_t)i + d] = tmp[i];
+            for (int i = 0; i < N; i++) out[3 * (size_t)orig[i] + d] = tmp[i];
         } else {
             for (int i = 0; i < N; i++) out[3 * (size_t)i + d] = 0.0;
         }
@@ -658,8 +724,8 @@ void sph_destroy(void *ctx)
     drop_graph(c);
     free_list(c->part_allocs);
     free_list(c->cell_allocs);
-    if (c->A.ctl) hipFree(c->A.ctl);
-    if (c->A.rigid_idx) hipFree(c->A.rigid_idx);
+    if (c->AA[0].ctl) hipFree(c->AA[0].ctl);
+    if (c->AA[0].rigid_idx) hipFree(c->AA[0].rigid_idx);
     if (c->canon) hipFree(c->canon);
     if (c->hctl) hipHostFree(c->hctl);
     harvest_timing(c);
@@ -677,28 +743,28 @@ const char *sph_last_error(void *ctx)
 int sph_neighbours(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_setup_linked_list(c->P, c->A, c->st); });
+    return run_phase(c, [&] { sort_step(c); });
 }
 
 int sph_ghost_setup(void *ctx)
 {
     Context *c = ctx_of(ctx);
     if (!c->P.periodic) return SPH_OK;
-    return run_phase(c, [&] { launch_ghost_setup(c->P, c->A, c->st); });
+    return run_phase(c, [&] { launch_ghost_setup(c->P, cur(c), c->st); });
 }
 
 int sph_linked_list_neighbours(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_neighbours(c->P, c->A, 0, c->st); });
+    return run_phase(c, [&] { launch_neighbours(c->P, cur(c), 0, c->st); });
 }
 
 int sph_strain(void *ctx)
 {
     Context *c = ctx_of(ctx);
     return run_phase(c, [&] {
-        launch_shift(c->P, c->A, c->st);
-        launch_strain(c->P, c->A, M_STRAIN, c->st);
+        launch_shift(c->P, cur(c), c->st);
+        launch_strain(c->P, cur(c), M_STRAIN, c->st);
     });
 }
 
@@ -706,19 +772,19 @@ int sph_calculate_surface_normals(void *ctx)
 {
     Context *c = ctx_of(ctx);
     if (!c->P.st) return SPH_OK;
-    return run_phase(c, [&] { launch_normals(c->P, c->A, c->st); });
+    return run_phase(c, [&] { launch_normals(c->P, cur(c), c->st); });
 }
 
 int sph_constitutive(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_strain(c->P, c->A, M_CONST | M_INPLACE, c->st); });
+    return run_phase(c, [&] { launch_strain(c->P, cur(c), M_CONST | M_INPLACE, c->st); });
 }
 
 int sph_calc_critical_timestep(void *ctx, double *critts)
 {
     Context *c = ctx_of(ctx);
-    int rc = run_phase(c, [&] { launch_strain(c->P, c->A, M_CRIT, c->st); });
+    int rc = run_phase(c, [&] { launch_strain(c->P, cur(c), M_CRIT, c->st); });
     if (rc == 0 && critts) *critts = dunkey(c->hctl->critts_key);
     return rc;
 }
@@ -726,27 +792,27 @@ int sph_calc_critical_timestep(void *ctx, double *critts)
 int sph_time_advance(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_advance(c->P, c->A, c->st); });
+    return run_phase(c, [&] { launch_advance(c->P, cur(c), c->st); });
 }
 
 int sph_momentum(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_momentum(c->P, c->A, 0, c->st); });
+    return run_phase(c, [&] { launch_momentum(c->P, cur(c), 0, c->st); });
 }
 
 int sph_boundcond(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_integrate(c->P, c->A, M_BC, c->st); });
+    return run_phase(c, [&] { launch_integrate(c->P, cur(c), M_BC, c->st); });
 }
 
 int sph_update_velocity(void *ctx)
 {
     Context *c = ctx_of(ctx);
     return run_phase(c, [&] {
-        launch_rigid_kin(c->P, c->A, c->st);
-        launch_integrate(c->P, c->A, M_VEL, c->st);
+        launch_rigid_kin(c->P, cur(c), c->st);
+        launch_integrate(c->P, cur(c), M_VEL, c->st);
     });
 }
 
@@ -754,15 +820,15 @@ int sph_move_particles(void *ctx)
 {
     Context *c = ctx_of(ctx);
     return run_phase(c, [&] {
-        launch_integrate(c->P, c->A, M_MOVE, c->st);
-        launch_finish(c->P, c->A, 1, 0, c->st);
+        launch_integrate(c->P, cur(c), M_MOVE, c->st);
+        launch_finish(c->P, cur(c), 1, 0, c->st);
     });
 }
 
 int sph_end_step(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_finish(c->P, c->A, 0, 1, c->st); });
+    return run_phase(c, [&] { launch_finish(c->P, cur(c), 0, 1, c->st); });
 }
 
 // Captures kGraphSteps fused steps on the context's stream into one graph.
@@ -864,19 +930,27 @@ int sph_download(void *ctx, int field, double *out)
     Context *c = ctx_of(ctx);
     HIPCHK(c, hipStreamSynchronize(c->st));
     const int N = c->N;
-    const Arrays &A = c->A;
+    const Arrays &A = cur(c);
     std::vector<uint32_t> info;
+    std::vector<int> orig;
+    std::vector<double> tmp;
     auto scalar = [&](const double *src) -> int {
         if (!src) {
             c->err = "field not kept for this scene";
             return SPH_ERROR;
         }
-        HIPCHK(c, hipMemcpy(out, src, N * 8, hipMemcpyDeviceToHost));
+        if (slot_orig(c, orig)) return SPH_ERR_HIP;
+        tmp.resize(N);
+        HIPCHK(c, hipMemcpy(tmp.data(), src, N * 8, hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++) out[orig[i]] = tmp[i];
         return 0;
     };
     auto get_info = [&]() -> int {
+        if (slot_orig(c, orig)) return SPH_ERR_HIP;
+        std::vector<uint32_t> raw(N);
+        HIPCHK(c, hipMemcpy(raw.data(), A.info, N * 4, hipMemcpyDeviceToHost));
         info.resize(N);
-        HIPCHK(c, hipMemcpy(info.data(), A.info, N * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++) info[orig[i]] = raw[i];
         return 0;
     };
     switch (field) {
@@ -924,9 +998,14 @@ int sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr)
     const int N = c->N;
     if (!c->canon) HIPCHK(c, hipMalloc((void **)&c->canon, (size_t)N * c->maxnbr * 8));
     const int mn = std::min(maxnbr, c->maxnbr);
-    launch_canon(c->P, c->A, c->canon, mn, c->st);
+    launch_canon(c->P, cur(c), c->canon, mn, c->st);
     HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(nnbr, c->A.nnbr, N * 4, hipMemcpyDeviceToHost));
+    {
+        std::vector<int> orig, raw(N);
+        if (slot_orig(c, orig)) return SPH_ERR_HIP;
+        HIPCHK(c, hipMemcpy(raw.data(), cur(c).nnbr, N * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++) nnbr[orig[i]] = raw[i];
+    }
     std::vector<long long> tmp((size_t)N * mn);
     HIPCHK(c, hipMemcpy(tmp.data(), c->canon, tmp.size() * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < N; i++)
@@ -980,7 +1059,7 @@ struct Batch {
     std::vector<Context *> slots;          // nullptr = free slot
     std::vector<int> live;                 // resident, not finished, no error
     Params *dP = nullptr;
-    Arrays *dA = nullptr;
+    Arrays *dA = nullptr, *dA2 = nullptr;   // per slot: current / other buffer set
     int cap_slots = 0;
     int *dscenes = nullptr;
     int2 *dpmap = nullptr, *dtmap = nullptr;
@@ -989,8 +1068,8 @@ struct Batch {
     int cap_ctl = 0;
     BatchView view{};
     bool dirty = true;
-    hipGraphExec_t gexec = nullptr;
-    int gsteps = 0;
+    hipGraphExec_t gexec2[2] = {nullptr, nullptr};   // captured steps, per starting parity
+    int phase = 0;                                   // 1: tables swapped since the rebuild
     std::string err;
 };
 
@@ -1005,9 +1084,10 @@ struct Batch {
 
 void batch_drop_graph(Batch *b)
 {
-    if (b->gexec) hipGraphExecDestroy(b->gexec);
-    b->gexec = nullptr;
-    b->gsteps = 0;
+    for (auto &g : b->gexec2) {
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 template <class T>
@@ -1058,20 +1138,23 @@ int batch_rebuild(Batch *b)
     if (b->hctl) hipHostFree(b->hctl);
     BCHK(b, hipHostMalloc((void **)&b->hctl, (size_t)b->cap_ctl * sizeof(Ctl), hipHostMallocDefault));
     // parameter / array tables indexed by slot
-    static thread_local std::vector<Params> hp;
-    static thread_local std::vector<Arrays> ha;
-    hp.assign(nslot, Params{});
-    ha.assign(nslot, Arrays{});
+    std::vector<Params> hp(nslot);
+    std::vector<Arrays> ha(nslot), ha2(nslot);
     for (int s = 0; s < nslot; s++)
         if (b->slots[s]) {
-            hp[s] = b->slots[s]->P;
-            ha[s] = b->slots[s]->A;
+            Context *c = b->slots[s];
+            hp[s] = c->P;
+            ha[s] = c->AA[c->cur];
+            ha2[s] = c->AA[c->cur ^ 1];
         }
     if (b->dA) BCHK(b, hipFree(b->dA));
+    if (b->dA2) BCHK(b, hipFree(b->dA2));
     BCHK(b, hipMalloc((void **)&b->dA, (size_t)std::max(nslot, 1) * sizeof(Arrays)));
+    BCHK(b, hipMalloc((void **)&b->dA2, (size_t)std::max(nslot, 1) * sizeof(Arrays)));
     if (nslot) {
         BCHK(b, hipMemcpy(b->dP, hp.data(), nslot * sizeof(Params), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->dA, ha.data(), nslot * sizeof(Arrays), hipMemcpyHostToDevice));
+        BCHK(b, hipMemcpy(b->dA2, ha2.data(), nslot * sizeof(Arrays), hipMemcpyHostToDevice));
     }
     if (!scenes.empty()) {
         BCHK(b, hipMemcpy(b->dscenes, scenes.data(), scenes.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1081,6 +1164,7 @@ int batch_rebuild(Batch *b)
     BatchView &v = b->view;
     v.P = b->dP;
     v.A = b->dA;
+    v.A2 = b->dA2;
     v.scenes = b->dscenes;
     v.pmap = b->dpmap;
     v.tmap = b->dtmap;
@@ -1235,26 +1319,36 @@ int sph_batch_remove(void *batch, int slot)
 int sph_batch_step(void *batch, int nsteps, int *live)
 {
     Batch *b = (Batch *)batch;
-    constexpr int kBatchGraphSteps = 8;
+    constexpr int kBatchGraphSteps = 8;    // even: a graph starts and ends on the same parity
     int done_steps = 0;
     while (done_steps < nsteps && !b->live.empty()) {
-        if (b->dirty && batch_rebuild(b)) return SPH_ERR_HIP;
+        if (b->dirty) {
+            if (batch_rebuild(b)) return SPH_ERR_HIP;
+            b->phase = 0;
+        }
         const int chunk = std::min(nsteps - done_steps, 32);
         int k = 0;
+        BatchView views[2] = {b->view, b->view};
+        std::swap(views[1].A, views[1].A2);
         if (chunk >= kBatchGraphSteps) {
-            if (!b->gexec) {
+            hipGraphExec_t &gx = b->gexec2[b->phase];
+            if (!gx) {
                 hipGraph_t g = nullptr;
                 BCHK(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-                for (int q = 0; q < kBatchGraphSteps; q++) launch_batch_step(b->view, b->st);
+                for (int q = 0; q < kBatchGraphSteps; q++) launch_batch_step(views[(b->phase + q) & 1], b->st);
                 BCHK(b, hipStreamEndCapture(b->st, &g));
-                if (hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) != hipSuccess) b->gexec = nullptr;
+                if (hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess) gx = nullptr;
                 hipGraphDestroy(g);
-                b->gsteps = b->gexec ? kBatchGraphSteps : 0;
             }
-            if (b->gexec)
-                for (; k + b->gsteps <= chunk; k += b->gsteps) BCHK(b, hipGraphLaunch(b->gexec, b->st));
+            if (gx)
+                for (; k + kBatchGraphSteps <= chunk; k += kBatchGraphSteps) BCHK(b, hipGraphLaunch(gx, b->st));
         }
-        for (; k < chunk; k++) launch_batch_step(b->view, b->st);
+        // each remaining step sorts every live scene into its other buffer set
+        for (; k < chunk; k++) {
+            launch_batch_step(views[b->phase], b->st);
+            b->phase ^= 1;
+            for (int s : b->live) b->slots[s]->cur ^= 1;
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             b->err = std::string("HIP launch error: ") + hipGetErrorString(e);
@@ -1311,6 +1405,7 @@ void sph_batch_destroy(void *batch)
         if (c) sph_destroy(c);
     hipFree(b->dP);
     hipFree(b->dA);
+    hipFree(b->dA2);
     hipFree(b->dscenes);
     hipFree(b->dpmap);
     hipFree(b->dtmap);

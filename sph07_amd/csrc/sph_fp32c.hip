@@ -268,18 +268,18 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const P
 void launch_batch_strain32(const BatchView &B, hipStream_t st)
 {
     switch (B.ndim) {
-    case 1: hipLaunchKernelGGL(kb_strain32<1>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
-    case 2: hipLaunchKernelGGL(kb_strain32<2>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
-    default: hipLaunchKernelGGL(kb_strain32<3>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
+    case 1: hipLaunchKernelGGL(kb_strain32<1>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
+    case 2: hipLaunchKernelGGL(kb_strain32<2>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
+    default: hipLaunchKernelGGL(kb_strain32<3>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
     }
 }
 
 void launch_batch_momentum32(const BatchView &B, hipStream_t st)
 {
     switch (B.ndim) {
-    case 1: hipLaunchKernelGGL(kb_momentum32<1>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
-    case 2: hipLaunchKernelGGL(kb_momentum32<2>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
-    default: hipLaunchKernelGGL(kb_momentum32<3>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap); break;
+    case 1: hipLaunchKernelGGL(kb_momentum32<1>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
+    case 2: hipLaunchKernelGGL(kb_momentum32<2>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
+    default: hipLaunchKernelGGL(kb_momentum32<3>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
     }
 }
 

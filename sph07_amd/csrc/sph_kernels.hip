@@ -127,21 +127,22 @@ __device__ __forceinline__ void grid_setup_body(const Params &P, Ctl *ctl)
         }
     }
     for (int i = 0; i < 3; i++) ncell *= (ctl->glim[i] > 0 ? ctl->glim[i] : 0);
-    if (ncell <= 0 || ncell > P.ncell_cap) {
-        ctl->ncell = (int)(ncell > 0x7fffffffLL ? 0x7fffffff : ncell);
+    if (ncell <= 0 || ncell + 1 > P.ncell_cap) {
+        ctl->ncell = (int)(ncell > 0x7ffffff0LL ? 0x7ffffff0 : ncell);
         set_err(ctl, kErrGridCap, -1);
         return;
     }
     ctl->ncell = (int)ncell;
+    ctl->nbucket = (int)ncell + 1;
 }
 
-// cell_count[0..ncell] and image counts cleared
+// bucket counts (cells + the inactive bucket) and image counts cleared
 __device__ __forceinline__ void zero_cells_body(const Params &P, const Arrays &A, const int bid, const int nb)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
     const int n = ctl->ncell;
-    for (int c = bid * kBlock + threadIdx.x; c <= n; c += nb * kBlock) {
+    for (int c = bid * kBlock + threadIdx.x; c <= n + 1; c += nb * kBlock) {
         A.cell_count[c] = 0;
         if (P.periodic && c < n) A.gcount[c] = 0;
     }
@@ -158,19 +159,25 @@ __device__ __forceinline__ void bin_count_body(const Params &P, const Arrays &A,
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     uint32_t f = A.info[i];
-    if (!(f & kActive)) { A.cell_of[i] = -1; return; }
+    const int inactive_bucket = ctl->ncell;
+    if (!(f & kActive)) {
+        A.cell_of[i] = inactive_bucket;
+        A.slot[i] = atomicAdd(&A.cell_count[inactive_bucket], 1);
+        return;
+    }
     int bc[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < D; d++) bc[d] = (int)(ctl->gsize[d] * (A.x[d][i] - ctl->gmin[d]));
     if (ctl->timestep == 1 && (bc[0] < 0 || bc[0] > ctl->glim[0] - 1)) {
         A.info[i] = f & ~kActive;          // outside the (single) process slab
-        A.cell_of[i] = -1;
+        A.cell_of[i] = inactive_bucket;
+        A.slot[i] = atomicAdd(&A.cell_count[inactive_bucket], 1);
         return;
     }
 #pragma unroll
     for (int d = 0; d < 3; d++)
         if (bc[d] < 0 || bc[d] >= ctl->glim[d]) {
-            set_err(ctl, kErrOutOfGrid, i);
+            set_err(ctl, kErrOutOfGrid, A.orig[i]);
             A.cell_of[i] = -1;
             return;
         }
@@ -189,8 +196,9 @@ __device__ __forceinline__ void bin_scatter_body(const Params &P, const Arrays &
     A.tmp[A.cell_start[c] + A.slot[i]] = i;
 }
 
-// Order inside a cell: descending particle number, the order head insertion
-// leaves in the reference's linked lists (MSolution.cpp:143-144).
+// Order inside a cell: descending deck number, the order head insertion
+// leaves in the reference's linked lists (MSolution.cpp:143-144).  The
+// destination slot of particle i in the sorted storage goes to slot[i].
 __device__ __forceinline__ void bin_rank_body(const Params &P, const Arrays &A, const int bid)
 {
     if (CTL_STOP(A.ctl)) return;
@@ -199,9 +207,36 @@ __device__ __forceinline__ void bin_rank_body(const Params &P, const Arrays &A, 
     const int c = A.cell_of[i];
     if (c < 0) return;
     const int s = A.cell_start[c], e = A.cell_start[c + 1];
+    const int oi = A.orig[i];
     int r = 0;
-    for (int t = s; t < e; t++) r += A.tmp[t] > i;
-    A.entries[s + r] = i;
+    for (int t = s; t < e; t++) r += A.orig[A.tmp[t]] > oi;
+    A.slot[i] = s + r;
+}
+
+// Moves every persistent per-particle field to its sorted slot (src -> dst
+// buffer set).  After an error or the stop test the move is the identity, so
+// the two buffer sets still alternate once per queued step.
+template <int D>
+__device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, const Arrays &T, const int bid)
+{
+    const int i = bid * kBlock + threadIdx.x;
+    if (i >= P.N) return;
+    const int j = CTL_STOP(S.ctl) ? i : S.slot[i];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        T.x[d][j] = S.x[d][i];
+        T.v[d][j] = S.v[d][i];
+        T.a[d][j] = S.a[d][i];
+    }
+    T.rho[j] = S.rho[i];
+    T.m[j] = S.m[i];
+    T.info[j] = S.info[i];
+    T.orig[j] = S.orig[i];
+    if (P.rigid) {
+        const int rr = S.rrank[i];
+        T.rrank[j] = rr;
+        if (rr >= 0) T.rigid_idx[rr] = j;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -366,7 +401,7 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
         A.gcount[tc] = cnt;
         const int s0 = A.cell_start[c];
         for (int r = 0; r < cnt; r++) {
-            const int src = A.entries[s0 + r];   // descending particle number
+            const int src = s0 + r;              // descending deck number
             const int gi = P.N + g0 + r;
             for (int d = 0; d < P.ndim; d++) {
                 const double xv = A.x[d][src] + addv[d];
@@ -423,42 +458,59 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
     double mind = 1.0e20;
     int nn = 0;
     const size_t stride = (size_t)P.Ncap;
-    for (int r = lo[0]; r <= hi[0]; r++)
-        for (int s = lo[1]; s <= hi[1]; s++)
-            for (int t = lo[2]; t <= hi[2]; t++) {
-                const int c = gl2 * (gl1 * r + s) + t;
-                // images first (most recently created first), then reals
-                int ng = P.periodic ? A.gcount[c] : 0;
-                const int g0 = ng ? A.gfirst[c] : 0;
-                const int s0 = A.cell_start[c], s1 = A.cell_start[c + 1];
-                for (int q = ng + (s1 - s0) - 1; q >= 0; q--) {
-                    int j;
-                    if (q >= s1 - s0) j = P.N + g0 + (q - (s1 - s0));
-                    else j = A.entries[s1 - 1 - q];
-                    if (j == i) continue;
-                    double d2 = 0.0;
+    // The innermost looped axis (y in 2D, z in 3D) is contiguous in cell index
+    // and, cells being sorted ranges, in storage: a run of up to 3 cells with
+    // no periodic images is one storage range.
+    auto test = [&](int j) -> bool {
+        double d2 = 0.0;
 #pragma unroll
-                    for (int d = 0; d < D; d++) {
-                        const double dd = xi[d] - A.x[d][j];
-                        d2 += dd * dd;
-                    }
-                    if (!onemat) cut2 = P.pc[mat_i][info_mat(A.info[j])].cut2;
-                    if (d2 < cut2) {
-                        if (P.want_mindist) {
-                            const double r1 = sqrt(d2);
-                            mind = mind < r1 ? mind : r1;
-                        }
-                        if (nn < P.maxnbr) {
-                            A.nbr[(size_t)nn * stride + i] = j;
-                            nn++;
-                        } else {
-                            set_err(ctl, kErrNbrOverflow, i);
-                            A.nnbr[i] = nn;
-                            return;
-                        }
-                    }
+        for (int d = 0; d < D; d++) {
+            const double dd = xi[d] - A.x[d][j];
+            d2 += dd * dd;
+        }
+        if (!onemat) cut2 = P.pc[mat_i][info_mat(A.info[j])].cut2;
+        if (d2 < cut2) {
+            if (P.want_mindist) {
+                const double r1 = sqrt(d2);
+                mind = mind < r1 ? mind : r1;
+            }
+            if (nn >= P.maxnbr) return false;
+            A.nbr[(size_t)nn * stride + i] = j;
+            nn++;
+        }
+        return true;
+    };
+    bool ok = true;
+    const int inner = D == 3 ? 2 : (D == 2 ? 1 : 0);
+    for (int r = (inner == 0 ? 0 : lo[0]); r <= (inner == 0 ? 0 : hi[0]) && ok; r++)
+        for (int s = (inner == 1 ? 0 : lo[1]); s <= (inner == 1 ? 0 : hi[1]) && ok; s++) {
+            int cc[3] = {r, s, 0};
+            cc[inner] = lo[inner];
+            const int c0 = gl2 * (gl1 * cc[0] + cc[1]) + cc[2];
+            const int nrun = hi[inner] - lo[inner] + 1;      // cells c0 .. c0 + nrun - 1
+            bool imgs = false;
+            if (P.periodic)
+                for (int q = 0; q < nrun; q++) imgs |= A.gcount[c0 + q] != 0;
+            if (!imgs) {
+                const int s0 = A.cell_start[c0], s1 = A.cell_start[c0 + nrun];
+                for (int j = s0; j < s1 && ok; j++)
+                    if (j != i) ok = test(j);
+            } else {
+                for (int q = 0; q < nrun && ok; q++) {
+                    const int c = c0 + q;
+                    const int ng = A.gcount[c], g0 = ng ? A.gfirst[c] : 0;
+                    for (int g = ng - 1; g >= 0 && ok; g--) ok = test(P.N + g0 + g);   // newest image first
+                    const int s0 = A.cell_start[c], s1 = A.cell_start[c + 1];
+                    for (int j = s0; j < s1 && ok; j++)
+                        if (j != i) ok = test(j);
                 }
             }
+        }
+    if (!ok) {
+        set_err(ctl, kErrNbrOverflow, A.orig[i]);
+        A.nnbr[i] = nn;
+        return;
+    }
     A.nnbr[i] = nn;
     if (P.want_mindist) A.mindist[i] = mind;
     if (shift) {
@@ -1067,7 +1119,7 @@ __global__ __launch_bounds__(kBlock) void kb_scan_reduce(const Arrays *At, const
 {
     const int2 e = map[blockIdx.x];
     const Arrays &Ab = At[e.x];
-    scan_reduce_body(Ab.cell_count, &Ab.ctl->ncell, Ab.scan_aux, Ab.ctl, e.y);
+    scan_reduce_body(Ab.cell_count, &Ab.ctl->nbucket, Ab.scan_aux, Ab.ctl, e.y);
 }
 __global__ __launch_bounds__(kBlock) void k_scan_aux(int *aux, int nblocks, const int *n_ptr, int *out, const Ctl *ctl)
 {
@@ -1077,7 +1129,7 @@ __global__ __launch_bounds__(kBlock) void kb_scan_aux(const Params *Pt, const Ar
 {
     SCENE_OF(sc[blockIdx.x]);
     const int nb = (Pb.ncell_cap + kScanTile - 1) / kScanTile;
-    scan_aux_body(Ab.scan_aux, nb, &Ab.ctl->ncell, Ab.cell_start, Ab.ctl);
+    scan_aux_body(Ab.scan_aux, nb, &Ab.ctl->nbucket, Ab.cell_start, Ab.ctl);
 }
 __global__ __launch_bounds__(kBlock) void k_scan_down(const int *in, int *out, const int *n_ptr, const int *aux, const Ctl *ctl)
 {
@@ -1087,7 +1139,7 @@ __global__ __launch_bounds__(kBlock) void kb_scan_down(const Arrays *At, const i
 {
     const int2 e = map[blockIdx.x];
     const Arrays &Ab = At[e.x];
-    scan_down_body(Ab.cell_count, Ab.cell_start, &Ab.ctl->ncell, Ab.scan_aux, Ab.ctl, e.y);
+    scan_down_body(Ab.cell_count, Ab.cell_start, &Ab.ctl->nbucket, Ab.scan_aux, Ab.ctl, e.y);
 }
 __global__ void k_reset_ngp(Ctl *ctl) { reset_ngp_body(ctl); }
 __global__ void kb_reset_ngp(const Arrays *At, const int *sc, int ns)
@@ -1125,6 +1177,12 @@ __global__ __launch_bounds__(kBlock) void kb_ghost_emit(const Params *Pt, const 
 {
     SCENE_OF(sc[blockIdx.x / nb]);
     if (has_pass(Pb, m, side)) ghost_emit_body(Pb, Ab, m, side, blockIdx.x % nb, nb);
+}
+template <int D> __global__ __launch_bounds__(kBlock) void k_permute(Params P, Arrays S, Arrays T) { permute_body<D>(P, S, T, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Params *Pt, const Arrays *At, const Arrays *At2, const int2 *map)
+{
+    const int2 e = map[blockIdx.x];
+    permute_body<D>(Pt[e.x], At[e.x], At2[e.x], e.y);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { neighbours_body<D>(P, A, shift, blockIdx.x); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *Pt, const Arrays *At, const int2 *map, int shift)
@@ -1198,12 +1256,12 @@ __global__ void k_canon(Params P, Arrays A, long long *out, int maxnbr)
     for (int k = 0; k < nn && k < maxnbr; k++) {
         const int j = A.nbr[(size_t)k * P.Ncap + i];
         long long cid;
-        if (j < P.N) cid = j;
+        if (j < P.N) cid = A.orig[j];
         else {
             const int g = j - P.N;
-            cid = -(1 + (long long)A.gsrc[g] * 27 + A.gshift[g]);
+            cid = -(1 + (long long)A.orig[A.gsrc[g]] * 27 + A.gshift[g]);
         }
-        out[(size_t)i * maxnbr + k] = cid;
+        out[(size_t)A.orig[i] * maxnbr + k] = cid;
     }
 }
 
@@ -1234,9 +1292,14 @@ void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st)
     hipLaunchKernelGGL(k_zero_cells, dim3(nblk(P.ncell_cap + 1) < 1024 ? nblk(P.ncell_cap + 1) : 1024),
                        dim3(kBlock), 0, st, P, A);
     DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_bin_count<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
-    launch_scan(P, A, A.cell_count, A.cell_start, &A.ctl->ncell, P.ncell_cap, st);
+    launch_scan(P, A, A.cell_count, A.cell_start, &A.ctl->nbucket, P.ncell_cap, st);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A);
     hipLaunchKernelGGL(k_bin_rank, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A);
+}
+
+void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st)
+{
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_permute<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, S, T));
 }
 
 void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
@@ -1324,29 +1387,31 @@ void launch_batch_step(const BatchView &B, hipStream_t st)
     hipLaunchKernelGGL(kb_scan_down, dim3(B.ntile_blocks), dim3(kBlock), 0, st, B.A, B.tmap);
     hipLaunchKernelGGL(kb_bin_scatter, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap);
     hipLaunchKernelGGL(kb_bin_rank, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap);
+    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_permute<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.A2, B.pmap));
+    const Arrays *A2 = B.A2;     // the sorted buffer set from here on
     if (B.any_periodic) {
-        hipLaunchKernelGGL(kb_reset_ngp, dim3(sb), dim3(kBlock), 0, st, B.A, B.scenes, ns);
+        hipLaunchKernelGGL(kb_reset_ngp, dim3(sb), dim3(kBlock), 0, st, A2, B.scenes, ns);
         for (int m = 0; m < 3; m++)
             for (int side = 0; side < 2; side++) {
                 if (!(B.pass_mask & (1 << (2 * m + side)))) continue;
-                hipLaunchKernelGGL(kb_ghost_count, dim3(ns * B.slab_nb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, B.slab_nb, m, side);
-                hipLaunchKernelGGL(kb_ghost_scan, dim3(ns), dim3(kBlock), 0, st, B.P, B.A, B.scenes, m, side);
-                hipLaunchKernelGGL(kb_ghost_emit, dim3(ns * B.slab_nb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, B.slab_nb, m, side);
+                hipLaunchKernelGGL(kb_ghost_count, dim3(ns * B.slab_nb), dim3(kBlock), 0, st, B.P, A2, B.scenes, B.slab_nb, m, side);
+                hipLaunchKernelGGL(kb_ghost_scan, dim3(ns), dim3(kBlock), 0, st, B.P, A2, B.scenes, m, side);
+                hipLaunchKernelGGL(kb_ghost_emit, dim3(ns * B.slab_nb), dim3(kBlock), 0, st, B.P, A2, B.scenes, B.slab_nb, m, side);
             }
     }
-    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_neighbours<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap, 1));
+    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_neighbours<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap, 1));
     if (B.arm == 1) launch_batch_strain32(B, st);
-    else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_strain<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap, M_STRAIN | M_CONST | M_CRIT));
+    else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_strain<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap, M_STRAIN | M_CONST | M_CRIT));
     if (B.any_st) {
-        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_normals<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap));
-        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_curvature<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap));
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_normals<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap));
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_curvature<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap));
     }
-    hipLaunchKernelGGL(kb_advance, dim3(sb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, ns);
+    hipLaunchKernelGGL(kb_advance, dim3(sb), dim3(kBlock), 0, st, B.P, A2, B.scenes, ns);
     if (B.arm == 1) launch_batch_momentum32(B, st);
-    else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_momentum<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap));
-    if (B.any_rigid) hipLaunchKernelGGL(kb_rigid_reduce, dim3(ns), dim3(kRigidBlock), 0, st, B.P, B.A, B.scenes, 1);
-    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_integrate<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap, M_BC | M_VEL | M_MOVE | M_RHO));
-    hipLaunchKernelGGL(kb_finish, dim3(sb), dim3(kBlock), 0, st, B.P, B.A, B.scenes, ns, 1, 1);
+    else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_momentum<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap));
+    if (B.any_rigid) hipLaunchKernelGGL(kb_rigid_reduce, dim3(ns), dim3(kRigidBlock), 0, st, B.P, A2, B.scenes, 1);
+    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_integrate<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap, M_BC | M_VEL | M_MOVE | M_RHO));
+    hipLaunchKernelGGL(kb_finish, dim3(sb), dim3(kBlock), 0, st, B.P, A2, B.scenes, ns, 1, 1);
 }
 
 // gathers the control blocks of a batch into one array for a single copy

@@ -11,6 +11,7 @@ enum { M_STRAIN = 1, M_CONST = 2, M_CRIT = 4, M_INPLACE = 8 };
 enum { M_BC = 1, M_VEL = 2, M_MOVE = 4, M_RHO = 8 };
 
 void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st);
+void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st);
 void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st);
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st);
 void launch_shift(const Params &P, const Arrays &A, hipStream_t st);
@@ -29,7 +30,8 @@ void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, 
 // scan tiles.  All scenes share ndim and arithmetic arm.
 struct BatchView {
     const Params *P;
-    const Arrays *A;
+    const Arrays *A;         // current buffer sets (binning reads these)
+    const Arrays *A2;        // the other buffer sets (sorted into; the step continues on them)
     const int *scenes;       // scene indices into P/A, nscene entries
     const int2 *pmap;        // npart_blocks entries
     const int2 *tmap;        // ntile_blocks entries (scan tiles over cells)

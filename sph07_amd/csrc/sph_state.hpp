@@ -2,7 +2,11 @@
 // kernel parameter blocks shared by the HIP kernels and the host driver.
 //
 // Layout in HBM (structure of arrays, one array per field, FP64):
-//   [0, N)          real particles, deck order (storage index == deck index)
+//   [0, N)          real particles, sorted every step by neighbour cell (the
+//                   reference's cell order, z fastest) and, inside a cell, by
+//                   descending deck number -- so each cell is one contiguous
+//                   range in the reference's traversal order; orig[] holds
+//                   the deck number (0-based) of each slot
 //   [N, N + Gcap)   periodic images ("ghosts"), rebuilt every step; image g
 //                   (1-based creation number, MGhostParticleData.cpp:30-31)
 //                   lives at N + g - 1.
@@ -46,10 +50,11 @@ struct Ctl {
     unsigned long long bbox_key[2][3];
     int glim[3];
     int ncell;
+    int nbucket;          // ncell + 1: the last bucket holds inactive particles
     int timestep, done, err, ngp;
     int err_particle;
     int step_limit;       // > 0: the scene stops after this many steps (batched runs)
-    int pad_[2];
+    int pad_[1];
 };
 
 // Per-material constants (MMaterial.h:17-40 after CheckDefaults).
@@ -91,8 +96,10 @@ struct Arrays {
     double *nrm[3], *nn[3], *Nf, *kappa;
     uint32_t *info;
     int *nnbr, *nbr;
-    // binning of real particles
-    int *cell_of, *slot, *cell_count, *cell_start, *tmp, *entries;
+    // deck number of each slot, rigid-body rank (or -1) of each slot
+    int *orig, *rrank;
+    // binning of real particles (slot[] doubles as the sorted destination)
+    int *cell_of, *slot, *cell_count, *cell_start, *tmp;
     // periodic images: per target cell the first image index and the count
     // (the images of one cell are consecutive, created from one source cell)
     int *gcount, *gfirst, *gsrc, *gshift;
