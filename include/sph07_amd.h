@@ -91,7 +91,7 @@ typedef struct sph_globals {
 enum {
     SPH_F_X = 0, SPH_F_V, SPH_F_A, SPH_F_RHO, SPH_F_P, SPH_F_C, SPH_F_MASS, SPH_F_H,
     SPH_F_TRD, SPH_F_MINDIST, SPH_F_NORMAL, SPH_F_NNORMAL, SPH_F_N, SPH_F_KAPPA,
-    SPH_F_CRITTS, SPH_F_RHOOLD, SPH_F_ACTIVE, SPH_F_VABS
+    SPH_F_CRITTS, SPH_F_RHOOLD, SPH_F_ACTIVE, SPH_F_VABS, SPH_F_NNBR
 };
 
 /* --- lifetime --- */

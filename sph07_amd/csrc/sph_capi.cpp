@@ -976,6 +976,13 @@ int sph_download(void *ctx, int field, double *out)
             out[i] = field == SPH_F_C ? mt.c : field == SPH_F_H ? mt.h : ((info[i] & kActive) ? 1.0 : 0.0);
         }
         return SPH_OK;
+    case SPH_F_NNBR: {
+        if (slot_orig(c, orig)) return SPH_ERR_HIP;
+        std::vector<int> raw(N);
+        HIPCHK(c, hipMemcpy(raw.data(), A.nnbr, N * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++) out[orig[i]] = raw[i];
+        return SPH_OK;
+    }
     case SPH_F_VABS: {
         std::vector<double> v(3 * (size_t)N);
         if (download_vec(c, A.v, v.data())) return SPH_ERR_HIP;
