@@ -169,6 +169,11 @@ int alloc_particles(Context *c, int gcap)
         if (P.rigid) rc |= dalloc(c, L, &S->rrank, c->N);
         else S->rrank = nullptr;
     }
+    rc |= dalloc(c, L, &A.rsA, ncap);
+    rc |= dalloc(c, L, &A.rsB, ncap);
+    rc |= dalloc(c, L, &A.rmA, ncap);
+    rc |= dalloc(c, L, &A.rmB, ncap);
+    rc |= dalloc(c, L, &A.rmC, ncap);
     rc |= dalloc(c, L, &A.rho_new, ncap);
     rc |= dalloc(c, L, &A.p, ncap);
     rc |= dalloc(c, L, &A.prho2, ncap);
@@ -196,6 +201,7 @@ int alloc_particles(Context *c, int gcap)
         B.nrm[d] = A.nrm[d];
         B.nn[d] = A.nn[d];
     }
+    B.rsA = A.rsA; B.rsB = A.rsB; B.rmA = A.rmA; B.rmB = A.rmB; B.rmC = A.rmC;
     B.rho_new = A.rho_new; B.p = A.p; B.prho2 = A.prho2; B.vol = A.vol;
     B.trd = A.trd; B.mindist = A.mindist; B.critts_p = A.critts_p; B.rhoold = A.rhoold;
     B.Nf = A.Nf; B.kappa = A.kappa; B.nnbr = A.nnbr; B.nbr = A.nbr;

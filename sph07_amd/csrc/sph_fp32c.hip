@@ -134,28 +134,42 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
             const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
             for (int k = 0; k < nn; k++) {
                 const int j = A.nbr[(size_t)k * stride + i];
-                const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][info_mat(A.info[j])]);
+                const double4 ra = A.rsA[j], rb = A.rsB[j];
+                const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][info_mat((uint32_t)rb.w)]);
+                const double xj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
                 float dx[D], r2 = 0.0f;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
-                    dx[d] = (float)(xi[d] - A.xs[d][j]);
+                    dx[d] = (float)(xi[d] - xj[d]);
                     r2 += dx[d] * dx[d];
                 }
-                const float f = gradw_factor(r2, q) * (float)A.vol[j];
+                const float f = gradw_factor(r2, q) * (float)ra.w;
 #pragma unroll
                 for (int d = 0; d < D; d++)
-                    two_sum_acc(gs[d], gc[d], (float)(A.v[d][j] - vi[d]) * (f * dx[d]));
+                    two_sum_acc(gs[d], gc[d], (float)(vj[d] - vi[d]) * (f * dx[d]));
             }
             double tr = 0.0;
 #pragma unroll
             for (int d = 0; d < D; d++) tr += (double)gs[d] + (double)gc[d];
             A.trd[i] = tr;
+            double xr[3] = {0.0, 0.0, 0.0}, vr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-            for (int d = 0; d < D; d++) A.x[d][i] = xi[d] + 0.5 * vi[d] * dt;
+            for (int d = 0; d < D; d++) {
+                xr[d] = xi[d] + 0.5 * vi[d] * dt;
+                vr[d] = vi[d];
+                A.x[d][i] = xr[d];
+            }
             rho_i = rho_i * (1.0 - tr * dt);
             const double p = eos_p32(mt, rho_i);
+            const double pr = p / (rho_i * rho_i);
             A.p[i] = p;
-            A.prho2[i] = p / (rho_i * rho_i);
+            A.prho2[i] = pr;
+            double4 ma, mb;
+            ma.x = xr[0]; ma.y = xr[1]; ma.z = xr[2]; ma.w = A.m[i];
+            mb.x = vr[0]; mb.y = vr[1]; mb.z = vr[2]; mb.w = rho_i;
+            A.rmA[i] = ma;
+            A.rmB[i] = mb;
+            A.rmC[i] = make_double2(pr, (double)fi);
         }
         A.rho_new[i] = rho_i;
         const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
@@ -206,22 +220,25 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
         for (int k = 0; k < nn; k++) {
             const int j = A.nbr[(size_t)k * stride + i];
-            const uint32_t fj = A.info[j];
+            const double4 ra = A.rmA[j], rb = A.rmB[j];
+            const double2 rc = A.rmC[j];
+            const uint32_t fj = (uint32_t)rc.y;
             const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
             const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][mat_j]);
-            const float mj = (float)A.m[j];
-            const double rho_j64 = A.rho_new[j];
+            const float mj = (float)ra.w;
+            const double rho_j64 = rb.w;
             const float rho_j = (float)rho_j64;
+            const double xj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
             float dx[D], dv[D], r2 = 0.0f, vr = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                dx[d] = (float)(xi[d] - A.x[d][j]);
-                dv[d] = (float)(vi[d] - A.v[d][j]);
+                dx[d] = (float)(xi[d] - xj[d]);
+                dv[d] = (float)(vi[d] - vj[d]);
                 r2 += dx[d] * dx[d];
                 vr += dv[d] * dx[d];
             }
             const float f = gradw_factor(r2, q);            // grad W = f * dx
-            float sdw = -mj * ((float)A.prho2[j] + pr_i);   // coefficient of grad W
+            float sdw = -mj * ((float)rc.x + pr_i);         // coefficient of grad W
             float tv = 0.0f;                                // coefficient of (v_i - v_j)
             if (!(rig_i && (fj & kRigid))) {
                 const float qj = (float)(P.mat[mat_j].nu * rho_j64);

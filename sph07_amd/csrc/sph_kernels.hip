@@ -103,6 +103,40 @@ __device__ __forceinline__ void gradw64(double dw[D], const double *xi, const do
 }
 
 // ---------------------------------------------------------------------------
+// gather records (the same FP64 values as the per-field arrays)
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void put_srec(const Arrays &A, int j, const double *xs, double vol,
+                                         const double *v, uint32_t info)
+{
+    double4 a, b;
+    a.x = xs[0]; a.y = D > 1 ? xs[1] : 0.0; a.z = D > 2 ? xs[2] : 0.0; a.w = vol;
+    b.x = v[0];  b.y = D > 1 ? v[1] : 0.0;  b.z = D > 2 ? v[2] : 0.0;  b.w = (double)info;
+    A.rsA[j] = a;
+    A.rsB[j] = b;
+}
+
+template <int D>
+__device__ __forceinline__ void put_mrec(const Arrays &A, int j, const double *x, double m,
+                                         const double *v, double rho, double prho2, uint32_t info)
+{
+    double4 a, b;
+    a.x = x[0]; a.y = D > 1 ? x[1] : 0.0; a.z = D > 2 ? x[2] : 0.0; a.w = m;
+    b.x = v[0]; b.y = D > 1 ? v[1] : 0.0; b.z = D > 2 ? v[2] : 0.0; b.w = rho;
+    A.rmA[j] = a;
+    A.rmB[j] = b;
+    A.rmC[j] = make_double2(prho2, (double)info);
+}
+
+template <int D>
+__device__ __forceinline__ void vec_of(const double4 &r, double *out)
+{
+    out[0] = r.x;
+    if (D > 1) out[1] = r.y;
+    if (D > 2) out[2] = r.z;
+}
+
+// ---------------------------------------------------------------------------
 // grid: MNeighbourVars::InitGrid (MNeighbourVars.cpp:18-66)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void grid_setup_body(const Params &P, Ctl *ctl)
@@ -416,7 +450,18 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
             A.vol[gi] = mass / rho;
             A.p[gi] = 0.0;        // images keep p = 0 (not copied, :28)
             A.prho2[gi] = 0.0;
-            A.info[gi] = (A.info[src] & (kMatMask | kRigid | (0xFFFFu << kColorShift))) | kGhost;
+            const uint32_t ginfo = (A.info[src] & (kMatMask | kRigid | (0xFFFFu << kColorShift))) | kGhost;
+            A.info[gi] = ginfo;
+            double gx[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < P.ndim; d++) {
+                gx[d] = A.x[d][gi];
+                gv[d] = A.v[d][gi];
+            }
+            switch (P.ndim) {
+            case 1: put_srec<1>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<1>(A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
+            case 2: put_srec<2>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<2>(A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
+            default: put_srec<3>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<3>(A, gi, gx, mass, gv, rho, 0.0, ginfo);
+            }
             A.gsrc[g0 + r] = src;
             A.gshift[g0 + r] = shift_code;
         }
@@ -515,8 +560,14 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
     if (P.want_mindist) A.mindist[i] = mind;
     if (shift) {
         const double dt = ctl->dt;
+        double xs[D], vi[D];
 #pragma unroll
-        for (int d = 0; d < D; d++) A.xs[d][i] = xi[d] - 0.5 * A.v[d][i] * dt;
+        for (int d = 0; d < D; d++) {
+            vi[d] = A.v[d][i];
+            xs[d] = xi[d] - 0.5 * vi[d] * dt;
+            A.xs[d][i] = xs[d];
+        }
+        put_srec<D>(A, i, xs, mi / A.rho[i], vi, fi);
     }
 }
 
@@ -528,10 +579,17 @@ __device__ __forceinline__ void shift_body(const Params &P, const Arrays &A, con
     if (CTL_STOP(ctl)) return;
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
-    const bool act = A.info[i] & kActive;
+    const uint32_t fi = A.info[i];
+    const bool act = fi & kActive;
     const double dt = ctl->dt;
+    double xs[D], vi[D];
 #pragma unroll
-    for (int d = 0; d < D; d++) A.xs[d][i] = act ? A.x[d][i] - 0.5 * A.v[d][i] * dt : A.x[d][i];
+    for (int d = 0; d < D; d++) {
+        vi[d] = A.v[d][i];
+        xs[d] = act ? A.x[d][i] - 0.5 * vi[d] * dt : A.x[d][i];
+        A.xs[d][i] = xs[d];
+    }
+    put_srec<D>(A, i, xs, A.m[i] / A.rho[i], vi, fi);
 }
 
 // ---------------------------------------------------------------------------
@@ -623,14 +681,15 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
             const size_t stride = (size_t)P.Ncap;
             for (int k = 0; k < nn; k++) {
                 const int j = A.nbr[(size_t)k * stride + i];
-                const double volj = A.vol[j];
-                const PairConst &pc = P.pc[mat_i][P.nmat == 1 ? mat_i : info_mat(A.info[j])];
-                double xj[D], dw[D];
-#pragma unroll
-                for (int d = 0; d < D; d++) xj[d] = A.xs[d][j];
+                const double4 ra = A.rsA[j], rb = A.rsB[j];
+                const double volj = ra.w;
+                const PairConst &pc = P.pc[mat_i][P.nmat == 1 ? mat_i : info_mat((uint32_t)rb.w)];
+                double xj[D], vj[D], dw[D];
+                vec_of<D>(ra, xj);
+                vec_of<D>(rb, vj);
                 gradw64<D>(dw, xi, xj, pc);
 #pragma unroll
-                for (int d = 0; d < D; d++) g[d] += volj * (A.v[d][j] - vi[d]) * dw[d];
+                for (int d = 0; d < D; d++) g[d] += volj * (vj[d] - vi[d]) * dw[d];
             }
 #pragma unroll
             for (int d = 0; d < D; d++) tr += 0.5 * (g[d] + g[d]);
@@ -646,8 +705,16 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
             if (act) {
                 rho_i = rho_i * (1.0 - tr * dt);
                 const double p = eos_p(mt, rho_i);
+                const double pr = p / (rho_i * rho_i);
                 A.p[i] = p;
-                A.prho2[i] = p / (rho_i * rho_i);
+                A.prho2[i] = pr;
+                double xr[D], vr[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    xr[d] = A.x[d][i];
+                    vr[d] = A.v[d][i];
+                }
+                put_mrec<D>(A, i, xr, A.m[i], vr, rho_i, pr, fi);
             }
             A.rho_new[i] = rho_i;
             if (mask & S_INPLACE) A.rho[i] = rho_i;
@@ -792,16 +859,15 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
         const size_t stride = (size_t)P.Ncap;
         for (int k = 0; k < nn; k++) {
             const int j = A.nbr[(size_t)k * stride + i];
-            const uint32_t fj = A.info[j];
+            const double4 ra = A.rmA[j], rb = A.rmB[j];
+            const double2 rc = A.rmC[j];
+            const uint32_t fj = (uint32_t)rc.y;
             const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
             const PairConst &pc = P.pc[mat_i][mat_j];
-            const double mj = A.m[j], rho_j = A.rho_new[j];
+            const double mj = ra.w, rho_j = rb.w;
             double xj[D], vj[D], dw[D];
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-                xj[d] = A.x[d][j];
-                vj[d] = A.v[d][j];
-            }
+            vec_of<D>(ra, xj);
+            vec_of<D>(rb, vj);
             gradw64<D>(dw, xi, xj, pc);
             double rab2 = 0.0, prod = 0.0;
 #pragma unroll
@@ -809,7 +875,7 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
                 rab2 += (xi[d] - xj[d]) * (xi[d] - xj[d]);
                 prod += (xi[d] - xj[d]) * dw[d];
             }
-            const double sp = -mj * (A.prho2[j] + pr_i);
+            const double sp = -mj * (rc.x + pr_i);
 #pragma unroll
             for (int d = 0; d < D; d++) ai[d] = ai[d] + sp * dw[d];
             if (rig_i && (fj & kRigid)) continue;

@@ -94,6 +94,12 @@ struct Arrays {
     double *rho, *rho_new, *p, *prho2, *vol, *m;
     double *trd, *mindist, *critts_p, *rhoold;
     double *nrm[3], *nn[3], *Nf, *kappa;
+    // gather records of the pair sweeps (one or two 32-byte loads per
+    // neighbour instead of one load per field):
+    //   continuity: rsA = {xs[0..2], m/rho}, rsB = {v[0..2], info}
+    //   momentum:   rmA = {x[0..2], m}, rmB = {v[0..2], rho}, rmC = {p/rho^2, info}
+    double4 *rsA, *rsB, *rmA, *rmB;
+    double2 *rmC;
     uint32_t *info;
     int *nnbr, *nbr;
     // deck number of each slot, rigid-body rank (or -1) of each slot
