@@ -189,8 +189,9 @@ int alloc_particles(Context *c, int gcap)
         A.Nf = A.kappa = nullptr;
     }
     rc |= dalloc(c, L, &A.nnbr, c->N);
-    rc |= dalloc(c, L, &A.nbr, (size_t)c->maxnbr * ncap);
+    rc |= dalloc(c, L, &A.nbr, (size_t)nbr_round(c->maxnbr) * ncap);
     rc |= dalloc(c, L, &A.cell_of, c->N);
+    rc |= dalloc(c, L, &A.occ, (size_t)c->N + 1);
     rc |= dalloc(c, L, &A.slot, c->N);
     rc |= dalloc(c, L, &A.tmp, c->N);
     rc |= dalloc(c, L, &A.gsrc, gcap);
@@ -205,7 +206,7 @@ int alloc_particles(Context *c, int gcap)
     B.rho_new = A.rho_new; B.p = A.p; B.prho2 = A.prho2; B.vol = A.vol;
     B.trd = A.trd; B.mindist = A.mindist; B.critts_p = A.critts_p; B.rhoold = A.rhoold;
     B.Nf = A.Nf; B.kappa = A.kappa; B.nnbr = A.nnbr; B.nbr = A.nbr;
-    B.cell_of = A.cell_of; B.slot = A.slot; B.tmp = A.tmp;
+    B.cell_of = A.cell_of; B.slot = A.slot; B.tmp = A.tmp; B.occ = A.occ;
     B.gsrc = A.gsrc; B.gshift = A.gshift;
     return rc;
 }
@@ -322,7 +323,7 @@ int check_device(Context *c)
     Ctl &h = *c->hctl;
     if (h.err == 0) return 0;
     if (h.err == kErrGridCap) {
-        long long want = (long long)h.ncell * 3 / 2 + 1024;
+        long long want = (long long)h.ncell * 3 / 2 + 1024;   // h.ncell: cells or codes needed
         if (want > 0x7fffffffLL - 1) {
             c->err = "ERROR: neighbour grid too large.";
             return SPH_ERROR;
@@ -492,6 +493,13 @@ int build_context(Context *c, const sph_scene_desc *d)
             const double hav = 2.0 * (0.5 * (P.mat[a].h + P.mat[b].h));
             pc.cut2 = hav * hav;
         }
+    P.cut2max = 0.0;
+    P.cut2min = 1e300;
+    for (int a = 1; a <= d->nummat; a++)
+        for (int b = 1; b <= d->nummat; b++) {
+            P.cut2max = std::max(P.cut2max, P.pc[a][b].cut2);
+            P.cut2min = std::min(P.cut2min, P.pc[a][b].cut2);
+        }
     for (int m = 0; m < D; m++)
         if (P.btype[m] == 4) {
             const double gs = 1.0 / (2.0 * P.hmax);
@@ -572,17 +580,22 @@ int create_impl(Context *c, const sph_scene_desc *d)
     P.n_rigid = (int)rigid_idx.size();
 
     // ---- capacities ----
-    long long ncell0 = 1;
+    long long ncell0 = 1, ncode0 = 1;
     {
         const double gs = 1.0 / (2.0 * P.hmax);
         for (int m = 0; m < D; m++) {
             long long gl;
             if (P.btype[m] == 4) gl = (long long)(gs * (P.bx[1][m] - P.bx[0][m])) + 2;
             else gl = (long long)(gs * (cmm[1][m] - (cmm[0][m] - 0.01 * P.hmax))) + 1;
-            ncell0 *= std::max(gl, 1LL);
+            gl = std::max(gl, 1LL);
+            ncell0 *= gl;
+            long long p2 = 1;
+            while (p2 < gl) p2 <<= 1;
+            ncode0 *= p2;          // cell-code space (power of two per axis)
         }
     }
-    long long cap = ncell0 + ncell0 / 4 + 4096;
+    long long cap = std::max(ncell0, ncode0);
+    cap += cap / 4 + 4096;
     if (cap > 0x7ffffff0LL) { c->err = "neighbour grid too large"; return SPH_ERROR; }
     int gcap = 0;
     if (P.periodic) {
@@ -1277,9 +1290,15 @@ long long sph_scene_footprint(const sph_scene_desc *d)
     const long long n = d->np;
     const long long D = d->ndim;
     const long long maxnbr = d->maxnbr > 0 ? d->maxnbr : 40;
-    const long long per = 8 * (4 * D) + 8 * 6 + 4 + 8 * 4 + 4 * maxnbr + 4 * 6 + (d->surface_tension ? 8 * (2 * D + 2) : 0);
+    // per slot (reals and images): x, v in both buffer sets, xs, rho and m in
+    // both sets, info in both sets, the two gather records, rho_new/p/prho2/vol,
+    // the neighbour list
+    const long long per_slot = 32 * D + 8 * D + 32 + 8 + 144 + 32 + 4 * maxnbr;
+    // per real particle: a in both sets, orig, trd/mindist/critts, nnbr, sort
+    // scratch, CSF fields
+    const long long per_real = 16 * D + 8 + 24 + 4 + 12 + (d->surface_tension ? 16 * D + 16 : 0);
     const long long img = d->boundary ? n / 4 + 4096 : 0;
-    return (n + img) * per + n * 8 /* cells, upper bound */ + 65536;
+    return (n + img) * per_slot + n * per_real + n * 16 /* cell arrays, estimate */ + 65536;
 }
 
 int sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int *slot, char *err, int errlen)

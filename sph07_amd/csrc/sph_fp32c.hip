@@ -129,11 +129,8 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
                 gs[d] = 0.0f;
                 gc[d] = 0.0f;
             }
-            const int nn = A.nnbr[i];
-            const size_t stride = (size_t)P.Ncap;
             const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
-            for (int k = 0; k < nn; k++) {
-                const int j = A.nbr[(size_t)k * stride + i];
+            for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
                 const double4 ra = A.rsA[j], rb = A.rsB[j];
                 const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][info_mat((uint32_t)rb.w)]);
                 const double xj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
@@ -147,7 +144,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
 #pragma unroll
                 for (int d = 0; d < D; d++)
                     two_sum_acc(gs[d], gc[d], (float)(vj[d] - vi[d]) * (f * dx[d]));
-            }
+            });
             double tr = 0.0;
 #pragma unroll
             for (int d = 0; d < D; d++) tr += (double)gs[d] + (double)gc[d];
@@ -215,11 +212,8 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             as[d] = 0.0f;
             ac[d] = 0.0f;
         }
-        const int nn = A.nnbr[i];
-        const size_t stride = (size_t)P.Ncap;
         const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
-        for (int k = 0; k < nn; k++) {
-            const int j = A.nbr[(size_t)k * stride + i];
+        for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
             const double4 ra = A.rmA[j], rb = A.rmB[j];
             const double2 rc = A.rmC[j];
             const uint32_t fj = (uint32_t)rc.y;
@@ -256,7 +250,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             const float sf = sdw * f;
 #pragma unroll
             for (int d = 0; d < D; d++) two_sum_acc(as[d], ac[d], sf * dx[d] + tv * dv[d]);
-        }
+        });
 #pragma unroll
         for (int d = 0; d < D; d++) ai[d] = (double)as[d] + (double)ac[d];
         if (P.st) {
@@ -269,13 +263,21 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
     for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
 }
 
-template <int D> __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A) { strain32_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
+{
+    const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
+    if (b < nb) strain32_body<D>(P, A, b);
+}
 template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     strain32_body<D>(Pt[e.x], At[e.x], e.y);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A) { momentum32_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
+{
+    const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
+    if (b < nb) momentum32_body<D>(P, A, b);
+}
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
@@ -305,18 +307,18 @@ static inline int nblk32(int n) { return (n + kBlock - 1) / kBlock; }
 void launch_strain32(const Params &P, const Arrays &A, hipStream_t st)
 {
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL(k_strain32<1>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL(k_strain32<2>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
-    default: hipLaunchKernelGGL(k_strain32<3>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL(k_strain32<1>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL(k_strain32<2>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
+    default: hipLaunchKernelGGL(k_strain32<3>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
     }
 }
 
 void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st)
 {
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL(k_momentum32<1>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL(k_momentum32<2>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
-    default: hipLaunchKernelGGL(k_momentum32<3>, dim3(nblk32(P.N)), dim3(kBlock), 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL(k_momentum32<1>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL(k_momentum32<2>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
+    default: hipLaunchKernelGGL(k_momentum32<3>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
     }
 }
 

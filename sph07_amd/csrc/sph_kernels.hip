@@ -161,13 +161,49 @@ __device__ __forceinline__ void grid_setup_body(const Params &P, Ctl *ctl)
         }
     }
     for (int i = 0; i < 3; i++) ncell *= (ctl->glim[i] > 0 ? ctl->glim[i] : 0);
-    if (ncell <= 0 || ncell + 1 > P.ncell_cap) {
-        ctl->ncell = (int)(ncell > 0x7ffffff0LL ? 0x7ffffff0 : ncell);
+    // Storage order of the cells: a bit-interleaved (Morton) code with the
+    // innermost traversal axis in the lowest bit of every level and each axis
+    // contributing ceil(log2(glim)) bits, so that a contiguous range of sorted
+    // particles is a compact region of space.  The traversal order of the
+    // neighbour search is unaffected (it walks cells by coordinates).
+    int mb[3] = {0, 0, 0};
+    for (int d = 0; d < 3; d++)
+        while (mb[d] < 31 && (1LL << mb[d]) < (long long)ctl->glim[d]) mb[d]++;
+    unsigned int msk[3] = {0u, 0u, 0u};
+    int pos = 0;
+    for (int l = 0; l < 31 && pos <= 30; l++)
+        for (int e = 2; e >= 0; e--)
+            if (l < mb[e] && pos <= 30) msk[e] |= 1u << pos++;
+    const long long ncode = pos <= 29 ? (1LL << pos) : 0x7fffffffLL;
+    const long long need = ncode > ncell ? ncode : ncell;
+    if (ncell <= 0 || pos > 29 || need + 2 > P.ncell_cap) {
+        ctl->ncell = (int)(need > 0x7ffffff0LL ? 0x7ffffff0 : need);
         set_err(ctl, kErrGridCap, -1);
         return;
     }
     ctl->ncell = (int)ncell;
-    ctl->nbucket = (int)ncell + 1;
+    ctl->ncode = (int)ncode;
+
+    ctl->nbucket = (int)ncode + 1;
+    for (int d = 0; d < 3; d++) ctl->cmask[d] = msk[d];
+}
+
+// deposits the low bits of v on the set bits of m (cell coordinate -> code bits)
+__device__ __forceinline__ int code_dep(int v, unsigned int m)
+{
+    int r = 0;
+    while (m) {
+        const unsigned int low = m & (0u - m);
+        if (v & 1) r |= (int)low;
+        v >>= 1;
+        m ^= low;
+    }
+    return r;
+}
+
+__device__ __forceinline__ int cell_code(const Ctl *ctl, int i, int j, int k)
+{
+    return code_dep(i, ctl->cmask[0]) | code_dep(j, ctl->cmask[1]) | code_dep(k, ctl->cmask[2]);
 }
 
 // bucket counts (cells + the inactive bucket) and image counts cleared
@@ -175,8 +211,9 @@ __device__ __forceinline__ void zero_cells_body(const Params &P, const Arrays &A
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
-    const int n = ctl->ncell;
-    for (int c = bid * kBlock + threadIdx.x; c <= n + 1; c += nb * kBlock) {
+    const int n = ctl->ncell, nc = ctl->ncode;
+    const int top = (nc > n ? nc : n) + 1;
+    for (int c = bid * kBlock + threadIdx.x; c <= top; c += nb * kBlock) {
         A.cell_count[c] = 0;
         if (P.periodic && c < n) A.gcount[c] = 0;
     }
@@ -193,7 +230,7 @@ __device__ __forceinline__ void bin_count_body(const Params &P, const Arrays &A,
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     uint32_t f = A.info[i];
-    const int inactive_bucket = ctl->ncell;
+    const int inactive_bucket = ctl->ncode;
     if (!(f & kActive)) {
         A.cell_of[i] = inactive_bucket;
         A.slot[i] = atomicAdd(&A.cell_count[inactive_bucket], 1);
@@ -215,7 +252,7 @@ __device__ __forceinline__ void bin_count_body(const Params &P, const Arrays &A,
             A.cell_of[i] = -1;
             return;
         }
-    const int c = ctl->glim[2] * (ctl->glim[1] * bc[0] + bc[1]) + bc[2];
+    const int c = cell_code(ctl, bc[0], bc[1], bc[2]);
     A.cell_of[i] = c;
     A.slot[i] = atomicAdd(&A.cell_count[c], 1);
 }
@@ -255,7 +292,8 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
 {
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
-    const int j = CTL_STOP(S.ctl) ? i : S.slot[i];
+    const bool moving = !CTL_STOP(S.ctl);
+    const int j = moving ? S.slot[i] : i;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         T.x[d][j] = S.x[d][i];
@@ -271,6 +309,11 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
         T.rrank[j] = rr;
         if (rr >= 0) T.rigid_idx[rr] = j;
     }
+    T.vol[j] = S.m[i] / S.rho[i];
+    if (!moving) return;
+    // cell code of every sorted slot (the inactive bucket is ncode)
+    T.occ[j] = S.cell_of[i];
+    if (!(S.info[i] & kActive)) T.nnbr[j] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -380,7 +423,8 @@ __device__ __forceinline__ void ghost_count_body(const Params &P, const Arrays &
         int i, j, k;
         slab_coords(g, s, i, j, k);
         const int c = ctl->glim[2] * (ctl->glim[1] * i + j) + k;
-        const int cnt = A.cell_start[c + 1] - A.cell_start[c];
+        const int q = cell_code(ctl, i, j, k);
+        const int cnt = A.cell_start[q + 1] - A.cell_start[q];
         A.slab_cnt[s] = (cnt > 0 && A.gcount[c] == 0) ? cnt : 0;
     }
 }
@@ -433,7 +477,7 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
         const int g0 = A.slab_off[s];
         A.gfirst[tc] = g0;
         A.gcount[tc] = cnt;
-        const int s0 = A.cell_start[c];
+        const int s0 = A.cell_start[cell_code(ctl, i, j, k)];
         for (int r = 0; r < cnt; r++) {
             const int src = s0 + r;              // descending deck number
             const int gi = P.N + g0 + r;
@@ -476,12 +520,30 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
 // MSolution::Strain (:334-338) for the fused step.
 // ---------------------------------------------------------------------------
 template <int D>
-__device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A, int shift, const int bid)
+__device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, const Ctl *ctl, int i,
+                                             const double *xi, double mi, uint32_t fi, int nn, double mind,
+                                             int shift)
+{
+    A.nnbr[i] = nn;
+    if (P.want_mindist) A.mindist[i] = mind;
+    if (shift) {
+        const double dt = ctl->dt;
+        double xs[D], vi[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            vi[d] = A.v[d][i];
+            xs[d] = xi[d] - 0.5 * vi[d] * dt;
+            A.xs[d][i] = xs[d];
+        }
+        put_srec<D>(A, i, xs, mi / A.rho[i], vi, fi);
+    }
+}
+
+// one particle's walk (the batched path, and the fallback of the cell kernel)
+template <int D>
+__device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A, int shift, const int i)
 {
     Ctl *ctl = A.ctl;
-    if (CTL_STOP(ctl)) return;
-    const int i = bid * kBlock + threadIdx.x;
-    if (i >= P.N) return;
     const uint32_t fi = A.info[i];
     const double mi = A.m[i];
     A.vol[i] = mi / A.rho[i];
@@ -502,72 +564,289 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
     double cut2 = P.pc[mat_i][mat_i].cut2;
     double mind = 1.0e20;
     int nn = 0;
-    const size_t stride = (size_t)P.Ncap;
-    // The innermost looped axis (y in 2D, z in 3D) is contiguous in cell index
-    // and, cells being sorted ranges, in storage: a run of up to 3 cells with
-    // no periodic images is one storage range.
-    auto test = [&](int j) -> bool {
-        double d2 = 0.0;
+    // Cells x -> y -> z ascending (z fastest), walked as ONE loop over this
+    // particle's candidates, so that lanes of a wave whose particles lie in
+    // different cells still execute a single loop body.  The cursor (r, s, t)
+    // carries the code bits of each coordinate (q0 | q1 | q2 is the cell code),
+    // stepped by masked increments.  Following cells of a row whose codes are
+    // consecutive and that hold no images extend the open storage range.
+    // (Plain scalars, no lambdas: captured arrays would be kept in scratch.)
+    const unsigned int m0 = ctl->cmask[0], m1 = ctl->cmask[1], m2 = ctl->cmask[2];
+    const int lo0 = lo[0], lo1 = lo[1], lo2 = lo[2], hi0 = hi[0], hi1 = hi[1], hi2 = hi[2];
+    const int q1lo = code_dep(lo1, m1), q2lo = code_dep(lo2, m2);
+    int r = lo0, s = lo1, t = lo2;
+    int q0 = code_dep(lo0, m0), q1 = q1lo, q2 = q2lo;
+    // Squared distance from xi to the faces of its own cell: a stencil cell
+    // whose box lies beyond the largest cut-off (with a relative margin far
+    // above FP64 rounding) holds no neighbour and is not walked.
+    double dlo2[3] = {0.0, 0.0, 0.0}, dhi2[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            const double dd = xi[d] - A.x[d][j];
-            d2 += dd * dd;
-        }
-        if (!onemat) cut2 = P.pc[mat_i][info_mat(A.info[j])].cut2;
-        if (d2 < cut2) {
-            if (P.want_mindist) {
-                const double r1 = sqrt(d2);
-                mind = mind < r1 ? mind : r1;
-            }
-            if (nn >= P.maxnbr) return false;
-            A.nbr[(size_t)nn * stride + i] = j;
-            nn++;
-        }
-        return true;
-    };
+    for (int d = 0; d < D; d++) {
+        const double f0 = xi[d] - (ctl->gmin[d] + bc[d] / ctl->gsize[d]);
+        const double f1 = (ctl->gmin[d] + (bc[d] + 1) / ctl->gsize[d]) - xi[d];
+        dlo2[d] = f0 > 0.0 ? f0 * f0 : 0.0;
+        dhi2[d] = f1 > 0.0 ? f1 * f1 : 0.0;
+    }
+    const double skip2 = P.cut2max * (1.0 + 1.0e-9);
+    const int bc0 = bc[0], bc1 = bc[1], bc2 = bc[2];
+    const double dl0 = dlo2[0], dl1 = dlo2[1], dl2 = dlo2[2], dh0 = dhi2[0], dh1 = dhi2[1], dh2 = dhi2[2];
+    const int inner = D == 3 ? 1 : (D == 2 ? 2 : 3);     // level of the innermost walked axis
+    int cells = 1;                 // level of the last cursor step; 0: walk finished
+    int gleft = 0, gbase = 0;      // images of the open cell still to test (newest first)
+    int jc = 0, e = 0;             // real particles still to test
     bool ok = true;
-    const int inner = D == 3 ? 2 : (D == 2 ? 1 : 0);
-    for (int r = (inner == 0 ? 0 : lo[0]); r <= (inner == 0 ? 0 : hi[0]) && ok; r++)
-        for (int s = (inner == 1 ? 0 : lo[1]); s <= (inner == 1 ? 0 : hi[1]) && ok; s++) {
-            int cc[3] = {r, s, 0};
-            cc[inner] = lo[inner];
-            const int c0 = gl2 * (gl1 * cc[0] + cc[1]) + cc[2];
-            const int nrun = hi[inner] - lo[inner] + 1;      // cells c0 .. c0 + nrun - 1
-            bool imgs = false;
-            if (P.periodic)
-                for (int q = 0; q < nrun; q++) imgs |= A.gcount[c0 + q] != 0;
-            if (!imgs) {
-                const int s0 = A.cell_start[c0], s1 = A.cell_start[c0 + nrun];
-                for (int j = s0; j < s1 && ok; j++)
-                    if (j != i) ok = test(j);
-            } else {
-                for (int q = 0; q < nrun && ok; q++) {
-                    const int c = c0 + q;
-                    const int ng = A.gcount[c], g0 = ng ? A.gfirst[c] : 0;
-                    for (int g = ng - 1; g >= 0 && ok; g--) ok = test(P.N + g0 + g);   // newest image first
-                    const int s0 = A.cell_start[c], s1 = A.cell_start[c + 1];
-                    for (int j = s0; j < s1 && ok; j++)
-                        if (j != i) ok = test(j);
+    // The walk yields segments of consecutive slots: a cell's images (newest
+    // first), then merged real ranges.
+    while (ok) {
+        int base, cnt, dir;
+        if (gleft > 0) {
+            base = P.N + gbase + gleft - 1;
+            cnt = gleft;
+            dir = -1;
+            gleft = 0;
+        } else if (jc < e) {
+            base = jc;
+            cnt = e - jc;
+            dir = 1;
+            jc = e;
+        } else if (cells) {
+            int q = q0 | q1 | q2;
+            const double box2 = (r < bc0 ? dl0 : (r > bc0 ? dh0 : 0.0)) + (s < bc1 ? dl1 : (s > bc1 ? dh1 : 0.0)) +
+                                (t < bc2 ? dl2 : (t > bc2 ? dh2 : 0.0));
+            const bool near = box2 <= skip2;
+            if (near) {
+                if (P.periodic) {
+                    const int c = gl2 * (gl1 * r + s) + t;
+                    gleft = A.gcount[c];
+                    if (gleft) gbase = A.gfirst[c];
                 }
+                jc = A.cell_start[q];
+                e = A.cell_start[q + 1];
+            }
+            // step the cursor; merge the following cells of the row while possible
+            while (true) {
+                if (t < hi2) { t++; q2 = (int)((((unsigned)q2 | ~m2) + 1u) & m2); cells = 1; }
+                else {
+                    t = lo2; q2 = q2lo;
+                    if (s < hi1) { s++; q1 = (int)((((unsigned)q1 | ~m1) + 1u) & m1); cells = 2; }
+                    else {
+                        s = lo1; q1 = q1lo;
+                        if (r < hi0) { r++; q0 = (int)((((unsigned)q0 | ~m0) + 1u) & m0); cells = 3; }
+                        else cells = 0;
+                    }
+                }
+                if (!near || cells != inner || (q0 | q1 | q2) != q + 1) break;
+                if (P.periodic && A.gcount[gl2 * (gl1 * r + s) + t] != 0) break;
+                const double nb2 = (r < bc0 ? dl0 : (r > bc0 ? dh0 : 0.0)) + (s < bc1 ? dl1 : (s > bc1 ? dh1 : 0.0)) +
+                                   (t < bc2 ? dl2 : (t > bc2 ? dh2 : 0.0));
+                if (nb2 > skip2) break;
+                q++;
+                e = A.cell_start[q + 1];
+            }
+            continue;
+        } else {
+            break;
+        }
+        for (int u = 0; u < cnt; u++) {
+            const int cand = base + dir * u;
+            if (cand == i) continue;
+            double d2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                const double dd = xi[d] - A.x[d][cand];
+                d2 += dd * dd;
+            }
+            if (!onemat) cut2 = P.pc[mat_i][info_mat(A.info[cand])].cut2;
+            if (d2 < cut2) {
+                if (P.want_mindist) {
+                    const double r1 = sqrt(d2);
+                    mind = mind < r1 ? mind : r1;
+                }
+                if (nn >= P.maxnbr) { ok = false; break; }
+                A.nbr[nbr_slot(nn, i, P.Ncap)] = cand;
+                nn++;
             }
         }
+    }
     if (!ok) {
         set_err(ctl, kErrNbrOverflow, A.orig[i]);
         A.nnbr[i] = nn;
         return;
     }
-    A.nnbr[i] = nn;
-    if (P.want_mindist) A.mindist[i] = mind;
-    if (shift) {
-        const double dt = ctl->dt;
-        double xs[D], vi[D];
+    nbr_epilogue<D>(P, A, ctl, i, xi, mi, fi, nn, mind, shift);
+}
+
+template <int D>
+__device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A, int shift, const int bid)
+{
+    if (CTL_STOP(A.ctl)) return;
+    const int i = bid * kBlock + threadIdx.x;
+    if (i >= P.N) return;
+    neighbours_one<D>(P, A, shift, i);
+}
+
+// ---------------------------------------------------------------------------
+// Cell-centric neighbour search (single-scene path).  One wave per occupied
+// cell: the stencil's candidates, in the traversal order of the walk above
+// (cells x -> y -> z, images newest first, then the cell's slots), are staged
+// as slot indices in LDS; lanes then take 64 candidates at a time (coalesced
+// position loads) and every target of the cell is broadcast in turn: a ballot
+// of the pairs inside the cut-off and a prefix count append them to the
+// target's list in candidate order, i.e. in the reference's order.
+// ---------------------------------------------------------------------------
+constexpr int kCandCap = 1024;        // staged candidates per wave
+constexpr int kCellWaves = kBlock / 64;
+
+__device__ __forceinline__ int code_ext(int v, unsigned int m)
+{
+    int r = 0, b = 0;
+    while (m) {
+        const unsigned int low = m & (0u - m);
+        if ((unsigned)v & low) r |= 1 << b;
+        b++;
+        m ^= low;
+    }
+    return r;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+template <int D, bool ONEMAT>
+__device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A, int shift, int wave, int nwave,
+                                               int *stage)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int lane = threadIdx.x & 63;
+    // this wave's cells: those whose first slot lies in its share of the slots
+    const int j0 = (int)((long long)P.N * wave / nwave), j1 = (int)((long long)P.N * (wave + 1) / nwave);
+    const int ncode = ctl->ncode;
+    const unsigned int m0 = ctl->cmask[0], m1 = ctl->cmask[1], m2 = ctl->cmask[2];
+    const int gl1 = ctl->glim[1], gl2 = ctl->glim[2];
+    const size_t cstride = (size_t)P.Ncap * kNbrChunk;     // distance between list chunks
+    const unsigned int maxnbr = (unsigned int)P.maxnbr;
+    int jn = j0;
+    while (jn < j1) {
+        const int q = __builtin_amdgcn_readfirstlane(A.occ[jn]);
+        if (q >= ncode) break;                              // inactive bucket: the end
+        const int head = __builtin_amdgcn_readfirstlane(A.cell_start[q]);
+        jn = __builtin_amdgcn_readfirstlane(A.cell_start[q + 1]);
+        if (head < j0) continue;                            // cell owned by the previous wave
+        const int bc[3] = {code_ext(q, m0), code_ext(q, m1), code_ext(q, m2)};
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            vi[d] = A.v[d][i];
-            xs[d] = xi[d] - 0.5 * vi[d] * dt;
-            A.xs[d][i] = xs[d];
+            lo[d] = bc[d] - 1 > 0 ? bc[d] - 1 : 0;
+            hi[d] = bc[d] + 1 < ctl->glim[d] - 1 ? bc[d] + 1 : ctl->glim[d] - 1;
         }
-        put_srec<D>(A, i, xs, mi / A.rho[i], vi, fi);
+        // stage the candidate slots in traversal order; the code bits of the
+        // stencil coordinates advance by masked increments
+        int ncand = 0;
+        const int qs_lo = code_dep(lo[1], m1), qt_lo = code_dep(lo[2], m2);
+        int qr = code_dep(lo[0], m0);
+        for (int r = lo[0]; r <= hi[0]; r++, qr = (int)((((unsigned)qr | ~m0) + 1u) & m0)) {
+            int qs = qs_lo;
+            for (int s = lo[1]; s <= hi[1]; s++, qs = (int)((((unsigned)qs | ~m1) + 1u) & m1)) {
+                int qt = qt_lo;
+                for (int t = lo[2]; t <= hi[2]; t++, qt = (int)((((unsigned)qt | ~m2) + 1u) & m2)) {
+                    if (P.periodic) {
+                        const int c = gl2 * (gl1 * r + s) + t;
+                        const int ng = A.gcount[c];
+                        if (ng) {
+                            const int g0 = P.N + A.gfirst[c] + ng - 1;     // newest image first
+                            for (int u = lane; u < ng; u += 64)
+                                if (ncand + u < kCandCap) stage[ncand + u] = g0 - u;
+                            ncand += ng;
+                        }
+                    }
+                    const int qc = qr | qs | qt;
+                    const int s0 = A.cell_start[qc], len = A.cell_start[qc + 1] - s0;
+                    for (int u = lane; u < len; u += 64)
+                        if (ncand + u < kCandCap) stage[ncand + u] = s0 + u;
+                    ncand += len;
+                }
+            }
+        }
+        const int c0 = head, c1 = jn;
+        if (ncand > kCandCap) {
+            for (int tg = c0; tg < c1; tg += 64)
+                if (tg + lane < c1) neighbours_one<D>(P, A, shift, tg + lane);
+            continue;
+        }
+        // staged slots are read back by other lanes of the same wave (LDS
+        // operations of one wave complete in order)
+        __builtin_amdgcn_wave_barrier();
+        for (int tg = c0; tg < c1; tg += 64) {
+            const int nt = c1 - tg < 64 ? c1 - tg : 64;
+            const bool tv = lane < nt;
+            const int it = tv ? tg + lane : tg;         // lanes past the cell read a valid slot
+            double xi[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) xi[d] = A.x[d][it];
+            const uint32_t fi = A.info[it];
+            const double mi = A.m[it];
+            int nn = 0;
+            double mind = 1.0e20;
+            const int mat0 = info_mat((uint32_t)__builtin_amdgcn_readfirstlane((int)fi));
+            const double cut2_0 = P.pc[mat0][mat0].cut2;
+            for (int k0 = 0; k0 < ncand; k0 += 64) {
+                const int kk = k0 + lane;
+                const bool cv = kk < ncand;
+                const int cand = cv ? stage[kk] : tg;      // lanes past the list read a valid slot
+                double xc[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) xc[d] = A.x[d][cand];
+                const int matc = ONEMAT ? 0 : info_mat(A.info[cand]);
+                const int self = cv ? cand - tg : -1;      // equals t for the target itself
+                for (int t = 0; t < nt; t++) {
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        const double dd = readlane_d(xi[d], t) - xc[d];
+                        d2 += dd * dd;
+                    }
+                    double cut2 = cut2_0;
+                    if (!ONEMAT) {
+                        const int matt = info_mat((uint32_t)__builtin_amdgcn_readlane((int)fi, t));
+                        cut2 = P.pc[matt][matc].cut2;
+                    }
+                    const bool pass = cv && self != t && d2 < cut2;
+                    const unsigned long long pm = __builtin_amdgcn_ballot_w64(pass);
+                    if (pm) {
+                        const unsigned int nnt = (unsigned int)__builtin_amdgcn_readlane(nn, t);
+                        const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
+                        int *lst = A.nbr + (size_t)(tg + t) * kNbrChunk;
+                        if (pass && pos < maxnbr) lst[(size_t)(pos >> 3) * cstride + (pos & 7u)] = cand;
+                        nn += lane == t ? (int)__popcll(pm) : 0;
+                        if (P.want_mindist) {
+                            double r1 = pass ? sqrt(d2) : 1.0e20;
+#pragma unroll
+                            for (int o = 32; o >= 1; o >>= 1) {
+                                const double y = __shfl_xor(r1, o, 64);
+                                r1 = y < r1 ? y : r1;
+                            }
+                            if (lane == t) mind = r1 < mind ? r1 : mind;
+                        }
+                    }
+                }
+            }
+            if (tv) {
+                if (nn > P.maxnbr) {
+                    set_err(ctl, kErrNbrOverflow, A.orig[it]);
+                    A.nnbr[it] = P.maxnbr;
+                } else {
+                    nbr_epilogue<D>(P, A, ctl, it, xi, mi, fi, nn, mind, shift);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -677,10 +956,7 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
                 vi[d] = A.v[d][i];
                 g[d] = 0.0;
             }
-            const int nn = A.nnbr[i];
-            const size_t stride = (size_t)P.Ncap;
-            for (int k = 0; k < nn; k++) {
-                const int j = A.nbr[(size_t)k * stride + i];
+            for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
                 const double4 ra = A.rsA[j], rb = A.rsB[j];
                 const double volj = ra.w;
                 const PairConst &pc = P.pc[mat_i][P.nmat == 1 ? mat_i : info_mat((uint32_t)rb.w)];
@@ -690,7 +966,7 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
                 gradw64<D>(dw, xi, xj, pc);
 #pragma unroll
                 for (int d = 0; d < D; d++) g[d] += volj * (vj[d] - vi[d]) * dw[d];
-            }
+            });
 #pragma unroll
             for (int d = 0; d < D; d++) tr += 0.5 * (g[d] + g[d]);
             A.trd[i] = tr;
@@ -748,10 +1024,7 @@ __device__ __forceinline__ void normals_body(const Params &P, const Arrays &A, c
         xi[d] = A.x[d][i];
         nrm[d] = 0.0;
     }
-    const int nn = A.nnbr[i];
-    const size_t stride = (size_t)P.Ncap;
-    for (int k = 0; k < nn; k++) {
-        const int j = A.nbr[(size_t)k * stride + i];
+    for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
         const uint32_t fj = A.info[j];
         const double volj = A.m[j] / A.rho[j];
         const PairConst &pc = P.pc[mat_i][info_mat(fj)];
@@ -762,7 +1035,7 @@ __device__ __forceinline__ void normals_body(const Params &P, const Arrays &A, c
         const double f = volj * (double)(info_color(fj) - col_i);
 #pragma unroll
         for (int d = 0; d < D; d++) nrm[d] += f * dw[d];
-    }
+    });
     double in = 0.0;
 #pragma unroll
     for (int d = 0; d < D; d++) in += nrm[d] * nrm[d];
@@ -794,10 +1067,7 @@ __device__ __forceinline__ void curvature_body(const Params &P, const Arrays &A,
     }
     const double Ni = A.Nf[i];
     double div = 0.0;
-    const int nn = A.nnbr[i];
-    const size_t stride = (size_t)P.Ncap;
-    for (int k = 0; k < nn; k++) {
-        const int j = A.nbr[(size_t)k * stride + i];
+    for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
         const uint32_t fj = A.info[j];
         const bool img = fj & kGhost;        // images: N = 0, normals zero
         const double volj = A.m[j] / A.rho[j];
@@ -811,7 +1081,7 @@ __device__ __forceinline__ void curvature_body(const Params &P, const Arrays &A,
         for (int d = 0; d < D; d++) prod += ((img ? 0.0 : A.nn[d][j]) - nni[d]) * dw[d];
         const double Nj = img ? 0.0 : A.Nf[j];
         div += (Ni < Nj ? Ni : Nj) * volj * prod;
-    }
+    });
     A.kappa[i] = div;
 }
 
@@ -855,10 +1125,7 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
             xi[d] = A.x[d][i];
             vi[d] = A.v[d][i];
         }
-        const int nn = A.nnbr[i];
-        const size_t stride = (size_t)P.Ncap;
-        for (int k = 0; k < nn; k++) {
-            const int j = A.nbr[(size_t)k * stride + i];
+        for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
             const double4 ra = A.rmA[j], rb = A.rmB[j];
             const double2 rc = A.rmC[j];
             const uint32_t fj = (uint32_t)rc.y;
@@ -878,7 +1145,7 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
             const double sp = -mj * (rc.x + pr_i);
 #pragma unroll
             for (int d = 0; d < D; d++) ai[d] = ai[d] + sp * dw[d];
-            if (rig_i && (fj & kRigid)) continue;
+            if (rig_i && (fj & kRigid)) return;
             const double nu_j = P.mat[mat_j].nu;
             const double tv = mj * (nu_i * rho_i + nu_j * rho_j) * prod / (rho_i * rho_j) / (rab2 + pc.eps2);
 #pragma unroll
@@ -897,7 +1164,7 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
                     for (int d = 0; d < D; d++) ai[d] = ai[d] + sv * dw[d];
                 }
             }
-        }
+        });
         if (P.st) {
             const double f = -mi.sigma / rho_i * A.kappa[i];
 #pragma unroll
@@ -1250,7 +1517,20 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Para
     const int2 e = map[blockIdx.x];
     permute_body<D>(Pt[e.x], At[e.x], At2[e.x], e.y);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { neighbours_body<D>(P, A, shift, blockIdx.x); }
+#define XCD_BID(P)                                       \
+    const int nb_ = ((P).N + kBlock - 1) / kBlock;       \
+    const int bid_ = xcd_block(nb_);                     \
+    if (bid_ >= nb_) return;
+template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { XCD_BID(P); neighbours_body<D>(P, A, shift, bid_); }
+constexpr int kCellBlocks = 2048;       // fixed grid of the cell kernel (8192 waves)
+template <int D> __global__ __launch_bounds__(kBlock) void k_nbr_cells(Params P, Arrays A, int shift)
+{
+    __shared__ int stage[kCellWaves][kCandCap];
+    const int b = xcd_block(kCellBlocks);
+    const int wv = (int)(threadIdx.x >> 6);
+    if (P.nmat == 1) nbr_cells_body<D, true>(P, A, shift, b * kCellWaves + wv, kCellBlocks * kCellWaves, stage[wv]);
+    else nbr_cells_body<D, false>(P, A, shift, b * kCellWaves + wv, kCellBlocks * kCellWaves, stage[wv]);
+}
 template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *Pt, const Arrays *At, const int2 *map, int shift)
 {
     const int2 e = map[blockIdx.x];
@@ -1258,21 +1538,21 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const P
     neighbours_body<D>(Pb, Ab, shift, e.y);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_shift(Params P, Arrays A) { shift_body<D>(P, A, blockIdx.x); }
-template <int D> __global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask) { strain_body<D>(P, A, mask, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask) { XCD_BID(P); strain_body<D>(P, A, mask, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_strain(const Params *Pt, const Arrays *At, const int2 *map, int mask)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
     strain_body<D>(Pb, Ab, mask, e.y);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void k_normals(Params P, Arrays A) { normals_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_normals(Params P, Arrays A) { XCD_BID(P); normals_body<D>(P, A, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_normals(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
     if (Pb.st) normals_body<D>(Pb, Ab, e.y);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void k_curvature(Params P, Arrays A) { curvature_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_curvature(Params P, Arrays A) { XCD_BID(P); curvature_body<D>(P, A, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_curvature(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
@@ -1285,7 +1565,7 @@ __global__ void kb_advance(const Params *Pt, const Arrays *At, const int *sc, in
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ns) { SCENE_OF(sc[t]); advance_body(Pb, Ab.ctl); }
 }
-template <int D> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { momentum_body<D>(P, A, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { XCD_BID(P); momentum_body<D>(P, A, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
@@ -1320,7 +1600,7 @@ __global__ void k_canon(Params P, Arrays A, long long *out, int maxnbr)
     if (i >= P.N) return;
     const int nn = A.nnbr[i];
     for (int k = 0; k < nn && k < maxnbr; k++) {
-        const int j = A.nbr[(size_t)k * P.Ncap + i];
+        const int j = A.nbr[nbr_slot(k, i, P.Ncap)];
         long long cid;
         if (j < P.N) cid = A.orig[j];
         else {
@@ -1384,7 +1664,7 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st)
 {
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, shift));
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_nbr_cells<DD>, dim3(kCellBlocks), dim3(kBlock), 0, st, P, A, shift));
 }
 
 void launch_shift(const Params &P, const Arrays &A, hipStream_t st)
@@ -1394,13 +1674,13 @@ void launch_shift(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st)
 {
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_strain<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, mask));
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_strain<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A, mask));
 }
 
 void launch_normals(const Params &P, const Arrays &A, hipStream_t st)
 {
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_normals<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_curvature<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_normals<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A));
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_curvature<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A));
 }
 
 void launch_advance(const Params &P, const Arrays &A, hipStream_t st)
@@ -1410,7 +1690,7 @@ void launch_advance(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st)
 {
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_momentum<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_momentum<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A));
     if (P.rigid)
         hipLaunchKernelGGL(k_rigid_reduce, dim3(1), dim3(kRigidBlock), 0, st, P, A, rigid_kin);
 }

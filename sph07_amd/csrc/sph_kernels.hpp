@@ -5,6 +5,38 @@
 
 namespace sph {
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// A launch of xcd_grid(nb) workgroups read through xcd_block() hands every
+// XCD one contiguous range of logical blocks: with cell-sorted storage the
+// neighbour records a range gathers stay in that XCD's L2.
+constexpr int kXcd = 8;
+inline int xcd_grid(int nb) { return (nb + kXcd - 1) / kXcd * kXcd; }
+#ifdef __HIPCC__
+__device__ __forceinline__ int xcd_block(int nb)
+{
+    const int per = (nb + kXcd - 1) / kXcd;
+    return (int)(blockIdx.x % kXcd) * per + (int)(blockIdx.x / kXcd);
+}
+#endif
+
+#ifdef __HIPCC__
+// Calls f(j) for the nn entries of particle i in list order, reading the list
+// one chunk (two 16-byte loads) at a time.
+template <class F>
+__device__ __forceinline__ void for_each_nbr(const int *nbr, int i, int nn, int ncap, F &&f)
+{
+    for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
+        const int4 *p = reinterpret_cast<const int4 *>(nbr + nbr_slot(k0, i, ncap));
+        const int4 c0 = p[0], c1 = p[1];
+        const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        const int m = nn - k0;
+#pragma unroll
+        for (int u = 0; u < kNbrChunk; u++)
+            if (u < m) f(e[u]);
+    }
+}
+#endif
+
 // masks of launch_strain
 enum { M_STRAIN = 1, M_CONST = 2, M_CRIT = 4, M_INPLACE = 8 };
 // masks of launch_integrate

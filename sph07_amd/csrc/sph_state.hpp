@@ -22,6 +22,17 @@ namespace sph {
 constexpr int kMaxMat = 8;
 constexpr int kBlock = 256;
 
+// Neighbour lists are stored in chunks of kNbrChunk entries: entry k of
+// particle i sits at ((k / kNbrChunk) * Ncap + i) * kNbrChunk + k % kNbrChunk,
+// so a particle's consecutive entries share one 32-byte sector (its writes
+// merge in L2) and a wave reads a chunk of 64 lists in whole lines.
+constexpr int kNbrChunk = 8;
+inline __host__ __device__ size_t nbr_slot(int k, int i, int ncap)
+{
+    return ((size_t)(k / kNbrChunk) * (size_t)ncap + (size_t)i) * kNbrChunk + (size_t)(k % kNbrChunk);
+}
+inline __host__ __device__ int nbr_round(int maxnbr) { return (maxnbr + kNbrChunk - 1) / kNbrChunk * kNbrChunk; }
+
 // packed per-particle flags (info[])
 constexpr uint32_t kMatMask = 0xFFu;         // bits 0-7 material (1-based)
 constexpr int kDispShift = 8;                // bits 8-11 dispbc
@@ -50,11 +61,12 @@ struct Ctl {
     unsigned long long bbox_key[2][3];
     int glim[3];
     int ncell;
-    int nbucket;          // ncell + 1: the last bucket holds inactive particles
+    int ncode;            // size of the cell-code space (storage order of the cells)
+    int nbucket;          // ncode + 1: the last bucket holds inactive particles
+    unsigned int cmask[3];// code bits owned by each axis (bit-interleaved cell code)
     int timestep, done, err, ngp;
     int err_particle;
     int step_limit;       // > 0: the scene stops after this many steps (batched runs)
-    int pad_[1];
 };
 
 // Per-material constants (MMaterial.h:17-40 after CheckDefaults).
@@ -84,6 +96,7 @@ struct Params {
     double base_a[3];
     double minadd[3][3], maxadd[3][3];
     double hmax, tssf, endtime;
+    double cut2max, cut2min;                   // over material pairs
     double rb_M, rb_I;
     Mat mat[kMaxMat + 1];                      // 1-based
     PairConst pc[kMaxMat + 1][kMaxMat + 1];    // 1-based
@@ -100,6 +113,7 @@ struct Arrays {
     //   momentum:   rmA = {x[0..2], m}, rmB = {v[0..2], rho}, rmC = {p/rho^2, info}
     double4 *rsA, *rsB, *rmA, *rmB;
     double2 *rmC;
+
     uint32_t *info;
     int *nnbr, *nbr;
     // deck number of each slot, rigid-body rank (or -1) of each slot
@@ -112,6 +126,8 @@ struct Arrays {
     int *slab_cnt, *slab_off;
     int *scan_aux;
     int *rigid_idx;
+    // cell code of each sorted slot (written while the particles are sorted)
+    int *occ;
     Ctl *ctl;
 };
 
