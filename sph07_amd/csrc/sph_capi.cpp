@@ -174,6 +174,16 @@ int alloc_particles(Context *c, int gcap)
     rc |= dalloc(c, L, &A.rmA, ncap);
     rc |= dalloc(c, L, &A.rmB, ncap);
     rc |= dalloc(c, L, &A.rmC, ncap);
+    if (c->arm == SPH_ARM_FP32C) {
+        rc |= dalloc(c, L, &A.fsA, ncap);
+        rc |= dalloc(c, L, &A.fsB, ncap);
+        rc |= dalloc(c, L, &A.fmA, ncap);
+        rc |= dalloc(c, L, &A.fmB, ncap);
+        rc |= dalloc(c, L, &A.fmC, ncap);
+    } else {
+        A.fsA = A.fsB = A.fmA = A.fmB = nullptr;
+        A.fmC = nullptr;
+    }
     rc |= dalloc(c, L, &A.rho_new, ncap);
     rc |= dalloc(c, L, &A.p, ncap);
     rc |= dalloc(c, L, &A.prho2, ncap);
@@ -203,6 +213,7 @@ int alloc_particles(Context *c, int gcap)
         B.nn[d] = A.nn[d];
     }
     B.rsA = A.rsA; B.rsB = A.rsB; B.rmA = A.rmA; B.rmB = A.rmB; B.rmC = A.rmC;
+    B.fsA = A.fsA; B.fsB = A.fsB; B.fmA = A.fmA; B.fmB = A.fmB; B.fmC = A.fmC;
     B.rho_new = A.rho_new; B.p = A.p; B.prho2 = A.prho2; B.vol = A.vol;
     B.trd = A.trd; B.mindist = A.mindist; B.critts_p = A.critts_p; B.rhoold = A.rhoold;
     B.Nf = A.Nf; B.kappa = A.kappa; B.nnbr = A.nnbr; B.nbr = A.nbr;
@@ -422,6 +433,7 @@ int build_context(Context *c, const sph_scene_desc *d)
     P.ndim = D;
     P.tcrit_opt = d->tcrit_opt;
     P.maxnbr = c->maxnbr;
+    P.arm = c->arm;
     P.st = d->surface_tension != 0;
     P.rigid = d->rigid_body != 0;
     P.baseaccel = d->baseaccel != 0;

@@ -2,16 +2,18 @@
 //
 // Positions, velocities, densities, the time integration and every
 // per-particle update stay FP64 (the same k_integrate / k_advance / rigid
-// kernels as the FP64 arm).  Only the bulk pair sums run in FP32: the pair
-// separation and velocity difference are formed in FP64 (exact cancellation
-// of the large coordinates) and rounded once to FP32; the kernel gradient,
-// pressure, viscous and Monaghan terms are FP32 with hardware reciprocal and
-// reciprocal square root; each particle's sums are accumulated with the
-// TwoSum error-free transformation (Knuth 1969) into (sum, compensation)
-// FP32 pairs and converted to FP64 once at the end.
+// kernels as the FP64 arm).  Only the bulk pair sums run in FP32: each
+// neighbour is one compact FP32 record (sph_rec32.hpp) holding its position
+// relative to its cell corner, so the pair separation is rebuilt as
+// off * cs32 + (r_i - r_j) without cancellation of large coordinates; the
+// kernel gradient, pressure, viscous and Monaghan terms are FP32 with hardware
+// reciprocal and reciprocal square root; each particle's sums are accumulated
+// with the TwoSum error-free transformation (Knuth 1969) into (sum,
+// compensation) FP32 pairs and converted to FP64 once at the end.
 //
 // Neighbour sets and their order are those of the FP64 arm (shared lists).
 #include "sph_kernels.hpp"
+#include "sph_rec32.hpp"
 
 #include <math.h>
 
@@ -120,40 +122,45 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
         const double dt = ctl->dt;
         double rho_i = A.rho[i];
         if (act) {
-            double xi[D], vi[D];
-            float gs[D], gc[D];
+            float cs[D], ri[D], vi32[D], gs[D], gc[D];
+            const float4 fa = A.fsA[i], fb = A.fsB[i];
+            const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
+            const uint32_t pk_i = __float_as_uint(fb.w);
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                xi[d] = A.xs[d][i];
-                vi[d] = A.v[d][i];
+                cs[d] = cell_size32(ctl, d);
+                ri[d] = ri3[d];
+                vi32[d] = vi3[d];
                 gs[d] = 0.0f;
                 gc[d] = 0.0f;
             }
             const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
             for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
-                const double4 ra = A.rsA[j], rb = A.rsB[j];
-                const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][info_mat((uint32_t)rb.w)]);
-                const double xj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
+                const float4 ra = A.fsA[j], rb = A.fsB[j];
+                const uint32_t pk_j = __float_as_uint(rb.w);
+                const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][info_mat(pk_j)]);
+                const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
                 float dx[D], r2 = 0.0f;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
-                    dx[d] = (float)(xi[d] - xj[d]);
+                    dx[d] = cell_off(pk_i, pk_j, d) * cs[d] + (ri[d] - rj[d]);
                     r2 += dx[d] * dx[d];
                 }
-                const float f = gradw_factor(r2, q) * (float)ra.w;
+                const float f = gradw_factor(r2, q) * ra.w;
 #pragma unroll
                 for (int d = 0; d < D; d++)
-                    two_sum_acc(gs[d], gc[d], (float)(vj[d] - vi[d]) * (f * dx[d]));
+                    two_sum_acc(gs[d], gc[d], (vj[d] - vi32[d]) * (f * dx[d]));
             });
             double tr = 0.0;
 #pragma unroll
             for (int d = 0; d < D; d++) tr += (double)gs[d] + (double)gc[d];
             A.trd[i] = tr;
-            double xr[3] = {0.0, 0.0, 0.0}, vr[3] = {0.0, 0.0, 0.0};
+            double xi[D], vi[D], xr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int d = 0; d < D; d++) {
+                xi[d] = A.xs[d][i];
+                vi[d] = A.v[d][i];
                 xr[d] = xi[d] + 0.5 * vi[d] * dt;
-                vr[d] = vi[d];
                 A.x[d][i] = xr[d];
             }
             rho_i = rho_i * (1.0 - tr * dt);
@@ -161,12 +168,9 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
             const double pr = p / (rho_i * rho_i);
             A.p[i] = p;
             A.prho2[i] = pr;
-            double4 ma, mb;
-            ma.x = xr[0]; ma.y = xr[1]; ma.z = xr[2]; ma.w = A.m[i];
-            mb.x = vr[0]; mb.y = vr[1]; mb.z = vr[2]; mb.w = rho_i;
-            A.rmA[i] = ma;
-            A.rmB[i] = mb;
-            A.rmC[i] = make_double2(pr, (double)fi);
+            const int q = A.occ[i];
+            const int bc[3] = {code_ext(q, ctl->cmask[0]), code_ext(q, ctl->cmask[1]), code_ext(q, ctl->cmask[2])};
+            put_mrec32<D>(A, ctl, i, xr, A.m[i], vi, rho_i, pr, fi, bc);
         }
         A.rho_new[i] = rho_i;
         const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
@@ -203,39 +207,41 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const float rho_i = (float)rho_i64, pr_i = (float)A.prho2[i];
         const float q_i = (float)(mi.nu * rho_i64);
         const bool monaghan = mi.visc_type == 2;
-        double xi[D], vi[D];
-        float as[D], ac[D];
+        float cs[D], ri[D], vi[D], as[D], ac[D];
+        const float4 fa = A.fmA[i], fb = A.fmB[i];
+        const uint32_t pk_i = __float_as_uint(A.fmC[i].y);
+        const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            xi[d] = A.x[d][i];
-            vi[d] = A.v[d][i];
+            cs[d] = cell_size32(ctl, d);
+            ri[d] = ri3[d];
+            vi[d] = vi3[d];
             as[d] = 0.0f;
             ac[d] = 0.0f;
         }
         const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
         for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
-            const double4 ra = A.rmA[j], rb = A.rmB[j];
-            const double2 rc = A.rmC[j];
-            const uint32_t fj = (uint32_t)rc.y;
+            const float4 ra = A.fmA[j], rb = A.fmB[j];
+            const float2 rc = A.fmC[j];
+            const uint32_t fj = __float_as_uint(rc.y);
             const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
             const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][mat_j]);
-            const float mj = (float)ra.w;
-            const double rho_j64 = rb.w;
-            const float rho_j = (float)rho_j64;
-            const double xj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
+            const float mj = ra.w;
+            const float rho_j = rb.w;
+            const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
             float dx[D], dv[D], r2 = 0.0f, vr = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                dx[d] = (float)(xi[d] - xj[d]);
-                dv[d] = (float)(vi[d] - vj[d]);
+                dx[d] = cell_off(pk_i, fj, d) * cs[d] + (ri[d] - rj[d]);
+                dv[d] = vi[d] - vj[d];
                 r2 += dx[d] * dx[d];
                 vr += dv[d] * dx[d];
             }
             const float f = gradw_factor(r2, q);            // grad W = f * dx
-            float sdw = -mj * ((float)rc.x + pr_i);         // coefficient of grad W
+            float sdw = -mj * (rc.x + pr_i);                // coefficient of grad W
             float tv = 0.0f;                                // coefficient of (v_i - v_j)
             if (!(rig_i && (fj & kRigid))) {
-                const float qj = (float)(P.mat[mat_j].nu * rho_j64);
+                const float qj = (float)P.mat[mat_j].nu * rho_j;
                 const float inv = __builtin_amdgcn_rcpf(rho_i * rho_j * (r2 + q.eps2));
                 tv = mj * (q_i + qj) * (f * r2) * inv;
                 if (monaghan && vr < 0.0f) {

@@ -9,6 +9,7 @@
 // at once when a previous kernel of the step flagged an error or the stop
 // test fired, so a queued sequence of steps is a no-op after either.
 #include "sph_kernels.hpp"
+#include "sph_rec32.hpp"
 
 #include <math.h>
 
@@ -506,6 +507,14 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
             case 2: put_srec<2>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<2>(A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
             default: put_srec<3>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<3>(A, gi, gx, mass, gv, rho, 0.0, ginfo);
             }
+            if (P.arm == 1) {
+                const int tb[3] = {i + g.off[0], j + g.off[1], k + g.off[2]};   // the image's cell
+                switch (P.ndim) {
+                case 1: put_srec32<1>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<1>(A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb); break;
+                case 2: put_srec32<2>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<2>(A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb); break;
+                default: put_srec32<3>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<3>(A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb);
+                }
+            }
             A.gsrc[g0 + r] = src;
             A.gshift[g0 + r] = shift_code;
         }
@@ -522,7 +531,7 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
 template <int D>
 __device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, const Ctl *ctl, int i,
                                              const double *xi, double mi, uint32_t fi, int nn, double mind,
-                                             int shift)
+                                             int shift, const int *bc)
 {
     A.nnbr[i] = nn;
     if (P.want_mindist) A.mindist[i] = mind;
@@ -535,7 +544,8 @@ __device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, c
             xs[d] = xi[d] - 0.5 * vi[d] * dt;
             A.xs[d][i] = xs[d];
         }
-        put_srec<D>(A, i, xs, mi / A.rho[i], vi, fi);
+        if (P.arm == 1) put_srec32<D>(A, ctl, i, xs, mi / A.rho[i], vi, fi, bc);
+        else put_srec<D>(A, i, xs, mi / A.rho[i], vi, fi);
     }
 }
 
@@ -673,7 +683,7 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
         A.nnbr[i] = nn;
         return;
     }
-    nbr_epilogue<D>(P, A, ctl, i, xi, mi, fi, nn, mind, shift);
+    nbr_epilogue<D>(P, A, ctl, i, xi, mi, fi, nn, mind, shift, bc);
 }
 
 template <int D>
@@ -696,18 +706,6 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
 // ---------------------------------------------------------------------------
 constexpr int kCandCap = 1024;        // staged candidates per wave
 constexpr int kCellWaves = kBlock / 64;
-
-__device__ __forceinline__ int code_ext(int v, unsigned int m)
-{
-    int r = 0, b = 0;
-    while (m) {
-        const unsigned int low = m & (0u - m);
-        if ((unsigned)v & low) r |= 1 << b;
-        b++;
-        m ^= low;
-    }
-    return r;
-}
 
 __device__ __forceinline__ double readlane_d(double v, int l)
 {
@@ -842,7 +840,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
                     set_err(ctl, kErrNbrOverflow, A.orig[it]);
                     A.nnbr[it] = P.maxnbr;
                 } else {
-                    nbr_epilogue<D>(P, A, ctl, it, xi, mi, fi, nn, mind, shift);
+                    nbr_epilogue<D>(P, A, ctl, it, xi, mi, fi, nn, mind, shift, bc);
                 }
             }
         }

@@ -1,0 +1,82 @@
+// sph_rec32.hpp -- compact FP32 gather records of the FP32-compensated arm.
+//
+// Positions are stored relative to the lower corner of the particle's bin
+// cell, origin(c) = gmin + c * cs32, where cs32 is the cell size rounded to
+// FP32.  Two adjacent cells' origins then differ by exactly +-cs32, so a pair
+// separation is rebuilt as off * cs32 + (r_i - r_j) with off in {-1, 0, 1}
+// taken from the low two bits of each cell coordinate (kept in the packed
+// word).  Relative positions are below one cell size, so FP32 keeps them to
+// ~1e-7 of the cut-off.
+#pragma once
+
+#include "sph_state.hpp"
+
+namespace sph {
+
+constexpr int kCellBitsShift = 16;     // packed word: info (material, rigid) | cell bits
+
+__device__ __forceinline__ float cell_size32(const Ctl *ctl, int d) { return (float)(1.0 / ctl->gsize[d]); }
+
+__device__ __forceinline__ double cell_origin(const Ctl *ctl, int d, int c)
+{
+    return ctl->gmin[d] + (double)c * (double)cell_size32(ctl, d);
+}
+
+__device__ __forceinline__ uint32_t pack32(uint32_t info, const int *bc)
+{
+    return (info & (kMatMask | kRigid)) | ((uint32_t)(bc[0] & 3) << kCellBitsShift) |
+           ((uint32_t)(bc[1] & 3) << (kCellBitsShift + 2)) | ((uint32_t)(bc[2] & 3) << (kCellBitsShift + 4));
+}
+
+// cell offset (-1, 0, 1) along axis d of particle a relative to particle b
+__device__ __forceinline__ float cell_off(uint32_t pa, uint32_t pb, int d)
+{
+    const int s = kCellBitsShift + 2 * d;
+    const int o = (((int)((pa >> s) & 3u) - (int)((pb >> s) & 3u) + 1) & 3) - 1;
+    return (float)o;
+}
+
+// inverse of the cell-code deposit: the cell coordinate held in the bits m of v
+__device__ __forceinline__ int code_ext(int v, unsigned int m)
+{
+    int r = 0, b = 0;
+    while (m) {
+        const unsigned int low = m & (0u - m);
+        if ((unsigned)v & low) r |= 1 << b;
+        b++;
+        m ^= low;
+    }
+    return r;
+}
+
+// continuity record: {r[0..2], vol}, {v[0..2], packed}
+template <int D>
+__device__ __forceinline__ void put_srec32(const Arrays &A, const Ctl *ctl, int j, const double *xs, double vol,
+                                           const double *v, uint32_t info, const int *bc)
+{
+    float4 a = make_float4(0.f, 0.f, 0.f, (float)vol), b = make_float4(0.f, 0.f, 0.f, 0.f);
+    a.x = (float)(xs[0] - cell_origin(ctl, 0, bc[0]));
+    b.x = (float)v[0];
+    if (D > 1) { a.y = (float)(xs[1] - cell_origin(ctl, 1, bc[1])); b.y = (float)v[1]; }
+    if (D > 2) { a.z = (float)(xs[2] - cell_origin(ctl, 2, bc[2])); b.z = (float)v[2]; }
+    b.w = __uint_as_float(pack32(info, bc));
+    A.fsA[j] = a;
+    A.fsB[j] = b;
+}
+
+// momentum record: {r[0..2], m}, {v[0..2], rho}, {p/rho^2, packed}
+template <int D>
+__device__ __forceinline__ void put_mrec32(const Arrays &A, const Ctl *ctl, int j, const double *x, double m,
+                                           const double *v, double rho, double prho2, uint32_t info, const int *bc)
+{
+    float4 a = make_float4(0.f, 0.f, 0.f, (float)m), b = make_float4(0.f, 0.f, 0.f, (float)rho);
+    a.x = (float)(x[0] - cell_origin(ctl, 0, bc[0]));
+    b.x = (float)v[0];
+    if (D > 1) { a.y = (float)(x[1] - cell_origin(ctl, 1, bc[1])); b.y = (float)v[1]; }
+    if (D > 2) { a.z = (float)(x[2] - cell_origin(ctl, 2, bc[2])); b.z = (float)v[2]; }
+    A.fmA[j] = a;
+    A.fmB[j] = b;
+    A.fmC[j] = make_float2((float)prho2, __uint_as_float(pack32(info, bc)));
+}
+
+}  // namespace sph

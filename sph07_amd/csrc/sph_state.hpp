@@ -90,6 +90,7 @@ struct Params {
     int nmat, rb_id, n_rigid;
     int ncell_cap;
     int want_mindist;
+    int arm;              // 0 FP64, 1 FP32-compensated (selects the records the fused step writes)
     int btype[3];
     int bcode[2][3];
     double bx[2][3];
@@ -113,6 +114,11 @@ struct Arrays {
     //   momentum:   rmA = {x[0..2], m}, rmB = {v[0..2], rho}, rmC = {p/rho^2, info}
     double4 *rsA, *rsB, *rmA, *rmB;
     double2 *rmC;
+    // FP32 records of the FP32-compensated arm (cell-relative positions):
+    //   continuity: fsA = {r[0..2], vol}, fsB = {v[0..2], packed}
+    //   momentum:   fmA = {r[0..2], m}, fmB = {v[0..2], rho}, fmC = {p/rho^2, packed}
+    float4 *fsA, *fsB, *fmA, *fmB;
+    float2 *fmC;
 
     uint32_t *info;
     int *nnbr, *nbr;
