@@ -231,6 +231,7 @@ void share_cells(Context *c)
     B.slab_cnt = A.slab_cnt; B.slab_off = A.slab_off; B.scan_aux = A.scan_aux;
     B.rigid_idx = A.rigid_idx;
     B.ctl = A.ctl;
+    B.bbox = A.bbox;
 }
 
 int alloc_cells(Context *c, int cap)
@@ -623,8 +624,21 @@ int create_impl(Context *c, const sph_scene_desc *d)
     HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIPCHK(c, hipHostMalloc((void **)&c->hctl, sizeof(Ctl), hipHostMallocDefault));
     memset(c->hctl, 0, sizeof(Ctl));
+    // the control block and the bounding-box slots live as long as the context
+    // (not part of the particle allocations): take them off the list
     if (dalloc(c, c->part_allocs, &c->AA[0].ctl, 1)) return SPH_ERR_HIP;
-    // the control block is not part of part_allocs' lifetime: keep it separate
+    c->part_allocs.pop_back();
+    if (dalloc(c, c->part_allocs, &c->AA[0].bbox, (size_t)kBboxSlots * kBboxStride)) return SPH_ERR_HIP;
+    {
+        std::vector<unsigned long long> hb((size_t)kBboxSlots * kBboxStride, 0ull);
+        for (int s = 0; s < kBboxSlots; s++)
+            for (int m = 0; m < 3; m++) {
+                hb[(size_t)s * kBboxStride + m] = dkey(1.0e20);
+                hb[(size_t)s * kBboxStride + 3 + m] = dkey(-1.0e20);
+                hb[(size_t)s * kBboxStride + kCritSlot] = dkey(1.0e20);
+            }
+        HIPCHK(c, hipMemcpy(c->AA[0].bbox, hb.data(), hb.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    }
     c->part_allocs.pop_back();
     if (alloc_particles(c, gcap)) return SPH_ERR_HIP;
     if (alloc_cells(c, (int)cap)) return SPH_ERR_HIP;
@@ -675,7 +689,6 @@ int create_impl(Context *c, const sph_scene_desc *d)
     for (int s = 0; s < 2; s++)
         for (int m = 0; m < 3; m++) {
             h.cmm[s][m] = cmm[s][m];
-            h.bbox_key[s][m] = dkey(s == 0 ? 1.0e20 : -1.0e20);
         }
     h.critts_key = dkey(1.0e20);
     if (push_ctl(c)) return SPH_ERR_HIP;
@@ -684,6 +697,7 @@ int create_impl(Context *c, const sph_scene_desc *d)
         sort_step(c);
         launch_neighbours(c->P, cur(c), 0, c->st);
         launch_strain(c->P, cur(c), M_CRIT, c->st);
+        launch_crit_fold(c->P, cur(c), c->st);
     });
     if (rc) return rc;
     const double critts = dunkey(h.critts_key);
@@ -756,6 +770,7 @@ void sph_destroy(void *ctx)
     free_list(c->part_allocs);
     free_list(c->cell_allocs);
     if (c->AA[0].ctl) hipFree(c->AA[0].ctl);
+    if (c->AA[0].bbox) hipFree(c->AA[0].bbox);
     if (c->AA[0].rigid_idx) hipFree(c->AA[0].rigid_idx);
     if (c->canon) hipFree(c->canon);
     if (c->hctl) hipHostFree(c->hctl);
@@ -815,7 +830,10 @@ int sph_constitutive(void *ctx)
 int sph_calc_critical_timestep(void *ctx, double *critts)
 {
     Context *c = ctx_of(ctx);
-    int rc = run_phase(c, [&] { launch_strain(c->P, cur(c), M_CRIT, c->st); });
+    int rc = run_phase(c, [&] {
+        launch_strain(c->P, cur(c), M_CRIT, c->st);
+        launch_crit_fold(c->P, cur(c), c->st);
+    });
     if (rc == 0 && critts) *critts = dunkey(c->hctl->critts_key);
     return rc;
 }

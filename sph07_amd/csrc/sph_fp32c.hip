@@ -183,7 +183,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
     if (threadIdx.x == 0) {
         double m = red[0];
         for (int k = 1; k < kBlock / 64; k++) m = fmin(m, red[k]);
-        atomicMin(&ctl->critts_key, dkey(m));
+        atomicMin(A.bbox + (size_t)(bid % kBboxSlots) * kBboxStride + kCritSlot, dkey(m));
     }
 }
 

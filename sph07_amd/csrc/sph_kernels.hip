@@ -911,8 +911,10 @@ __device__ __forceinline__ double crit_one(const Params &P, const Arrays &A, con
     return ct;
 }
 
-__device__ __forceinline__ void block_min_to(unsigned long long *key, double v)
+// block minimum into the block's reduction slot (atomics spread over lines)
+__device__ __forceinline__ void block_min_to(const Arrays &A, int bid, double v)
 {
+    unsigned long long *key = A.bbox + (size_t)(bid % kBboxSlots) * kBboxStride + kCritSlot;
     __shared__ double red[kBlock / 64];
     v = wave_min(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -999,7 +1001,7 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
             if (act && !(fi & kRigid)) crit = ct;
         }
     }
-    if (mask & S_CRIT) block_min_to(&ctl->critts_key, crit);
+    if (mask & S_CRIT) block_min_to(A, bid, crit);
 }
 
 // ---------------------------------------------------------------------------
@@ -1084,9 +1086,29 @@ __device__ __forceinline__ void curvature_body(const Params &P, const Arrays &A,
 }
 
 // dt rule of MSolution::Solution (:275-277); resets the min key
-__device__ __forceinline__ void advance_body(const Params &P, Ctl *ctl)
+// one wave: folds the critical-step slots into Ctl::critts_key and resets them
+__device__ __forceinline__ void crit_fold_body(const Arrays &A)
 {
+    Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
+    unsigned long long *slot = A.bbox + (size_t)threadIdx.x * kBboxStride + kCritSlot;
+    unsigned long long k = *slot;
+    *slot = dkey(1.0e20);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long y = __shfl_xor(k, o, 64);
+        k = y < k ? y : k;
+    }
+    if (threadIdx.x == 0 && k < ctl->critts_key) ctl->critts_key = k;
+}
+
+// one wave (lane 0 applies the rule after the fold)
+__device__ __forceinline__ void advance_body(const Params &P, const Arrays &A)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    crit_fold_body(A);
+    if (threadIdx.x != 0) return;
     const double critts = dunkey(ctl->critts_key);
     ctl->critts = critts;
     ctl->critts_key = dkey(1.0e20);
@@ -1367,6 +1389,7 @@ __device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A,
         }
         __syncthreads();
         if (threadIdx.x == 0) {
+            unsigned long long *slot = A.bbox + (size_t)(bid % kBboxSlots) * kBboxStride;
 #pragma unroll
             for (int d = 0; d < D; d++) {
                 double l = red[0][d][0], h = red[1][d][0];
@@ -1374,8 +1397,8 @@ __device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A,
                     l = fmin(l, red[0][d][k]);
                     h = fmax(h, red[1][d][k]);
                 }
-                atomicMin(&ctl->bbox_key[0][d], dkey(l));
-                atomicMax(&ctl->bbox_key[1][d], dkey(h));
+                atomicMin(&slot[d], dkey(l));
+                atomicMax(&slot[3 + d], dkey(h));
             }
         }
     }
@@ -1383,18 +1406,38 @@ __device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A,
 
 // bounding box of MoveParticles into coord_minmax; with end != 0 also the
 // step counter and stop test of MSolution::Solution (:317-319)
-__device__ __forceinline__ void finish_body(const Params &P, Ctl *ctl, int bbox, int end)
+// one wave: lane l reduces and resets bounding-box slot l
+__device__ __forceinline__ void finish_body(const Params &P, const Arrays &A, int bbox, int end)
 {
+    Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
+    const int lane = threadIdx.x;
     if (bbox) {
+        static_assert(kBboxSlots == 64, "one slot per lane");
+        unsigned long long *slot = A.bbox + (size_t)lane * kBboxStride;
+        unsigned long long kmin[3], kmax[3];
+#pragma unroll
         for (int d = 0; d < 3; d++) {
-            ctl->cmm[0][d] = d < P.ndim ? dunkey(ctl->bbox_key[0][d]) : 1.0e20;
-            ctl->cmm[1][d] = d < P.ndim ? dunkey(ctl->bbox_key[1][d]) : -1.0e20;
-            ctl->bbox_key[0][d] = dkey(1.0e20);
-            ctl->bbox_key[1][d] = dkey(-1.0e20);
+            kmin[d] = slot[d];
+            kmax[d] = slot[3 + d];
+            slot[d] = dkey(1.0e20);
+            slot[3 + d] = dkey(-1.0e20);
         }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const unsigned long long a = __shfl_xor(kmin[d], o, 64), b = __shfl_xor(kmax[d], o, 64);
+                kmin[d] = a < kmin[d] ? a : kmin[d];
+                kmax[d] = b > kmax[d] ? b : kmax[d];
+            }
+        if (lane == 0)
+            for (int d = 0; d < 3; d++) {
+                ctl->cmm[0][d] = d < P.ndim ? dunkey(kmin[d]) : 1.0e20;
+                ctl->cmm[1][d] = d < P.ndim ? dunkey(kmax[d]) : -1.0e20;
+            }
     }
-    if (end) {
+    if (end && lane == 0) {
         ctl->timestep++;
         if (ctl->ptime - ctl->dt >= P.endtime) ctl->done = 1;
         if (ctl->step_limit > 0 && ctl->timestep - 1 >= ctl->step_limit) ctl->done = 1;
@@ -1557,11 +1600,13 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_curvature(const Pa
     SCENE_OF(e.x);
     if (Pb.st) curvature_body<D>(Pb, Ab, e.y);
 }
-__global__ void k_advance(Params P, Ctl *ctl) { advance_body(P, ctl); }
-__global__ void kb_advance(const Params *Pt, const Arrays *At, const int *sc, int ns)
+__global__ __launch_bounds__(64) void k_advance(Params P, Arrays A) { advance_body(P, A); }
+__global__ __launch_bounds__(64) void k_crit_fold(Arrays A) { crit_fold_body(A); }
+// one wave per scene
+__global__ __launch_bounds__(64) void kb_advance(const Params *Pt, const Arrays *At, const int *sc)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ns) { SCENE_OF(sc[t]); advance_body(Pb, Ab.ctl); }
+    SCENE_OF(sc[blockIdx.x]);
+    advance_body(Pb, Ab);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { XCD_BID(P); momentum_body<D>(P, A, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Params *Pt, const Arrays *At, const int2 *map)
@@ -1584,11 +1629,12 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_integrate(const Pa
     SCENE_OF(e.x);
     integrate_body<D>(Pb, Ab, mask, e.y);
 }
-__global__ void k_finish(Params P, Ctl *ctl, int bbox, int end) { finish_body(P, ctl, bbox, end); }
-__global__ void kb_finish(const Params *Pt, const Arrays *At, const int *sc, int ns, int bbox, int end)
+__global__ __launch_bounds__(64) void k_finish(Params P, Arrays A, int bbox, int end) { finish_body(P, A, bbox, end); }
+// one wave per scene
+__global__ __launch_bounds__(64) void kb_finish(const Params *Pt, const Arrays *At, const int *sc, int bbox, int end)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ns) { SCENE_OF(sc[t]); finish_body(Pb, Ab.ctl, bbox, end); }
+    SCENE_OF(sc[blockIdx.x]);
+    finish_body(Pb, Ab, bbox, end);
 }
 
 // canonical neighbour ids (tests and the sph_get_neighbours entry point)
@@ -1683,7 +1729,13 @@ void launch_normals(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_advance(const Params &P, const Arrays &A, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, P, A.ctl);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, P, A);
+}
+
+void launch_crit_fold(const Params &P, const Arrays &A, hipStream_t st)
+{
+    (void)P;
+    hipLaunchKernelGGL(k_crit_fold, dim3(1), dim3(64), 0, st, A);
 }
 
 void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st)
@@ -1711,7 +1763,7 @@ void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st
 
 void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, st, P, A.ctl, bbox, end);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, P, A, bbox, end);
 }
 
 void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st)
@@ -1750,12 +1802,12 @@ void launch_batch_step(const BatchView &B, hipStream_t st)
         DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_normals<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap));
         DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_curvature<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap));
     }
-    hipLaunchKernelGGL(kb_advance, dim3(sb), dim3(kBlock), 0, st, B.P, A2, B.scenes, ns);
+    if (ns > 0) hipLaunchKernelGGL(kb_advance, dim3(ns), dim3(64), 0, st, B.P, A2, B.scenes);
     if (B.arm == 1) launch_batch_momentum32(B, st);
     else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_momentum<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap));
     if (B.any_rigid) hipLaunchKernelGGL(kb_rigid_reduce, dim3(ns), dim3(kRigidBlock), 0, st, B.P, A2, B.scenes, 1);
     DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_integrate<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap, M_BC | M_VEL | M_MOVE | M_RHO));
-    hipLaunchKernelGGL(kb_finish, dim3(sb), dim3(kBlock), 0, st, B.P, A2, B.scenes, ns, 1, 1);
+    if (ns > 0) hipLaunchKernelGGL(kb_finish, dim3(ns), dim3(64), 0, st, B.P, A2, B.scenes, 1, 1);
 }
 
 // gathers the control blocks of a batch into one array for a single copy

@@ -50,6 +50,7 @@ void launch_shift(const Params &P, const Arrays &A, hipStream_t st);
 void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st);
 void launch_normals(const Params &P, const Arrays &A, hipStream_t st);
 void launch_advance(const Params &P, const Arrays &A, hipStream_t st);
+void launch_crit_fold(const Params &P, const Arrays &A, hipStream_t st);
 void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st);
 void launch_rigid_kin(const Params &P, const Arrays &A, hipStream_t st);
 void launch_rigid_reduce(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st);

@@ -27,6 +27,7 @@ constexpr int kBlock = 256;
 // so a particle's consecutive entries share one 32-byte sector (its writes
 // merge in L2) and a wave reads a chunk of 64 lists in whole lines.
 constexpr int kNbrChunk = 8;
+constexpr int kBboxSlots = 64, kBboxStride = 16, kCritSlot = 8;
 inline __host__ __device__ size_t nbr_slot(int k, int i, int ncap)
 {
     return ((size_t)(k / kNbrChunk) * (size_t)ncap + (size_t)i) * kNbrChunk + (size_t)(k % kNbrChunk);
@@ -58,7 +59,6 @@ struct Ctl {
     double cmm[2][3];
     double rbR[3], rbV[3], rbW[3], rbA[3], rbAL[3];
     unsigned long long critts_key;
-    unsigned long long bbox_key[2][3];
     int glim[3];
     int ncell;
     int ncode;            // size of the cell-code space (storage order of the cells)
@@ -132,6 +132,11 @@ struct Arrays {
     int *slab_cnt, *slab_off;
     int *scan_aux;
     int *rigid_idx;
+    // per-block reductions spread over kBboxSlots slots on separate cache lines
+    // (ordered keys): [0..2] min and [3..5] max of the moved positions (reduced
+    // and reset by the step's finish), [kCritSlot] the critical step (folded
+    // into Ctl::critts_key); a block updates slot (block % kBboxSlots)
+    unsigned long long *bbox;
     // cell code of each sorted slot (written while the particles are sorted)
     int *occ;
     Ctl *ctl;
