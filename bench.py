@@ -96,7 +96,69 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for HBM counters)")
     ap.add_argument("--arms", default="fp64,fp32c", help="comma-separated arms to run")
+    ap.add_argument("--serve-requests", type=int, default=32,
+                    help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate, 1/s")
+    ap.add_argument("--serve-steps-scale", type=float, default=0.5,
+                    help="scales the 500-5000 step counts of the serving mix")
+    ap.add_argument("--ensemble", type=int, default=512, help="scenes of the ensemble leg (0 skips it)")
+    ap.add_argument("--ensemble-steps", type=int, default=200,
+                    help="steps per ensemble scene (the ensemble workload runs 2000; a shorter "
+                         "sample measures the same rate)")
     return ap.parse_args()
+
+
+def serving_leg(args):
+    """Continuous batching against one-scene-at-a-time on the same seeded
+    Poisson stream (40% Poiseuille, 30% cavity, 20% droplet, 10% cylinder;
+    4k-500k particles), FP32-compensated arm."""
+    from sph07_amd import serving
+    from sph07_amd.native import ARM_FP32C
+    kw = dict(seed=0x5EED0004, n=args.serve_requests, rate=args.serve_rate,
+              steps_scale=args.serve_steps_scale)
+    out = {"requests": args.serve_requests, "rate_per_s": args.serve_rate,
+           "steps_scale": args.serve_steps_scale, "arm": "fp32c", "hbm_budget_gb": 200}
+    print(f"bench: serving leg, {args.serve_requests} requests", file=sys.stderr, flush=True)
+    reqs = serving.serving_stream(**kw)
+    out["particles"] = [r.deck.np for r in reqs]
+    out["steps"] = [r.steps for r in reqs]
+    b = serving.serve_batched(reqs, ARM_FP32C, hbm_budget=200e9, chunk=25).summary()
+    print(f"bench:   batched {b}", file=sys.stderr, flush=True)
+    reqs = serving.serving_stream(**kw)
+    o = serving.serve_one_at_a_time(reqs, ARM_FP32C).summary()
+    print(f"bench:   one at a time {o}", file=sys.stderr, flush=True)
+    out["batched"], out["one_at_a_time"] = b, o
+    out["throughput_ratio"] = b["particle_updates_per_s"] / o["particle_updates_per_s"]
+    del out["particles"], out["steps"]
+    out["particles_total"] = int(sum(r.deck.np for r in reqs))
+    return out
+
+
+def ensemble_leg(args):
+    """Independent 2D scenes (alternately Poiseuille and lid-driven cavity,
+    2k-64k particles, nu log-uniform 1e-6..1e-4) advanced together by the
+    batched step, FP32-compensated arm."""
+    from sph07_amd import serving
+    from sph07_amd.native import ARM_FP32C, Batch
+    reqs = serving.ensemble(n=args.ensemble, steps=args.ensemble_steps)
+    print(f"bench: ensemble leg, {len(reqs)} scenes x {args.ensemble_steps} steps",
+          file=sys.stderr, flush=True)
+    b = Batch(2, ARM_FP32C)
+    t0 = time.perf_counter()
+    for r in reqs:
+        b.add(r.deck, r.steps)
+    t_add = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    live = b.live
+    while live:
+        live = b.step(100)
+    el = time.perf_counter() - t0
+    errs = sum(1 for s in b.finished() if b.status(s)["error"])
+    b.close()
+    upd = sum(r.deck.np * r.steps for r in reqs)
+    return {"scenes": len(reqs), "steps": args.ensemble_steps, "particles_total": int(sum(r.deck.np for r in reqs)),
+            "seconds": round(el, 3), "init_seconds": round(t_add, 3),
+            "particle_updates_per_s": upd / el, "failed": errs}
 
 
 def main():
@@ -217,6 +279,9 @@ def main():
                          f"(after its untimed initialisation), single thread, {tc:.1f} s"}
         del o
 
+    serve = serving_leg(args) if args.serve_requests > 0 else None
+    ens = ensemble_leg(args) if args.ensemble > 0 else None
+
     out = {
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
@@ -235,6 +300,10 @@ def main():
         "algorithmic_gbs_per_kernel": hbm,
         "valu_roofline_per_kernel": valu,
         "cpu_baseline": cpu,
+        "serving": serve,
+        "p50_time_to_solution_s": serve["batched"]["p50_time_to_solution_s"] if serve else None,
+        "p99_time_to_solution_s": serve["batched"]["p99_time_to_solution_s"] if serve else None,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 
