@@ -202,6 +202,7 @@ int alloc_particles(Context *c, int gcap)
     rc |= dalloc(c, L, &A.nbr, (size_t)nbr_round(c->maxnbr) * ncap);
     rc |= dalloc(c, L, &A.cell_of, c->N);
     rc |= dalloc(c, L, &A.occ, (size_t)c->N + 1);
+    rc |= dalloc(c, L, &A.ovf, (size_t)c->N + 1);
     rc |= dalloc(c, L, &A.slot, c->N);
     rc |= dalloc(c, L, &A.tmp, c->N);
     rc |= dalloc(c, L, &A.gsrc, gcap);
@@ -217,7 +218,7 @@ int alloc_particles(Context *c, int gcap)
     B.rho_new = A.rho_new; B.p = A.p; B.prho2 = A.prho2; B.vol = A.vol;
     B.trd = A.trd; B.mindist = A.mindist; B.critts_p = A.critts_p; B.rhoold = A.rhoold;
     B.Nf = A.Nf; B.kappa = A.kappa; B.nnbr = A.nnbr; B.nbr = A.nbr;
-    B.cell_of = A.cell_of; B.slot = A.slot; B.tmp = A.tmp; B.occ = A.occ;
+    B.cell_of = A.cell_of; B.slot = A.slot; B.tmp = A.tmp; B.occ = A.occ; B.ovf = A.ovf;
     B.gsrc = A.gsrc; B.gshift = A.gshift;
     return rc;
 }

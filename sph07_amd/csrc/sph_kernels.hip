@@ -184,6 +184,7 @@ __device__ __forceinline__ void grid_setup_body(const Params &P, Ctl *ctl)
     }
     ctl->ncell = (int)ncell;
     ctl->ncode = (int)ncode;
+    ctl->novf = 0;
 
     ctl->nbucket = (int)ncode + 1;
     for (int d = 0; d < 3; d++) ctl->cmask[d] = msk[d];
@@ -229,33 +230,46 @@ __device__ __forceinline__ void bin_count_body(const Params &P, const Arrays &A,
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
     const int i = bid * kBlock + threadIdx.x;
-    if (i >= P.N) return;
-    uint32_t f = A.info[i];
-    const int inactive_bucket = ctl->ncode;
-    if (!(f & kActive)) {
-        A.cell_of[i] = inactive_bucket;
-        A.slot[i] = atomicAdd(&A.cell_count[inactive_bucket], 1);
-        return;
-    }
-    int bc[3] = {0, 0, 0};
+    const int lane = threadIdx.x & 63;
+    // bucket of this particle (-1: none), every lane of the wave takes part
+    int key = -1;
+    if (i < P.N) {
+        const uint32_t f = A.info[i];
+        const int inactive_bucket = ctl->ncode;
+        key = inactive_bucket;
+        if (f & kActive) {
+            int bc[3] = {0, 0, 0};
 #pragma unroll
-    for (int d = 0; d < D; d++) bc[d] = (int)(ctl->gsize[d] * (A.x[d][i] - ctl->gmin[d]));
-    if (ctl->timestep == 1 && (bc[0] < 0 || bc[0] > ctl->glim[0] - 1)) {
-        A.info[i] = f & ~kActive;          // outside the (single) process slab
-        A.cell_of[i] = inactive_bucket;
-        A.slot[i] = atomicAdd(&A.cell_count[inactive_bucket], 1);
-        return;
-    }
+            for (int d = 0; d < D; d++) bc[d] = (int)(ctl->gsize[d] * (A.x[d][i] - ctl->gmin[d]));
+            if (ctl->timestep == 1 && (bc[0] < 0 || bc[0] > ctl->glim[0] - 1)) {
+                A.info[i] = f & ~kActive;          // outside the (single) process slab
+            } else {
+                bool out = false;
 #pragma unroll
-    for (int d = 0; d < 3; d++)
-        if (bc[d] < 0 || bc[d] >= ctl->glim[d]) {
-            set_err(ctl, kErrOutOfGrid, A.orig[i]);
-            A.cell_of[i] = -1;
-            return;
+                for (int d = 0; d < 3; d++) out |= bc[d] < 0 || bc[d] >= ctl->glim[d];
+                if (out) {
+                    set_err(ctl, kErrOutOfGrid, A.orig[i]);
+                    key = -1;
+                } else {
+                    key = cell_code(ctl, bc[0], bc[1], bc[2]);
+                }
+            }
         }
-    const int c = cell_code(ctl, bc[0], bc[1], bc[2]);
-    A.cell_of[i] = c;
-    A.slot[i] = atomicAdd(&A.cell_count[c], 1);
+        A.cell_of[i] = key;
+    }
+    // The storage is sorted by last step's cells, so equal keys come in runs
+    // of neighbouring lanes: one atomic per run, slots handed out in the run.
+    const int prev = __shfl(key, lane > 0 ? lane - 1 : 0, 64);
+    const bool head = lane == 0 || prev != key;
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const int start = 63 - __clzll((long long)(hm & upto));
+    const unsigned long long after = hm & ~upto;
+    const int end = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (head && key >= 0) base = atomicAdd(&A.cell_count[key], end - start);
+    base = __shfl(base, start, 64);
+    if (i < P.N && key >= 0) A.slot[i] = base + (lane - start);
 }
 
 __device__ __forceinline__ void bin_scatter_body(const Params &P, const Arrays &A, const int bid)
@@ -773,8 +787,8 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         }
         const int c0 = head, c1 = jn;
         if (ncand > kCandCap) {
-            for (int tg = c0; tg < c1; tg += 64)
-                if (tg + lane < c1) neighbours_one<D>(P, A, shift, tg + lane);
+            // left to k_nbr_ovf (per-particle walk)
+            if (lane == 0) A.ovf[atomicAdd(&ctl->novf, 1)] = q;
             continue;
         }
         // staged slots are read back by other lanes of the same wave (LDS
@@ -1564,13 +1578,26 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Para
     if (bid_ >= nb_) return;
 template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { XCD_BID(P); neighbours_body<D>(P, A, shift, bid_); }
 constexpr int kCellBlocks = 2048;       // fixed grid of the cell kernel (8192 waves)
-template <int D> __global__ __launch_bounds__(kBlock) void k_nbr_cells(Params P, Arrays A, int shift)
+template <int D, bool ONEMAT> __global__ __launch_bounds__(kBlock) void k_nbr_cells(Params P, Arrays A, int shift)
 {
     __shared__ int stage[kCellWaves][kCandCap];
     const int b = xcd_block(kCellBlocks);
     const int wv = (int)(threadIdx.x >> 6);
-    if (P.nmat == 1) nbr_cells_body<D, true>(P, A, shift, b * kCellWaves + wv, kCellBlocks * kCellWaves, stage[wv]);
-    else nbr_cells_body<D, false>(P, A, shift, b * kCellWaves + wv, kCellBlocks * kCellWaves, stage[wv]);
+    nbr_cells_body<D, ONEMAT>(P, A, shift, b * kCellWaves + wv, kCellBlocks * kCellWaves, stage[wv]);
+}
+// cells whose stencil did not fit the staging space: one block per cell,
+// the per-particle walk for each of its particles
+constexpr int kOvfBlocks = 256;
+template <int D> __global__ __launch_bounds__(kBlock) void k_nbr_ovf(Params P, Arrays A, int shift)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int n = ctl->novf;
+    for (int w = blockIdx.x; w < n; w += gridDim.x) {
+        const int q = A.ovf[w];
+        const int c0 = A.cell_start[q], c1 = A.cell_start[q + 1];
+        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(P, A, shift, i);
+    }
 }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *Pt, const Arrays *At, const int2 *map, int shift)
 {
@@ -1708,7 +1735,12 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st)
 {
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_nbr_cells<DD>, dim3(kCellBlocks), dim3(kBlock), 0, st, P, A, shift));
+    if (P.nmat == 1) {
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true>), dim3(kCellBlocks), dim3(kBlock), 0, st, P, A, shift));
+    } else {
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false>), dim3(kCellBlocks), dim3(kBlock), 0, st, P, A, shift));
+    }
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_nbr_ovf<DD>, dim3(kOvfBlocks), dim3(kBlock), 0, st, P, A, shift));
 }
 
 void launch_shift(const Params &P, const Arrays &A, hipStream_t st)

@@ -65,6 +65,7 @@ struct Ctl {
     int nbucket;          // ncode + 1: the last bucket holds inactive particles
     unsigned int cmask[3];// code bits owned by each axis (bit-interleaved cell code)
     int timestep, done, err, ngp;
+    int novf;             // cells whose stencil exceeds the cell kernel's staging space
     int err_particle;
     int step_limit;       // > 0: the scene stops after this many steps (batched runs)
 };
@@ -139,6 +140,8 @@ struct Arrays {
     unsigned long long *bbox;
     // cell code of each sorted slot (written while the particles are sorted)
     int *occ;
+    // cells left to the per-particle neighbour walk (oversized stencils)
+    int *ovf;
     Ctl *ctl;
 };
 
