@@ -142,6 +142,10 @@ int alloc_particles(Context *c, int gcap)
     Params &P = c->P;
     Arrays &A = c->AA[0], &B = c->AA[1];
     const size_t ncap = (size_t)c->N + gcap;
+    if (ncap * kNbrChunk > 0xffffffffull) {     // neighbour-list chunk stride is 32-bit
+        c->err = "too many particles and images for one context";
+        return 1;
+    }
     std::vector<void *> &L = c->part_allocs;
     P.gcap = gcap;
     P.Ncap = (int)ncap;

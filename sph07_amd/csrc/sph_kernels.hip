@@ -741,7 +741,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
     const int ncode = ctl->ncode;
     const unsigned int m0 = ctl->cmask[0], m1 = ctl->cmask[1], m2 = ctl->cmask[2];
     const int gl1 = ctl->glim[1], gl2 = ctl->glim[2];
-    const size_t cstride = (size_t)P.Ncap * kNbrChunk;     // distance between list chunks
+    const unsigned int cstride32 = (unsigned int)P.Ncap * kNbrChunk;   // distance between list chunks
     const unsigned int maxnbr = (unsigned int)P.maxnbr;
     int jn = j0;
     while (jn < j1) {
@@ -759,7 +759,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         }
         // stage the candidate slots in traversal order; the code bits of the
         // stencil coordinates advance by masked increments
-        int ncand = 0;
+        int ncand = 0, own = 0;      // own: staged position of the cell's first particle
         const int qs_lo = code_dep(lo[1], m1), qt_lo = code_dep(lo[2], m2);
         int qr = code_dep(lo[0], m0);
         for (int r = lo[0]; r <= hi[0]; r++, qr = (int)((((unsigned)qr | ~m0) + 1u) & m0)) {
@@ -778,6 +778,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
                         }
                     }
                     const int qc = qr | qs | qt;
+                    if (qc == q) own = ncand;
                     const int s0 = A.cell_start[qc], len = A.cell_start[qc + 1] - s0;
                     for (int u = lane; u < len; u += 64)
                         if (ncand + u < kCandCap) stage[ncand + u] = s0 + u;
@@ -795,7 +796,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         // operations of one wave complete in order)
         __builtin_amdgcn_wave_barrier();
         for (int tg = c0; tg < c1; tg += 64) {
-            const int nt = c1 - tg < 64 ? c1 - tg : 64;
+            const int nt = __builtin_amdgcn_readfirstlane(c1 - tg < 64 ? c1 - tg : 64);
             const bool tv = lane < nt;
             const int it = tv ? tg + lane : tg;         // lanes past the cell read a valid slot
             double xi[D];
@@ -815,7 +816,11 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
 #pragma unroll
                 for (int d = 0; d < D; d++) xc[d] = A.x[d][cand];
                 const int matc = ONEMAT ? 0 : info_mat(A.info[cand]);
-                const int self = cv ? cand - tg : -1;      // equals t for the target itself
+                // lanes holding a candidate, and the lane of target t itself
+                // (staged at own + (tg + t - c0)): wave-uniform masks
+                const int nv = ncand - k0;
+                const unsigned long long valid = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
+                const int self0 = own + (tg - c0) - k0;   // lane of target 0 in this chunk
                 for (int t = 0; t < nt; t++) {
                     double d2 = 0.0;
 #pragma unroll
@@ -828,15 +833,18 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
                         const int matt = info_mat((uint32_t)__builtin_amdgcn_readlane((int)fi, t));
                         cut2 = P.pc[matt][matc].cut2;
                     }
-                    const bool pass = cv && self != t && d2 < cut2;
-                    const unsigned long long pm = __builtin_amdgcn_ballot_w64(pass);
+                    const int sl = self0 + t;
+                    const unsigned long long selfm = (sl >= 0 && sl < 64) ? (1ull << sl) : 0ull;
+                    const unsigned long long pm = __builtin_amdgcn_ballot_w64(d2 < cut2) & valid & ~selfm;
                     if (pm) {
                         const unsigned int nnt = (unsigned int)__builtin_amdgcn_readlane(nn, t);
                         const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
                                                                        __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
                         int *lst = A.nbr + (size_t)(tg + t) * kNbrChunk;
-                        if (pass && pos < maxnbr) lst[(size_t)(pos >> 3) * cstride + (pos & 7u)] = cand;
-                        nn += lane == t ? (int)__popcll(pm) : 0;
+                        const bool pass = (pm >> lane) & 1ull;
+                        if (pass && pos < maxnbr) lst[(size_t)(pos >> 3) * cstride32 + (pos & 7u)] = cand;
+                        const int nnew = (int)(nnt + (unsigned)__popcll(pm));
+                        nn = lane == t ? nnew : nn;
                         if (P.want_mindist) {
                             double r1 = pass ? sqrt(d2) : 1.0e20;
 #pragma unroll

@@ -134,6 +134,26 @@ def test_fused_steps(name, steps):
             assert abs(gg[k] - go[k]) <= 1e-9 * sc, (k, gg[k], go[k])
 
 
+@pytest.mark.parametrize("hfac,steps", [(4.5, 30), (6.0, 20)])
+def test_dense_cells(hfac, steps):
+    """Large h/dx: 2h cells of 81 (h = 4.5 dx) and 144 (h = 6 dx) particles.
+    The first has more targets per cell than a wave (two target groups per
+    cell); the second has stencils of ~1300 candidates, beyond the cell
+    kernel's staging space, so those cells take the per-particle walk.  Both
+    stay bit-identical to the oracle, lists included."""
+    deck = scenes.cavity(n=24, hfac=hfac, maxnbr=700)
+    o = orc.Oracle(deck)
+    g = gpu_scene(deck)
+    nn_o, ids_o = o.neighbours()
+    nn_g, ids_g = g.neighbours()
+    assert np.array_equal(nn_o, nn_g)
+    assert nn_g.max() > 64
+    for i in range(len(nn_o)):
+        assert np.array_equal(ids_o[i, :nn_o[i]], ids_g[i, :nn_g[i]]), i
+    assert o.run(steps) == g.Solution(steps) == steps
+    compare_state(g, o, RTOL, exact=True, what=f"dense cells h={hfac}dx after {steps} steps")
+
+
 def test_neighbour_overflow_is_reported():
     """maxnbr too small: the reference's 'Neighbour list exceeded maximum.'"""
     from sph07_amd.native import SphError
