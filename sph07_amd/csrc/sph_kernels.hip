@@ -11,6 +11,8 @@
 #include "sph_kernels.hpp"
 #include "sph_rec32.hpp"
 
+#include <stdlib.h>
+
 #include <math.h>
 
 namespace sph {
@@ -757,41 +759,48 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             lo[d] = bc[d] - 1 > 0 ? bc[d] - 1 : 0;
             hi[d] = bc[d] + 1 < ctl->glim[d] - 1 ? bc[d] + 1 : ctl->glim[d] - 1;
         }
-        // stage the candidate slots in traversal order; the code bits of the
-        // stencil coordinates advance by masked increments
-        int ncand = 0, own = 0;      // own: staged position of the cell's first particle
-        const int qs_lo = code_dep(lo[1], m1), qt_lo = code_dep(lo[2], m2);
-        int qr = code_dep(lo[0], m0);
-        for (int r = lo[0]; r <= hi[0]; r++, qr = (int)((((unsigned)qr | ~m0) + 1u) & m0)) {
-            int qs = qs_lo;
-            for (int s = lo[1]; s <= hi[1]; s++, qs = (int)((((unsigned)qs | ~m1) + 1u) & m1)) {
-                int qt = qt_lo;
-                for (int t = lo[2]; t <= hi[2]; t++, qt = (int)((((unsigned)qt | ~m2) + 1u) & m2)) {
-                    if (P.periodic) {
-                        const int c = gl2 * (gl1 * r + s) + t;
-                        const int ng = A.gcount[c];
-                        if (ng) {
-                            const int g0 = P.N + A.gfirst[c] + ng - 1;     // newest image first
-                            for (int u = lane; u < ng; u += 64)
-                                if (ncand + u < kCandCap) stage[ncand + u] = g0 - u;
-                            ncand += ng;
-                        }
-                    }
-                    const int qc = qr | qs | qt;
-                    if (qc == q) own = ncand;
-                    const int s0 = A.cell_start[qc], len = A.cell_start[qc + 1] - s0;
-                    for (int u = lane; u < len; u += 64)
-                        if (ncand + u < kCandCap) stage[ncand + u] = s0 + u;
-                    ncand += len;
-                }
+        // Stage the candidate slots in traversal order.  One lane per stencil
+        // cell (x -> y -> z, z fastest) loads its images and slot range in a
+        // single round of loads; a wave scan gives each cell's offset in the
+        // staged list (images first, newest first, then the slots).
+        int sr = 0, ss = 0, sz = 0;
+        bool inst;
+        if (D == 3) {
+            inst = lane < 27;
+            sr = lane / 9 - 1; ss = (lane / 3) % 3 - 1; sz = lane % 3 - 1;
+        } else if (D == 2) {
+            inst = lane < 9;
+            sr = lane / 3 - 1; ss = lane % 3 - 1;
+        } else {
+            inst = lane < 3;
+            sr = lane - 1;
+        }
+        const int cr = bc[0] + sr, cs = bc[1] + ss, cz = bc[2] + sz;
+        inst = inst && cr >= 0 && cr < ctl->glim[0] && cs >= 0 && cs < gl1 && cz >= 0 && cz < gl2;
+        int ng = 0, g0 = 0, s0 = 0, len = 0;
+        if (inst) {
+            const int qc = code_dep(cr, m0) | code_dep(cs, m1) | code_dep(cz, m2);
+            s0 = A.cell_start[qc];
+            len = A.cell_start[qc + 1] - s0;
+            if (P.periodic) {
+                const int c = gl2 * (gl1 * cr + cs) + cz;
+                ng = A.gcount[c];
+                if (ng) g0 = A.gfirst[c];
             }
         }
+        const int cnt = ng + len;
+        const int incl = wave_incl_scan(cnt);
+        const int off = incl - cnt;
+        const int ncand = __builtin_amdgcn_readlane(incl, 63);
+        const int own = __builtin_amdgcn_readlane(off + ng, D == 3 ? 13 : (D == 2 ? 4 : 1));
         const int c0 = head, c1 = jn;
         if (ncand > kCandCap) {
             // left to k_nbr_ovf (per-particle walk)
             if (lane == 0) A.ovf[atomicAdd(&ctl->novf, 1)] = q;
             continue;
         }
+        for (int u = 0; u < ng; u++) stage[off + u] = P.N + g0 + ng - 1 - u;
+        for (int u = 0; u < len; u++) stage[off + ng + u] = s0 + u;
         // staged slots are read back by other lanes of the same wave (LDS
         // operations of one wave complete in order)
         __builtin_amdgcn_wave_barrier();
@@ -1585,13 +1594,31 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Para
     const int bid_ = xcd_block(nb_);                     \
     if (bid_ >= nb_) return;
 template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { XCD_BID(P); neighbours_body<D>(P, A, shift, bid_); }
-constexpr int kCellBlocks = 2048;       // fixed grid of the cell kernel (8192 waves)
-template <int D, bool ONEMAT> __global__ __launch_bounds__(kBlock) void k_nbr_cells(Params P, Arrays A, int shift)
+// The grid is a multiple of 8 blocks; each wave takes an equal share of the
+// slots.  At most 77 VGPRs (6 waves per SIMD) without spills.
+template <int D, bool ONEMAT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_nbr_cells(Params P, Arrays A, int shift)
 {
     __shared__ int stage[kCellWaves][kCandCap];
-    const int b = xcd_block(kCellBlocks);
+    const int nb = (int)gridDim.x;
+    const int b = xcd_block(nb);
     const int wv = (int)(threadIdx.x >> 6);
-    nbr_cells_body<D, ONEMAT>(P, A, shift, b * kCellWaves + wv, kCellBlocks * kCellWaves, stage[wv]);
+    nbr_cells_body<D, ONEMAT>(P, A, shift, b * kCellWaves + wv, nb * kCellWaves, stage[wv]);
+}
+
+// grid of the cell kernel: about kCellSlotsPerWave slots per wave (a few
+// cells), so that waves of uneven cost balance over several rounds
+constexpr int kCellSlotsPerWave = 32;
+static int cell_blocks(int n)
+{
+    static int forced = -1;
+    if (forced < 0) {
+        const char *e = getenv("SPH07_CELL_BLOCKS");
+        forced = e ? atoi(e) : 0;
+    }
+    long long nb = forced > 0 ? forced : ((long long)n + kCellSlotsPerWave * kCellWaves - 1) / (kCellSlotsPerWave * kCellWaves);
+    nb = nb < 8 ? 8 : (nb > (1 << 20) ? (1 << 20) : nb);
+    return xcd_grid((int)nb);
 }
 // cells whose stencil did not fit the staging space: one block per cell,
 // the per-particle walk for each of its particles
@@ -1744,9 +1771,9 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st)
 {
     if (P.nmat == 1) {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true>), dim3(kCellBlocks), dim3(kBlock), 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true>), dim3(cell_blocks(P.N)), dim3(kBlock), 0, st, P, A, shift));
     } else {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false>), dim3(kCellBlocks), dim3(kBlock), 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false>), dim3(cell_blocks(P.N)), dim3(kBlock), 0, st, P, A, shift));
     }
     DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_nbr_ovf<DD>, dim3(kOvfBlocks), dim3(kBlock), 0, st, P, A, shift));
 }
