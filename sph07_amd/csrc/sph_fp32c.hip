@@ -21,6 +21,16 @@ namespace sph {
 
 #define CTL_STOP32(ctl) ((ctl)->err != 0 || (ctl)->done != 0)
 
+// neighbour records in flight per lane (more hides more gather latency, at
+// the cost of registers and occupancy; measured on the 3D dam break)
+#ifndef SPH_STRAIN32_BATCH
+#define SPH_STRAIN32_BATCH 8
+#endif
+#ifndef SPH_MOMENTUM32_BATCH
+#define SPH_MOMENTUM32_BATCH 4
+#endif
+constexpr int kStrain32Batch = SPH_STRAIN32_BATCH, kMomentum32Batch = SPH_MOMENTUM32_BATCH;
+
 // TwoSum: s + e == a + b exactly (Knuth, TAOCP vol. 2, 4.2.2)
 __device__ __forceinline__ void two_sum_acc(float &s, float &c, float x)
 {
@@ -135,8 +145,10 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
                 gc[d] = 0.0f;
             }
             const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
-            for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
-                const float4 ra = A.fsA[j], rb = A.fsB[j];
+            struct Rec { float4 a, b; };
+            auto load = [&](int j) { return Rec{A.fsA[j], A.fsB[j]}; };
+            for_each_nbr_batched<kStrain32Batch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+                const float4 ra = rec.a, rb = rec.b;
                 const uint32_t pk_j = __float_as_uint(rb.w);
                 const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][info_mat(pk_j)]);
                 const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
@@ -220,9 +232,11 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             ac[d] = 0.0f;
         }
         const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
-        for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
-            const float4 ra = A.fmA[j], rb = A.fmB[j];
-            const float2 rc = A.fmC[j];
+        struct Rec { float4 a, b; float2 c; };
+        auto load = [&](int j) { return Rec{A.fmA[j], A.fmB[j], A.fmC[j]}; };
+        for_each_nbr_batched<kMomentum32Batch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+            const float4 ra = rec.a, rb = rec.b;
+            const float2 rc = rec.c;
             const uint32_t fj = __float_as_uint(rc.y);
             const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
             const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][mat_j]);

@@ -13,9 +13,19 @@
 
 #include <stdlib.h>
 
+// neighbour records in flight per lane in the FP64 sweeps
+#ifndef SPH_STRAIN_BATCH
+#define SPH_STRAIN_BATCH 4
+#endif
+#ifndef SPH_MOMENTUM_BATCH
+#define SPH_MOMENTUM_BATCH 2
+#endif
+
 #include <math.h>
 
 namespace sph {
+
+constexpr int kStrainBatch = SPH_STRAIN_BATCH, kMomentumBatch = SPH_MOMENTUM_BATCH;
 
 #define CTL_STOP(ctl) ((ctl)->err != 0 || (ctl)->done != 0)
 
@@ -987,8 +997,10 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
                 vi[d] = A.v[d][i];
                 g[d] = 0.0;
             }
-            for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
-                const double4 ra = A.rsA[j], rb = A.rsB[j];
+            struct Rec { double4 a, b; };
+            auto load = [&](int j) { return Rec{A.rsA[j], A.rsB[j]}; };
+            for_each_nbr_batched<kStrainBatch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+                const double4 ra = rec.a, rb = rec.b;
                 const double volj = ra.w;
                 const PairConst &pc = P.pc[mat_i][P.nmat == 1 ? mat_i : info_mat((uint32_t)rb.w)];
                 double xj[D], vj[D], dw[D];
@@ -1176,9 +1188,11 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
             xi[d] = A.x[d][i];
             vi[d] = A.v[d][i];
         }
-        for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
-            const double4 ra = A.rmA[j], rb = A.rmB[j];
-            const double2 rc = A.rmC[j];
+        struct Rec { double4 a, b; double2 c; };
+        auto load = [&](int j) { return Rec{A.rmA[j], A.rmB[j], A.rmC[j]}; };
+        for_each_nbr_batched<kMomentumBatch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+            const double4 ra = rec.a, rb = rec.b;
+            const double2 rc = rec.c;
             const uint32_t fj = (uint32_t)rc.y;
             const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
             const PairConst &pc = P.pc[mat_i][mat_j];

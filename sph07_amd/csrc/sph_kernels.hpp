@@ -35,6 +35,32 @@ __device__ __forceinline__ void for_each_nbr(const int *nbr, int i, int nn, int 
             if (u < m) f(e[u]);
     }
 }
+
+// As for_each_nbr, with the record loads of G consecutive entries issued
+// before any is used: load(j) returns a neighbour's record, use(rec) consumes
+// it; entries past the list end load the particle's own record (a valid
+// address) and are not used.  The use order is the list order.
+template <int G, class Load, class Use>
+__device__ __forceinline__ void for_each_nbr_batched(const int *nbr, int i, int nn, int ncap, Load &&load, Use &&use)
+{
+    static_assert(kNbrChunk % G == 0, "batch divides the chunk");
+    for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
+        const int4 *p = reinterpret_cast<const int4 *>(nbr + nbr_slot(k0, i, ncap));
+        const int4 c0 = p[0], c1 = p[1];
+        const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        const int m = nn - k0;
+#pragma unroll
+        for (int g0 = 0; g0 < kNbrChunk; g0 += G) {
+            if (g0 >= m) break;
+            decltype(load(0)) rec[G];
+#pragma unroll
+            for (int u = 0; u < G; u++) rec[u] = load(g0 + u < m ? e[g0 + u] : i);
+#pragma unroll
+            for (int u = 0; u < G; u++)
+                if (g0 + u < m) use(rec[u]);
+        }
+    }
+}
 #endif
 
 // masks of launch_strain

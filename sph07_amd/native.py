@@ -16,7 +16,8 @@ import numpy as np
 from .deck import CDeck, CMaterial, Deck
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsph07_amd.so")
+# SPH07_LIB overrides the library path (alternative builds of the same sources)
+LIB_PATH = os.environ.get("SPH07_LIB") or os.path.join(HERE, "libsph07_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sph07_amd.h")
 
 ARM_FP64 = 0
