@@ -150,12 +150,14 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
             for_each_nbr_batched<kStrain32Batch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
                 const float4 ra = rec.a, rb = rec.b;
                 const uint32_t pk_j = __float_as_uint(rb.w);
-                const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][info_mat(pk_j)]);
+                const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)]);
                 const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
+                float off[3];
+                cell_offs(pk_i, pk_j, off);
                 float dx[D], r2 = 0.0f;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
-                    dx[d] = cell_off(pk_i, pk_j, d) * cs[d] + (ri[d] - rj[d]);
+                    dx[d] = off[d] * cs[d] + (ri[d] - rj[d]);
                     r2 += dx[d] * dx[d];
                 }
                 const float f = gradw_factor(r2, q) * ra.w;
@@ -238,15 +240,17 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             const float4 ra = rec.a, rb = rec.b;
             const float2 rc = rec.c;
             const uint32_t fj = __float_as_uint(rc.y);
-            const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
+            const int mat_j = P.nmat == 1 ? mat_i : rec_mat(fj);
             const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][mat_j]);
             const float mj = ra.w;
             const float rho_j = rb.w;
             const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
+            float off[3];
+            cell_offs(pk_i, fj, off);
             float dx[D], dv[D], r2 = 0.0f, vr = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                dx[d] = cell_off(pk_i, fj, d) * cs[d] + (ri[d] - rj[d]);
+                dx[d] = off[d] * cs[d] + (ri[d] - rj[d]);
                 dv[d] = vi[d] - vj[d];
                 r2 += dx[d] * dx[d];
                 vr += dv[d] * dx[d];
@@ -254,7 +258,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             const float f = gradw_factor(r2, q);            // grad W = f * dx
             float sdw = -mj * (rc.x + pr_i);                // coefficient of grad W
             float tv = 0.0f;                                // coefficient of (v_i - v_j)
-            if (!(rig_i && (fj & kRigid))) {
+            if (!(rig_i && (fj & kRecRigid))) {
                 const float qj = (float)P.mat[mat_j].nu * rho_j;
                 const float inv = __builtin_amdgcn_rcpf(rho_i * rho_j * (r2 + q.eps2));
                 tv = mj * (q_i + qj) * (f * r2) * inv;

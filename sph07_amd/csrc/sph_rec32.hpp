@@ -7,13 +7,18 @@
 // taken from the low two bits of each cell coordinate (kept in the packed
 // word).  Relative positions are below one cell size, so FP32 keeps them to
 // ~1e-7 of the cut-off.
+//
+// Packed word: bits 0-3 material, bit 7 rigid, bytes 1-3 the low two bits of
+// the cell coordinates (one byte per axis, so that the three offsets come out
+// of one byte-lane subtraction).
 #pragma once
 
 #include "sph_state.hpp"
 
 namespace sph {
 
-constexpr int kCellBitsShift = 16;     // packed word: info (material, rigid) | cell bits
+constexpr uint32_t kRecMat = 0x0Fu, kRecRigid = 0x80u;
+static_assert(kMaxMat < 16, "material fits the packed word");
 
 __device__ __forceinline__ float cell_size32(const Ctl *ctl, int d) { return (float)(1.0 / ctl->gsize[d]); }
 
@@ -24,16 +29,20 @@ __device__ __forceinline__ double cell_origin(const Ctl *ctl, int d, int c)
 
 __device__ __forceinline__ uint32_t pack32(uint32_t info, const int *bc)
 {
-    return (info & (kMatMask | kRigid)) | ((uint32_t)(bc[0] & 3) << kCellBitsShift) |
-           ((uint32_t)(bc[1] & 3) << (kCellBitsShift + 2)) | ((uint32_t)(bc[2] & 3) << (kCellBitsShift + 4));
+    return (info & kRecMat) | ((info & kRigid) ? kRecRigid : 0u) | ((uint32_t)(bc[0] & 3) << 8) |
+           ((uint32_t)(bc[1] & 3) << 16) | ((uint32_t)(bc[2] & 3) << 24);
 }
 
-// cell offset (-1, 0, 1) along axis d of particle a relative to particle b
-__device__ __forceinline__ float cell_off(uint32_t pa, uint32_t pb, int d)
+__device__ __forceinline__ int rec_mat(uint32_t pk) { return (int)(pk & kRecMat); }
+
+// Cell offsets (-1, 0, 1) of particle a relative to particle b on the three
+// axes: per byte (4 + ca - cb + 1) mod 4 = offset + 1, no borrow across bytes.
+__device__ __forceinline__ void cell_offs(uint32_t pa, uint32_t pb, float off[3])
 {
-    const int s = kCellBitsShift + 2 * d;
-    const int o = (((int)((pa >> s) & 3u) - (int)((pb >> s) & 3u) + 1) & 3) - 1;
-    return (float)o;
+    const uint32_t x = (((pa | 0x04040400u) - (pb & 0x03030300u)) + 0x01010100u) & 0x03030300u;
+    off[0] = (float)((x >> 8) & 0xFFu) - 1.0f;
+    off[1] = (float)((x >> 16) & 0xFFu) - 1.0f;
+    off[2] = (float)(x >> 24) - 1.0f;
 }
 
 // inverse of the cell-code deposit: the cell coordinate held in the bits m of v
