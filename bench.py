@@ -201,6 +201,7 @@ def main():
         tm = sc.timing()
         sc.set_timing(False)
         kern = {k: (ms / cnt if cnt else 0.0, cnt) for k, (ms, cnt) in tm.items()}
+        sc.launch_shape = sc.autotune(-1)
         return sc, elapsed, t_init, kern
 
     arms = {}
@@ -296,6 +297,7 @@ def main():
         "neighbour_interactions_per_s": pairs * args.steps / elapsed,
         "init_s": round(t_init, 3),
         "kernels_avg_ms": kernels,
+        "launch_shapes": {a: arms[a]["scene"].launch_shape for a in arms},
         "roofline": roof,
         "algorithmic_gbs_per_kernel": hbm,
         "valu_roofline_per_kernel": valu,

@@ -189,6 +189,16 @@ const char *sph_deck_title(void *deck);
 int  sph_deck_status_interval(void *deck);
 void sph_deck_free(void *deck);
 
+/* --- autotuner (no reference counterpart) ---
+   Launch shapes that change timing but never results: shape[0] = slots per
+   wave of the cell-centric neighbour search, shape[1] / shape[2] = neighbour
+   records in flight per lane in the FP32c continuity / momentum sweeps.
+   reps > 0 times every alternative reps times on the scene's current state
+   and keeps the fastest (scenes of 65536 particles or more are tuned this way
+   on their first sph_solution call); reps == 0 keeps the current shapes and
+   turns the automatic tuning off; reps < 0 only reports.  shape may be NULL. */
+int  sph_autotune(void *ctx, int reps, int32_t *shape);
+
 /* --- measurement (no reference counterpart: the reference only prints total
    wall seconds, MSimulationData.cpp:110-122) --- */
 /* Turns HIP-event timing of the kernel groups of sph_solution on or off and

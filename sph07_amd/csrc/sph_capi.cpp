@@ -57,7 +57,13 @@ struct Context {
     hipGraphExec_t gexec = nullptr;
     int gsteps = 0;
     bool use_graph = true;
+    // launch shapes (timing only, never results) and whether they were tuned
+    Tune tune;
+    bool tuned = false;
 };
+
+// scenes at least this large are tuned on their first sph_solution call
+constexpr int kAutotuneMinN = 65536;
 
 inline Arrays &cur(Context *c) { return c->AA[c->cur]; }
 
@@ -400,16 +406,16 @@ void fused_step(Context *c)
     const Arrays &A = cur(c);
     hipStream_t st = c->st;
     if (P.periodic) timed(c, K_IMAGES, [&] { launch_ghost_setup(P, A, st); });
-    timed(c, K_NBR, [&] { launch_neighbours(P, A, 1, st); });
+    timed(c, K_NBR, [&] { launch_neighbours(P, A, 1, st, c->tune.cell_slots); });
     if (c->arm == SPH_ARM_FP32C)
-        timed(c, K_STRAIN, [&] { launch_strain32(P, A, st); });
+        timed(c, K_STRAIN, [&] { launch_strain32(P, A, st, c->tune.strain32_batch); });
     else
         timed(c, K_STRAIN, [&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st); });
     if (P.st) timed(c, K_NORMALS, [&] { launch_normals(P, A, st); });
     timed(c, K_ADVANCE, [&] { launch_advance(P, A, st); });
     timed(c, K_MOMENTUM, [&] {
         if (c->arm == SPH_ARM_FP32C) {
-            launch_momentum32(P, A, st);
+            launch_momentum32(P, A, st, c->tune.momentum32_batch);
             launch_rigid_reduce(P, A, 1, st);
         } else {
             launch_momentum(P, A, 1, st);
@@ -905,10 +911,103 @@ int build_graph(Context *c)
     return 0;
 }
 
+// Autotuner: times the alternatives of each launch shape on the scene's
+// current state and keeps the fastest.  It begins a step (sort and images,
+// repairing capacities as the step loop does) and re-runs kernels that are
+// idempotent on one step's inputs: the neighbour search (every slot
+// count), the continuity sweep (whose rewrite of the positions is undone
+// from a saved copy) and the momentum sweep.  The following step re-sorts
+// already sorted particles, so results are those of an untuned run.
+int autotune(Context *c, int reps)
+{
+    const Params &P = c->P;
+    hipStream_t st = c->st;
+    for (int guard = 0;; guard++) {
+        if (sync_ctl(c)) return SPH_ERR_HIP;
+        if (c->hctl->done || c->hctl->err) break;
+        sort_step(c);
+        if (P.periodic) launch_ghost_setup(P, cur(c), st);
+        const int rc = check_device(c);
+        if (rc > 0) return rc;
+        if (rc == 0) break;
+        if (guard > 16) {
+            c->err = "ERROR: capacity repair did not converge.";
+            return SPH_ERROR;
+        }
+    }
+    if (c->hctl->done || c->hctl->err) {
+        c->tuned = true;
+        return SPH_OK;
+    }
+    const Arrays &A = cur(c);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    auto time_ms = [&](auto &&launch) -> float {
+        float ms = 1e30f;
+        if (hipEventRecord(e0, st) != hipSuccess) return ms;
+        for (int r = 0; r < reps; r++) launch();
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return ms;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
+        return ms;
+    };
+    Tune best = c->tune;
+    float tbest = 1e30f;
+    for (int slots : {16, 32, 64, 128}) {
+        const float t = time_ms([&] {
+            hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
+            launch_neighbours(P, A, 1, st, slots);
+        });
+        if (t < tbest) { tbest = t; best.cell_slots = slots; }
+    }
+    // the chosen shape runs last, so the lists are from it (all shapes agree)
+    hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
+    launch_neighbours(P, A, 1, st, best.cell_slots);
+    int rc = SPH_OK;
+    if (c->arm == SPH_ARM_FP32C) {
+        const size_t n = (size_t)c->N;
+        double *save = nullptr;
+        if (hipMalloc((void **)&save, n * sizeof(double) * c->ndim) != hipSuccess) rc = SPH_ERR_HIP;
+        if (rc == SPH_OK) {
+            for (int d = 0; d < c->ndim; d++)
+                hipMemcpyAsync(save + d * n, A.x[d], n * sizeof(double), hipMemcpyDeviceToDevice, st);
+            tbest = 1e30f;
+            for (int g : {4, 8}) {
+                const float t = time_ms([&] { launch_strain32(P, A, st, g); });
+                if (t < tbest) { tbest = t; best.strain32_batch = g; }
+            }
+            tbest = 1e30f;
+            for (int g : {2, 4, 8}) {
+                const float t = time_ms([&] { launch_momentum32(P, A, st, g); });
+                if (t < tbest) { tbest = t; best.momentum32_batch = g; }
+            }
+            for (int d = 0; d < c->ndim; d++)
+                hipMemcpyAsync(A.x[d], save + d * n, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+            hipStreamSynchronize(st);
+            hipFree(save);
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) {
+        c->err = "ERROR: autotuner allocation failed.";
+        return rc;
+    }
+    c->tune = best;
+    c->tuned = true;
+    drop_graph(c);
+    rc = check_device(c);
+    return rc > 0 ? rc : SPH_OK;
+}
+
 int sph_solution(void *ctx, int max_steps, int *steps_done)
 {
     Context *c = ctx_of(ctx);
     if (steps_done) *steps_done = 0;
+    if (!c->tuned && c->N >= kAutotuneMinN && max_steps > 0) {
+        const int rc = autotune(c, 3);
+        if (rc) return rc;
+    }
     if (sync_ctl(c)) return SPH_ERR_HIP;
     const int t0 = c->hctl->timestep;
     int remaining = max_steps;
@@ -941,6 +1040,20 @@ int sph_solution(void *ctx, int max_steps, int *steps_done)
     }
     if (steps_done) *steps_done = c->hctl->timestep - t0;
     return SPH_OK;
+}
+
+int sph_autotune(void *ctx, int reps, int32_t *shape)
+{
+    Context *c = ctx_of(ctx);
+    int rc = SPH_OK;
+    if (reps > 0) rc = autotune(c, reps);
+    else if (reps == 0) c->tuned = true;
+    if (shape) {
+        shape[0] = c->tune.cell_slots;
+        shape[1] = c->tune.strain32_batch;
+        shape[2] = c->tune.momentum32_batch;
+    }
+    return rc;
 }
 
 int sph_synchronize(void *ctx)

@@ -116,8 +116,9 @@ __device__ __forceinline__ double wave_min32(double v)
     return v;
 }
 
-// continuity sweep + RhoUpdate/EOS + critical step (fused mode of k_strain)
-template <int D>
+// continuity sweep + RhoUpdate/EOS + critical step (fused mode of k_strain);
+// G neighbour records in flight per lane
+template <int D, int G = kStrain32Batch>
 __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, const int bid)
 {
     Ctl *ctl = A.ctl;
@@ -147,7 +148,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
             const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
             struct Rec { float4 a, b; };
             auto load = [&](int j) { return Rec{A.fsA[j], A.fsB[j]}; };
-            for_each_nbr_batched<kStrain32Batch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+            for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
                 const float4 ra = rec.a, rb = rec.b;
                 const uint32_t pk_j = __float_as_uint(rb.w);
                 const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)]);
@@ -202,7 +203,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
 }
 
 // momentum sweep: pressure gradient, Morris viscosity, Monaghan term
-template <int D>
+template <int D, int G = kMomentum32Batch>
 __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
@@ -236,7 +237,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
         struct Rec { float4 a, b; float2 c; };
         auto load = [&](int j) { return Rec{A.fmA[j], A.fmB[j], A.fmC[j]}; };
-        for_each_nbr_batched<kMomentum32Batch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+        for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
             const float2 rc = rec.c;
             const uint32_t fj = __float_as_uint(rc.y);
@@ -287,20 +288,20 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
     for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
 }
 
-template <int D> __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
+template <int D, int G> __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) strain32_body<D>(P, A, b);
+    if (b < nb) strain32_body<D, G>(P, A, b);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     strain32_body<D>(Pt[e.x], At[e.x], e.y);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
+template <int D, int G> __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) momentum32_body<D>(P, A, b);
+    if (b < nb) momentum32_body<D, G>(P, A, b);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *Pt, const Arrays *At, const int2 *map)
 {
@@ -328,21 +329,43 @@ void launch_batch_momentum32(const BatchView &B, hipStream_t st)
 
 static inline int nblk32(int n) { return (n + kBlock - 1) / kBlock; }
 
-void launch_strain32(const Params &P, const Arrays &A, hipStream_t st)
+template <int G>
+static void strain32_g(const Params &P, const Arrays &A, hipStream_t st)
 {
+    const dim3 grid(xcd_grid(nblk32(P.N))), blk(kBlock);
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL(k_strain32<1>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL(k_strain32<2>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
-    default: hipLaunchKernelGGL(k_strain32<3>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_strain32<1, G>), grid, blk, 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL((k_strain32<2, G>), grid, blk, 0, st, P, A); break;
+    default: hipLaunchKernelGGL((k_strain32<3, G>), grid, blk, 0, st, P, A); break;
     }
 }
 
-void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st)
+template <int G>
+static void momentum32_g(const Params &P, const Arrays &A, hipStream_t st)
 {
+    const dim3 grid(xcd_grid(nblk32(P.N))), blk(kBlock);
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL(k_momentum32<1>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL(k_momentum32<2>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
-    default: hipLaunchKernelGGL(k_momentum32<3>, dim3(xcd_grid(nblk32(P.N))), dim3(kBlock), 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_momentum32<1, G>), grid, blk, 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL((k_momentum32<2, G>), grid, blk, 0, st, P, A); break;
+    default: hipLaunchKernelGGL((k_momentum32<3, G>), grid, blk, 0, st, P, A); break;
+    }
+}
+
+void launch_strain32(const Params &P, const Arrays &A, hipStream_t st, int batch)
+{
+    switch (batch) {
+    case 2: strain32_g<2>(P, A, st); break;
+    case 4: strain32_g<4>(P, A, st); break;
+    default: strain32_g<8>(P, A, st); break;
+    }
+}
+
+void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st, int batch)
+{
+    switch (batch) {
+    case 2: momentum32_g<2>(P, A, st); break;
+    case 8: momentum32_g<8>(P, A, st); break;
+    default: momentum32_g<4>(P, A, st); break;
     }
 }
 

@@ -1620,17 +1620,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
     nbr_cells_body<D, ONEMAT>(P, A, shift, b * kCellWaves + wv, nb * kCellWaves, stage[wv]);
 }
 
-// grid of the cell kernel: about kCellSlotsPerWave slots per wave (a few
-// cells), so that waves of uneven cost balance over several rounds
-constexpr int kCellSlotsPerWave = 32;
-static int cell_blocks(int n)
+// grid of the cell kernel: about `slots` slots per wave (a few cells), so
+// that waves of uneven cost balance over several rounds
+static int cell_blocks(int n, int slots)
 {
-    static int forced = -1;
-    if (forced < 0) {
-        const char *e = getenv("SPH07_CELL_BLOCKS");
-        forced = e ? atoi(e) : 0;
-    }
-    long long nb = forced > 0 ? forced : ((long long)n + kCellSlotsPerWave * kCellWaves - 1) / (kCellSlotsPerWave * kCellWaves);
+    slots = slots < 4 ? 4 : slots;
+    long long nb = ((long long)n + (long long)slots * kCellWaves - 1) / ((long long)slots * kCellWaves);
     nb = nb < 8 ? 8 : (nb > (1 << 20) ? (1 << 20) : nb);
     return xcd_grid((int)nb);
 }
@@ -1782,12 +1777,12 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
         }
 }
 
-void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st)
+void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots)
 {
     if (P.nmat == 1) {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true>), dim3(cell_blocks(P.N)), dim3(kBlock), 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true>), dim3(cell_blocks(P.N, cell_slots)), dim3(kBlock), 0, st, P, A, shift));
     } else {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false>), dim3(cell_blocks(P.N)), dim3(kBlock), 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false>), dim3(cell_blocks(P.N, cell_slots)), dim3(kBlock), 0, st, P, A, shift));
     }
     DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_nbr_ovf<DD>, dim3(kOvfBlocks), dim3(kBlock), 0, st, P, A, shift));
 }

@@ -71,7 +71,15 @@ enum { M_BC = 1, M_VEL = 2, M_MOVE = 4, M_RHO = 8 };
 void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st);
 void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st);
 void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st);
-void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st);
+// Launch shapes that change timing, never results (chosen per scene by the
+// autotuner of sph_capi.cpp; defaults measured on the 3D dam break).
+struct Tune {
+    int cell_slots = 32;       // slots per wave of the cell-centric neighbour kernel
+    int strain32_batch = 8;    // neighbour records in flight per lane, FP32c continuity
+    int momentum32_batch = 4;  // the same, FP32c momentum
+};
+
+void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots = 32);
 void launch_shift(const Params &P, const Arrays &A, hipStream_t st);
 void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st);
 void launch_normals(const Params &P, const Arrays &A, hipStream_t st);
@@ -107,7 +115,7 @@ void launch_batch_strain32(const BatchView &B, hipStream_t st);
 void launch_batch_momentum32(const BatchView &B, hipStream_t st);
 
 // FP32-compensated pair sweeps (sph_fp32c.hip); fused-step forms only
-void launch_strain32(const Params &P, const Arrays &A, hipStream_t st);
-void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st);
+void launch_strain32(const Params &P, const Arrays &A, hipStream_t st, int batch = 8);
+void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st, int batch = 4);
 
 }  // namespace sph

@@ -207,6 +207,15 @@ class Scene:
     def set_timing(self, on: bool):
         self._chk(lib().sph_set_timing(self._h, 1 if on else 0))
 
+    def autotune(self, reps: int = 3) -> dict:
+        """sph_autotune: reps > 0 tunes now, 0 freezes the current launch
+        shapes (no automatic tuning), < 0 only reports them."""
+        L = lib()
+        L.sph_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
+        shape = (ctypes.c_int32 * 3)()
+        self._chk(L.sph_autotune(self._h, reps, shape))
+        return {"cell_slots": shape[0], "strain32_batch": shape[1], "momentum32_batch": shape[2]}
+
     def timing(self) -> dict:
         """{group name: (total ms, launches)} from HIP events on the scene's stream."""
         L = lib()

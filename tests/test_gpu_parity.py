@@ -197,3 +197,21 @@ def test_reference_decks(deckfile, steps):
     g = gpu_scene(deck)
     assert o.run(steps) == g.Solution(steps) == steps
     compare_state(g, o, RTOL, exact=not deck.rigid_body, what=deckfile)
+
+
+@pytest.mark.parametrize("arm_name", ["fp64", "fp32c"])
+def test_autotune_changes_timing_not_results(arm_name):
+    """A scene tuned in the middle of a run ends bit-identical to the same
+    scene with the tuning turned off."""
+    from sph07_amd.native import Scene, ARM_FP64, ARM_FP32C, F_X, F_V, F_RHO
+    arm = ARM_FP64 if arm_name == "fp64" else ARM_FP32C
+    deck = scenes.dam_break(nx_fluid=24, ny=40, nz_fluid=20, nx_tank=60, nz_tank=30, maxnbr=100)
+    tuned, plain = Scene(deck, arm), Scene(deck, arm)
+    plain.autotune(0)
+    assert tuned.Solution(5) == 5
+    shape = tuned.autotune(3)
+    assert tuned.Solution(15) == 15
+    assert plain.Solution(20) == 20
+    assert shape["cell_slots"] in (16, 32, 64, 128)
+    for f in (F_X, F_V, F_RHO):
+        assert np.array_equal(tuned.field(f), plain.field(f))
