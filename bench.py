@@ -170,7 +170,7 @@ def main():
     import numpy as np
     import torch
     from sph07_amd import scenes
-    from sph07_amd.native import Scene, ARM_FP64, ARM_FP32C, F_V, F_X, F_NNBR
+    from sph07_amd.native import Scene, ARM_FP64, ARM_FP32C, F_V, F_X, F_RHO, F_NNBR
 
     assert torch.cuda.is_available(), "no GPU visible"
     if args.calib_copy:
@@ -222,11 +222,16 @@ def main():
                               cand=candidate_tests(sc.field(F_X), deck))
     # both arms advanced the same number of steps from the same deck
     moving = deck.dispbc == 0
-    vel_err = None
+    vel_err = rho_err = None
     if "fp64" in arms and "fp32c" in arms:
         v64 = arms["fp64"]["scene"].field(F_V)[moving]
         v32 = arms["fp32c"]["scene"].field(F_V)[moving]
         vel_err = float(np.linalg.norm(v32 - v64) / max(np.linalg.norm(v64), 1e-300))
+        r64 = arms["fp64"]["scene"].field(F_RHO)
+        r32 = arms["fp32c"]["scene"].field(F_RHO)
+        rho0 = deck.materials[0].rho
+        # against the density deviation rho - rho0 (rho itself is ~rho0 everywhere)
+        rho_err = float(np.linalg.norm(r32 - r64) / max(np.linalg.norm(r64 - rho0), 1e-300))
     main_arm = "fp32c" if "fp32c" in arms else "fp64"
     pairs = float(arms[main_arm]["scene"].field(F_NNBR).sum())
     kmean = pairs / N
@@ -294,6 +299,7 @@ def main():
         "fp64_value": arms["fp64"]["rate"] if "fp64" in arms else None,
         "speedup_vs_fp64": value / arms["fp64"]["rate"] if "fp64" in arms else None,
         "fp32c_rel_l2_velocity_error_vs_fp64": vel_err,
+        "fp32c_rel_l2_density_deviation_error_vs_fp64": rho_err,
         "neighbour_interactions_per_s": pairs * args.steps / elapsed,
         "init_s": round(t_init, 3),
         "kernels_avg_ms": kernels,
