@@ -741,7 +741,58 @@ __device__ __forceinline__ double readlane_d(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
-template <int D, bool ONEMAT>
+// Targets t0 .. t1-1 of a target group against one staged chunk of
+// candidates (lane = candidate): pairs inside the cut-off are appended to each
+// target's list in candidate order.  SELF: the target's own slot lies in this
+// chunk, at lane self0 + t.
+template <int D, bool ONEMAT, bool SELF, bool MIND>
+__device__ __forceinline__ void nbr_chunk_targets(const Params &P, const Arrays &A, int t0, int t1, int tg, int lane,
+                                                  const double (&xi)[D], uint32_t fi, const double (&xc)[D],
+                                                  int matc, int cand, unsigned long long valid, int self0,
+                                                  double cut2_0, unsigned int cstride32, unsigned int maxnbr,
+                                                  int &nn, double &mind)
+{
+    int *lst = A.nbr + (size_t)(tg + t0) * kNbrChunk;
+    unsigned long long selfm = SELF ? (1ull << (self0 + t0)) : 0ull;
+    for (int t = t0; t < t1; t++, lst += kNbrChunk) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const double dd = readlane_d(xi[d], t) - xc[d];
+            d2 += dd * dd;
+        }
+        double cut2 = cut2_0;
+        if (!ONEMAT) {
+            const int matt = info_mat((uint32_t)__builtin_amdgcn_readlane((int)fi, t));
+            cut2 = P.pc[matt][matc].cut2;
+        }
+        unsigned long long pm = __builtin_amdgcn_ballot_w64(d2 < cut2) & valid;
+        if (SELF) {
+            pm &= ~selfm;
+            selfm <<= 1;
+        }
+        if (pm) {
+            const unsigned int nnt = (unsigned int)__builtin_amdgcn_readlane(nn, t);
+            const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
+            const bool pass = (pm >> lane) & 1ull;
+            if (pass && pos < maxnbr) lst[(size_t)(pos >> 3) * cstride32 + (pos & 7u)] = cand;
+            const int nnew = (int)(nnt + (unsigned)__popcll(pm));
+            nn = lane == t ? nnew : nn;
+            if (MIND) {
+                double r1 = pass ? sqrt(d2) : 1.0e20;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    const double y = __shfl_xor(r1, o, 64);
+                    r1 = y < r1 ? y : r1;
+                }
+                if (lane == t) mind = r1 < mind ? r1 : mind;
+            }
+        }
+    }
+}
+
+template <int D, bool ONEMAT, bool MIND>
 __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A, int shift, int wave, int nwave,
                                                int *stage)
 {
@@ -840,41 +891,14 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
                 const int nv = ncand - k0;
                 const unsigned long long valid = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
                 const int self0 = own + (tg - c0) - k0;   // lane of target 0 in this chunk
-                for (int t = 0; t < nt; t++) {
-                    double d2 = 0.0;
-#pragma unroll
-                    for (int d = 0; d < D; d++) {
-                        const double dd = readlane_d(xi[d], t) - xc[d];
-                        d2 += dd * dd;
-                    }
-                    double cut2 = cut2_0;
-                    if (!ONEMAT) {
-                        const int matt = info_mat((uint32_t)__builtin_amdgcn_readlane((int)fi, t));
-                        cut2 = P.pc[matt][matc].cut2;
-                    }
-                    const int sl = self0 + t;
-                    const unsigned long long selfm = (sl >= 0 && sl < 64) ? (1ull << sl) : 0ull;
-                    const unsigned long long pm = __builtin_amdgcn_ballot_w64(d2 < cut2) & valid & ~selfm;
-                    if (pm) {
-                        const unsigned int nnt = (unsigned int)__builtin_amdgcn_readlane(nn, t);
-                        const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
-                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
-                        int *lst = A.nbr + (size_t)(tg + t) * kNbrChunk;
-                        const bool pass = (pm >> lane) & 1ull;
-                        if (pass && pos < maxnbr) lst[(size_t)(pos >> 3) * cstride32 + (pos & 7u)] = cand;
-                        const int nnew = (int)(nnt + (unsigned)__popcll(pm));
-                        nn = lane == t ? nnew : nn;
-                        if (P.want_mindist) {
-                            double r1 = pass ? sqrt(d2) : 1.0e20;
-#pragma unroll
-                            for (int o = 32; o >= 1; o >>= 1) {
-                                const double y = __shfl_xor(r1, o, 64);
-                                r1 = y < r1 ? y : r1;
-                            }
-                            if (lane == t) mind = r1 < mind ? r1 : mind;
-                        }
-                    }
-                }
+                // targets whose own slot is in this chunk: t in [ta, tb)
+                const int ta = min(max(-self0, 0), nt), tb = min(max(64 - self0, 0), nt);
+                nbr_chunk_targets<D, ONEMAT, false, MIND>(P, A, 0, ta, tg, lane, xi, fi, xc, matc, cand, valid,
+                                                          self0, cut2_0, cstride32, maxnbr, nn, mind);
+                nbr_chunk_targets<D, ONEMAT, true, MIND>(P, A, ta, tb, tg, lane, xi, fi, xc, matc, cand, valid,
+                                                         self0, cut2_0, cstride32, maxnbr, nn, mind);
+                nbr_chunk_targets<D, ONEMAT, false, MIND>(P, A, tb, nt, tg, lane, xi, fi, xc, matc, cand, valid,
+                                                          self0, cut2_0, cstride32, maxnbr, nn, mind);
             }
             if (tv) {
                 if (nn > P.maxnbr) {
@@ -1610,14 +1634,14 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Para
 template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { XCD_BID(P); neighbours_body<D>(P, A, shift, bid_); }
 // The grid is a multiple of 8 blocks; each wave takes an equal share of the
 // slots.  At most 77 VGPRs (6 waves per SIMD) without spills.
-template <int D, bool ONEMAT>
+template <int D, bool ONEMAT, bool MIND>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_nbr_cells(Params P, Arrays A, int shift)
 {
     __shared__ int stage[kCellWaves][kCandCap];
     const int nb = (int)gridDim.x;
     const int b = xcd_block(nb);
     const int wv = (int)(threadIdx.x >> 6);
-    nbr_cells_body<D, ONEMAT>(P, A, shift, b * kCellWaves + wv, nb * kCellWaves, stage[wv]);
+    nbr_cells_body<D, ONEMAT, MIND>(P, A, shift, b * kCellWaves + wv, nb * kCellWaves, stage[wv]);
 }
 
 // grid of the cell kernel: about `slots` slots per wave (a few cells), so
@@ -1779,10 +1803,15 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots)
 {
-    if (P.nmat == 1) {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true>), dim3(cell_blocks(P.N, cell_slots)), dim3(kBlock), 0, st, P, A, shift));
+    const dim3 grid(cell_blocks(P.N, cell_slots)), blk(kBlock);
+    if (P.nmat == 1 && !P.want_mindist) {
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true, false>), grid, blk, 0, st, P, A, shift));
+    } else if (P.nmat == 1) {
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true, true>), grid, blk, 0, st, P, A, shift));
+    } else if (!P.want_mindist) {
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false, false>), grid, blk, 0, st, P, A, shift));
     } else {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false>), dim3(cell_blocks(P.N, cell_slots)), dim3(kBlock), 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false, true>), grid, blk, 0, st, P, A, shift));
     }
     DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_nbr_ovf<DD>, dim3(kOvfBlocks), dim3(kBlock), 0, st, P, A, shift));
 }
