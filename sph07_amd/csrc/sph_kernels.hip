@@ -852,8 +852,8 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         const int cnt = ng + len;
         const int incl = wave_incl_scan(cnt);
         const int off = incl - cnt;
-        const int ncand = __builtin_amdgcn_readlane(incl, 63);
-        const int own = __builtin_amdgcn_readlane(off + ng, D == 3 ? 13 : (D == 2 ? 4 : 1));
+        int ncand = __builtin_amdgcn_readlane(incl, 63);
+        int own = __builtin_amdgcn_readlane(off + ng, D == 3 ? 13 : (D == 2 ? 4 : 1));
         const int c0 = head, c1 = jn;
         if (ncand > kCandCap) {
             // left to k_nbr_ovf (per-particle walk)
@@ -865,6 +865,42 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         // staged slots are read back by other lanes of the same wave (LDS
         // operations of one wave complete in order)
         __builtin_amdgcn_wave_barrier();
+        {
+            // Drop the candidates farther than the largest cut-off from the
+            // cell's box (parts of the edge and corner cells): no target of
+            // the cell can reach them.  The margin is far above FP64 rounding
+            // of the box edges; order is kept (in-place compaction).
+            double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                blo[d] = ctl->gmin[d] + bc[d] / ctl->gsize[d];
+                bhi[d] = ctl->gmin[d] + (bc[d] + 1) / ctl->gsize[d];
+            }
+            const double reach2 = P.cut2max * (1.0 + 1.0e-9);
+            int nkeep = 0, own_new = 0;
+            for (int k0 = 0; k0 < ncand; k0 += 64) {
+                const int kk = k0 + lane;
+                const bool in = kk < ncand;
+                const int j = in ? stage[kk] : c0;
+                double e2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    const double xj = A.x[d][j];
+                    const double e = xj < blo[d] ? blo[d] - xj : (xj > bhi[d] ? xj - bhi[d] : 0.0);
+                    e2 += e * e;
+                }
+                const unsigned long long km = __builtin_amdgcn_ballot_w64(in && e2 <= reach2);
+                const int pos = nkeep + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
+                if ((km >> lane) & 1ull) stage[pos] = j;
+                const int ol = own - k0;            // the cell's own particles are always kept
+                if (ol >= 0 && ol < 64) own_new = nkeep + (int)__popcll(km & ((1ull << ol) - 1ull));
+                nkeep += (int)__popcll(km);
+            }
+            ncand = nkeep;
+            own = own_new;
+            __builtin_amdgcn_wave_barrier();
+        }
         for (int tg = c0; tg < c1; tg += 64) {
             const int nt = __builtin_amdgcn_readfirstlane(c1 - tg < 64 ? c1 - tg : 64);
             const bool tv = lane < nt;
