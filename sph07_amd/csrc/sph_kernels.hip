@@ -878,14 +878,26 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             }
             const double reach2 = P.cut2max * (1.0 + 1.0e-9);
             int nkeep = 0, own_new = 0;
+            int j_n = lane < ncand ? stage[lane] : c0;
+            double xj_n[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) xj_n[d] = A.x[d][j_n];
             for (int k0 = 0; k0 < ncand; k0 += 64) {
                 const int kk = k0 + lane;
                 const bool in = kk < ncand;
-                const int j = in ? stage[kk] : c0;
+                const int j = j_n;
+                double xjc[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) xjc[d] = xj_n[d];
+                if (k0 + 64 < ncand) {          // next chunk's loads in flight during this one
+                    j_n = kk + 64 < ncand ? stage[kk + 64] : c0;
+#pragma unroll
+                    for (int d = 0; d < D; d++) xj_n[d] = A.x[d][j_n];
+                }
                 double e2 = 0.0;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
-                    const double xj = A.x[d][j];
+                    const double xj = xjc[d];
                     const double e = xj < blo[d] ? blo[d] - xj : (xj > bhi[d] ? xj - bhi[d] : 0.0);
                     e2 += e * e;
                 }
@@ -914,13 +926,22 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             double mind = 1.0e20;
             const int mat0 = info_mat((uint32_t)__builtin_amdgcn_readfirstlane((int)fi));
             const double cut2_0 = P.pc[mat0][mat0].cut2;
+            // the next chunk's candidates are loaded while this chunk is tested
+            int cand_n = lane < ncand ? stage[lane] : tg;      // lanes past the list read a valid slot
+            double xc_n[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) xc_n[d] = A.x[d][cand_n];
             for (int k0 = 0; k0 < ncand; k0 += 64) {
-                const int kk = k0 + lane;
-                const bool cv = kk < ncand;
-                const int cand = cv ? stage[kk] : tg;      // lanes past the list read a valid slot
+                const int cand = cand_n;
                 double xc[D];
 #pragma unroll
-                for (int d = 0; d < D; d++) xc[d] = A.x[d][cand];
+                for (int d = 0; d < D; d++) xc[d] = xc_n[d];
+                if (k0 + 64 < ncand) {
+                    const int kn = k0 + 64 + lane;
+                    cand_n = kn < ncand ? stage[kn] : tg;
+#pragma unroll
+                    for (int d = 0; d < D; d++) xc_n[d] = A.x[d][cand_n];
+                }
                 const int matc = ONEMAT ? 0 : info_mat(A.info[cand]);
                 // lanes holding a candidate, and the lane of target t itself
                 // (staged at own + (tg + t - c0)): wave-uniform masks
