@@ -26,6 +26,10 @@
 namespace sph {
 
 constexpr int kStrainBatch = SPH_STRAIN_BATCH, kMomentumBatch = SPH_MOMENTUM_BATCH;
+#ifndef SPH_NBR_BOX_FILTER
+#define SPH_NBR_BOX_FILTER 1
+#endif
+constexpr bool kNbrBoxFilter = SPH_NBR_BOX_FILTER;   // cell-box filter of the staged candidates
 
 #define CTL_STOP(ctl) ((ctl)->err != 0 || (ctl)->done != 0)
 
@@ -865,7 +869,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         // staged slots are read back by other lanes of the same wave (LDS
         // operations of one wave complete in order)
         __builtin_amdgcn_wave_barrier();
-        {
+        if (kNbrBoxFilter) {
             // Drop the candidates farther than the largest cut-off from the
             // cell's box (parts of the edge and corner cells): no target of
             // the cell can reach them.  The margin is far above FP64 rounding
