@@ -601,6 +601,20 @@ int create_impl(Context *c, const sph_scene_desc *d)
         hinfo[i] = f;
     }
     if (P.rigid && rigid_idx.empty()) { c->err = "rigid body has no particles"; return SPH_ERROR; }
+    // compact FP32c momentum records: one mass per material and the linear EOS
+    {
+        bool uniform = c->arm == SPH_ARM_FP32C;
+        std::vector<double> mref(kMaxMat + 1, -1.0);
+        for (int i = 0; i < N && uniform; i++) {
+            const int mt = d->mat_id[i];
+            if (mref[mt] < 0.0) mref[mt] = hm[i];
+            else if (hm[i] != mref[mt]) uniform = false;
+        }
+        for (int m = 1; m <= d->nummat && uniform; m++)
+            if (d->mat[m - 1].eos != 29) uniform = false;
+        P.mcompact = uniform ? 1 : 0;
+        for (int m = 0; m <= kMaxMat; m++) P.mass32[m] = mref[m] > 0.0 ? (float)mref[m] : 0.0f;
+    }
     P.n_rigid = (int)rigid_idx.size();
 
     // ---- capacities ----

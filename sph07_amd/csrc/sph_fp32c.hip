@@ -185,7 +185,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
             A.prho2[i] = pr;
             const int q = A.occ[i];
             const int bc[3] = {code_ext(q, ctl->cmask[0]), code_ext(q, ctl->cmask[1]), code_ext(q, ctl->cmask[2])};
-            put_mrec32<D>(A, ctl, i, xr, A.m[i], vi, rho_i, pr, fi, bc);
+            put_mrec32<D>(P, A, ctl, i, xr, A.m[i], vi, rho_i, pr, fi, bc);
         }
         A.rho_new[i] = rho_i;
         const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
@@ -203,7 +203,9 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
 }
 
 // momentum sweep: pressure gradient, Morris viscosity, Monaghan term
-template <int D, int G = kMomentum32Batch>
+// COMPACT: the 32-byte records of Params::mcompact (mass and p/rho^2 from the
+// material and rho - rho0; images, flagged in the packed word, have p = 0)
+template <int D, int G = kMomentum32Batch, bool COMPACT = false>
 __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
@@ -224,7 +226,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const bool monaghan = mi.visc_type == 2;
         float cs[D], ri[D], vi[D], as[D], ac[D];
         const float4 fa = A.fmA[i], fb = A.fmB[i];
-        const uint32_t pk_i = __float_as_uint(A.fmC[i].y);
+        const uint32_t pk_i = COMPACT ? __float_as_uint(fb.w) : __float_as_uint(A.fmC[i].y);
         const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
 #pragma unroll
         for (int d = 0; d < D; d++) {
@@ -236,15 +238,32 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         }
         const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
         struct Rec { float4 a, b; float2 c; };
-        auto load = [&](int j) { return Rec{A.fmA[j], A.fmB[j], A.fmC[j]}; };
+        auto load = [&](int j) {
+            Rec r;
+            r.a = A.fmA[j];
+            r.b = A.fmB[j];
+            if (!COMPACT) r.c = A.fmC[j];
+            return r;
+        };
+        const float rho0_i = (float)mi.rho0, c2_i = (float)(mi.c * mi.c), m_i = P.mass32[mat_i];
         for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
-            const float2 rc = rec.c;
-            const uint32_t fj = __float_as_uint(rc.y);
+            const uint32_t fj = COMPACT ? __float_as_uint(rb.w) : __float_as_uint(rec.c.y);
             const int mat_j = P.nmat == 1 ? mat_i : rec_mat(fj);
             const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][mat_j]);
-            const float mj = ra.w;
-            const float rho_j = rb.w;
+            float mj, rho_j, prho2_j;
+            if (COMPACT) {
+                const float rho0_j = P.nmat == 1 ? rho0_i : (float)P.mat[mat_j].rho0;
+                const float c2_j = P.nmat == 1 ? c2_i : (float)(P.mat[mat_j].c * P.mat[mat_j].c);
+                mj = P.nmat == 1 ? m_i : P.mass32[mat_j];
+                rho_j = rho0_j + ra.w;
+                const float ir = __builtin_amdgcn_rcpf(rho_j);
+                prho2_j = (fj & kRecGhost) ? 0.0f : c2_j * ra.w * ir * ir;
+            } else {
+                mj = ra.w;
+                rho_j = rb.w;
+                prho2_j = rec.c.x;
+            }
             const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
             float off[3];
             cell_offs(pk_i, fj, off);
@@ -257,7 +276,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
                 vr += dv[d] * dx[d];
             }
             const float f = gradw_factor(r2, q);            // grad W = f * dx
-            float sdw = -mj * (rc.x + pr_i);                // coefficient of grad W
+            float sdw = -mj * (prho2_j + pr_i);             // coefficient of grad W
             float tv = 0.0f;                                // coefficient of (v_i - v_j)
             if (!(rig_i && (fj & kRecRigid))) {
                 const float qj = (float)P.mat[mat_j].nu * rho_j;
@@ -298,15 +317,16 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Par
     const int2 e = map[blockIdx.x];
     strain32_body<D>(Pt[e.x], At[e.x], e.y);
 }
-template <int D, int G> __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
+template <int D, int G, bool COMPACT> __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) momentum32_body<D, G>(P, A, b);
+    if (b < nb) momentum32_body<D, G, COMPACT>(P, A, b);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    momentum32_body<D>(Pt[e.x], At[e.x], e.y);
+    if (Pt[e.x].mcompact) momentum32_body<D, kMomentum32Batch, true>(Pt[e.x], At[e.x], e.y);
+    else momentum32_body<D, kMomentum32Batch, false>(Pt[e.x], At[e.x], e.y);
 }
 
 void launch_batch_strain32(const BatchView &B, hipStream_t st)
@@ -340,15 +360,22 @@ static void strain32_g(const Params &P, const Arrays &A, hipStream_t st)
     }
 }
 
-template <int G>
-static void momentum32_g(const Params &P, const Arrays &A, hipStream_t st)
+template <int G, bool C>
+static void momentum32_gc(const Params &P, const Arrays &A, hipStream_t st)
 {
     const dim3 grid(xcd_grid(nblk32(P.N))), blk(kBlock);
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL((k_momentum32<1, G>), grid, blk, 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL((k_momentum32<2, G>), grid, blk, 0, st, P, A); break;
-    default: hipLaunchKernelGGL((k_momentum32<3, G>), grid, blk, 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_momentum32<1, G, C>), grid, blk, 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL((k_momentum32<2, G, C>), grid, blk, 0, st, P, A); break;
+    default: hipLaunchKernelGGL((k_momentum32<3, G, C>), grid, blk, 0, st, P, A); break;
     }
+}
+
+template <int G>
+static void momentum32_g(const Params &P, const Arrays &A, hipStream_t st)
+{
+    if (P.mcompact) momentum32_gc<G, true>(P, A, st);
+    else momentum32_gc<G, false>(P, A, st);
 }
 
 void launch_strain32(const Params &P, const Arrays &A, hipStream_t st, int batch)

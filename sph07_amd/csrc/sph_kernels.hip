@@ -540,9 +540,9 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
             if (P.arm == 1) {
                 const int tb[3] = {i + g.off[0], j + g.off[1], k + g.off[2]};   // the image's cell
                 switch (P.ndim) {
-                case 1: put_srec32<1>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<1>(A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb); break;
-                case 2: put_srec32<2>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<2>(A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb); break;
-                default: put_srec32<3>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<3>(A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb);
+                case 1: put_srec32<1>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<1>(P, A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb); break;
+                case 2: put_srec32<2>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<2>(P, A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb); break;
+                default: put_srec32<3>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<3>(P, A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb);
                 }
             }
             A.gsrc[g0 + r] = src;

@@ -17,7 +17,7 @@
 
 namespace sph {
 
-constexpr uint32_t kRecMat = 0x0Fu, kRecRigid = 0x80u;
+constexpr uint32_t kRecMat = 0x0Fu, kRecGhost = 0x40u, kRecRigid = 0x80u;
 static_assert(kMaxMat < 16, "material fits the packed word");
 
 __device__ __forceinline__ float cell_size32(const Ctl *ctl, int d) { return (float)(1.0 / ctl->gsize[d]); }
@@ -29,7 +29,8 @@ __device__ __forceinline__ double cell_origin(const Ctl *ctl, int d, int c)
 
 __device__ __forceinline__ uint32_t pack32(uint32_t info, const int *bc)
 {
-    return (info & kRecMat) | ((info & kRigid) ? kRecRigid : 0u) | ((uint32_t)(bc[0] & 3) << 8) |
+    return (info & kRecMat) | ((info & kRigid) ? kRecRigid : 0u) | ((info & kGhost) ? kRecGhost : 0u) |
+           ((uint32_t)(bc[0] & 3) << 8) |
            ((uint32_t)(bc[1] & 3) << 16) | ((uint32_t)(bc[2] & 3) << 24);
 }
 
@@ -73,16 +74,25 @@ __device__ __forceinline__ void put_srec32(const Arrays &A, const Ctl *ctl, int 
     A.fsB[j] = b;
 }
 
-// momentum record: {r[0..2], m}, {v[0..2], rho}, {p/rho^2, packed}
+// momentum record: {r[0..2], m}, {v[0..2], rho}, {p/rho^2, packed}; with
+// Params::mcompact {r[0..2], rho - rho0}, {v[0..2], packed} (32 bytes)
 template <int D>
-__device__ __forceinline__ void put_mrec32(const Arrays &A, const Ctl *ctl, int j, const double *x, double m,
-                                           const double *v, double rho, double prho2, uint32_t info, const int *bc)
+__device__ __forceinline__ void put_mrec32(const Params &P, const Arrays &A, const Ctl *ctl, int j, const double *x,
+                                           double m, const double *v, double rho, double prho2, uint32_t info,
+                                           const int *bc)
 {
     float4 a = make_float4(0.f, 0.f, 0.f, (float)m), b = make_float4(0.f, 0.f, 0.f, (float)rho);
     a.x = (float)(x[0] - cell_origin(ctl, 0, bc[0]));
     b.x = (float)v[0];
     if (D > 1) { a.y = (float)(x[1] - cell_origin(ctl, 1, bc[1])); b.y = (float)v[1]; }
     if (D > 2) { a.z = (float)(x[2] - cell_origin(ctl, 2, bc[2])); b.z = (float)v[2]; }
+    if (P.mcompact) {
+        a.w = (float)(rho - P.mat[info & kMatMask].rho0);
+        b.w = __uint_as_float(pack32(info, bc));
+        A.fmA[j] = a;
+        A.fmB[j] = b;
+        return;
+    }
     A.fmA[j] = a;
     A.fmB[j] = b;
     A.fmC[j] = make_float2((float)prho2, __uint_as_float(pack32(info, bc)));

@@ -92,6 +92,11 @@ struct Params {
     int ncell_cap;
     int want_mindist;
     int arm;              // 0 FP64, 1 FP32-compensated (selects the records the fused step writes)
+    // FP32c momentum record in 32 bytes: every material has one particle mass
+    // (mass32) and the linear EOS, so m and p/rho^2 follow from the material
+    // and the stored rho - rho0
+    int mcompact;
+    float mass32[kMaxMat + 1];
     int btype[3];
     int bcode[2][3];
     double bx[2][3];

@@ -186,6 +186,24 @@ def test_fp32c_arm_error(name, steps):
     assert np.array_equal(nn64, nn32)
 
 
+def test_fp32c_arm_error_varied_masses():
+    """Per-particle masses (massopt 1) that differ within a material: the
+    FP32c momentum sweep uses its 40-byte records (mass and p/rho^2 stored)."""
+    from sph07_amd.native import Scene, ARM_FP32C, F_V
+    deck = scenes.dam_break(nx_fluid=10, ny=8, nz_fluid=10, nx_tank=20, nz_tank=14, maxnbr=100)
+    m0 = deck.materials[0].mass / deck.np
+    k = np.arange(deck.np)
+    deck.node_mass = m0 * (1.0 + 0.01 * np.sin(0.37 * k))
+    deck.massopt = 1
+    g64, g32 = Scene(deck), Scene(deck, ARM_FP32C)
+    assert g64.Solution(60) == g32.Solution(60) == 60
+    moving = deck.dispbc == 0
+    v64, v32 = g64.field(F_V)[moving], g32.field(F_V)[moving]
+    err = np.linalg.norm(v32 - v64) / np.linalg.norm(v64)
+    print(f"varied masses: fp32c rel L2 velocity error {err:.3e}")
+    assert err <= 1e-5, err
+
+
 @pytest.mark.parametrize("deckfile,steps", [("rigidbody.mcm", 100), ("kuglica.mcm", 100)])
 def test_reference_decks(deckfile, steps):
     """Two of the reference's own input decks, read by the library's .mcm
