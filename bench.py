@@ -91,7 +91,7 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="extra steps run with per-kernel HIP-event timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--small", action="store_true", help="reduced dam break (for quick checks)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for HBM counters)")
@@ -279,10 +279,12 @@ def main():
         t0 = time.perf_counter()
         o.run(args.cpu_steps)
         tc = time.perf_counter() - t0
-        cpu = {"value": N * args.cpu_steps / tc, "unit": "particle-updates/s", "cores": 1,
+        nthr = orc.threads()
+        cpu = {"value": N * args.cpu_steps / tc, "unit": "particle-updates/s", "cores": nthr,
                "kind": "port",
                "sample": f"{args.cpu_steps} full steps of the same {N}-particle dam break "
-                         f"(after its untimed initialisation), single thread, {tc:.1f} s"}
+                         f"(after its untimed initialisation), FP64 restatement, {nthr} OpenMP "
+                         f"threads over particles, {tc:.1f} s"}
         del o
 
     serve = serving_leg(args) if args.serve_requests > 0 else None

@@ -136,3 +136,8 @@ def kernel_gradw(xi, xj, ndim, havg) -> np.ndarray:
     out = np.zeros(3)
     lib().orc_kernel_gradw(out.ctypes.data, xi.ctypes.data, xj.ctypes.data, ndim, havg)
     return out
+
+
+def threads() -> int:
+    """OpenMP threads of the oracle's per-particle loops (OMP_NUM_THREADS)."""
+    return int(lib().orc_threads())

@@ -342,10 +342,13 @@ static int ghost_setup(orc_scene *s)
 /* MSolution::LinkedListNeighbours, MSolution.cpp:150-215 */
 static int linked_list_neighbours(orc_scene *s)
 {
-    int bc[3], lo[3], hi[3];
+    int overflow = 0;
     for (int i = 1; i <= s->np; i++) s->nnbr[i] = 0;
+    /* particles are independent: OpenMP threads over i give the serial result */
+#pragma omp parallel for schedule(dynamic, 256) reduction(|: overflow)
     for (int i = 1; i <= s->np; i++) {
-        if (!s->active[i]) continue;
+        int bc[3], lo[3], hi[3];
+        if (!s->active[i] || overflow) continue;
         const double *xi = &s->x[3 * i];
         s->mindist[i] = 1.0e20;
         box_coords(s, xi, bc);
@@ -367,15 +370,20 @@ static int linked_list_neighbours(orc_scene *s)
                         double hav = 2.0 * AVG(s->h[i], pj.h);
                         if (d2 < SQR(hav)) {
                             s->mindist[i] = MINV(s->mindist[i], sqrt(d2));
-                            if (s->nnbr[i] < s->maxnbr)
+                            if (s->nnbr[i] < s->maxnbr) {
                                 s->nbr[(size_t)i * s->maxnbr + s->nnbr[i]++] = id;
-                            else
-                                return fail(s, "ERROR: Neighbour list exceeded maximum.");
+                            } else {
+                                overflow = 1;
+                                id = ORC_LLEND;
+                                r = hi[0] + 1; q = hi[1] + 1; t = hi[2] + 1;
+                                continue;
+                            }
                         }
                         id = next;
                     }
                 }
     }
+    if (overflow) return fail(s, "ERROR: Neighbour list exceeded maximum.");
     return 0;
 }
 
@@ -394,12 +402,13 @@ static int phase_neighbours(orc_scene *s, int with_ghosts)
    used by the Morris discretisation; MParticle::CalcTraceROD :43-47) */
 static int phase_strain(orc_scene *s)
 {
-    double dwdx[3];
     for (int i = 1; i <= s->np; i++) {
         if (!s->active[i]) continue;
         for (int n = 0; n < 3; n++) s->x[3 * i + n] = s->x[3 * i + n] - 0.5 * s->v[3 * i + n] * s->dt;
     }
+#pragma omp parallel for schedule(dynamic, 256)
     for (int i = 1; i <= s->np; i++) {
+        double dwdx[3];
         if (!s->active[i]) continue;
         const double *xi = &s->x[3 * i], *vi = &s->v[3 * i];
         double g[3] = {0.0, 0.0, 0.0};   /* diagonal of grad v */
@@ -496,7 +505,9 @@ static int phase_constitutive(orc_scene *s)
 /* MSolution::CalcCriticalTimestep, MSolution.cpp:34-94 */
 static int calc_critical_timestep(orc_scene *s)
 {
-    s->critts = 1.e20;
+    if (s->tcrit_opt < 1 || s->tcrit_opt > 3) return fail(s, "ERROR: Wrong critical time step option.");
+    double critts = 1.e20;
+#pragma omp parallel for schedule(static) reduction(min: critts)
     for (int i = 1; i <= s->np; i++) {
         double ct;
         const orc_material *mt = &s->mat[s->mat_id[i]];
@@ -519,11 +530,13 @@ static int calc_critical_timestep(orc_scene *s)
             break;
         }
         default:
-            return fail(s, "ERROR: Wrong critical time step option.");
+            ct = 1.e20;
+            break;
         }
         s->critts_p[i] = ct;
-        if (s->active[i] && !rigid_p(s, s->mat_id[i]) && ct < s->critts) s->critts = ct;
+        if (s->active[i] && !rigid_p(s, s->mat_id[i]) && ct < critts) critts = ct;
     }
+    s->critts = critts;
     return 0;
 }
 
@@ -577,8 +590,9 @@ static int rigid_acceleration(orc_scene *s)
    (its visc_type field is parsed and unused, MSimulationData.cpp:454-455). */
 static int phase_momentum(orc_scene *s)
 {
-    double dwdx[3];
+#pragma omp parallel for schedule(dynamic, 256)
     for (int i = 1; i <= s->np; i++) {
+        double dwdx[3];
         double *ai = &s->a[3 * i];
         ai[0] = ai[1] = ai[2] = 0.0;
         if (!s->active[i]) continue;
@@ -898,6 +912,12 @@ void orc_destroy(orc_scene *s)
 
 const char *orc_last_error(orc_scene *s) { return s ? s->err : "null scene"; }
 int orc_np(orc_scene *s) { return s->np; }
+#ifdef _OPENMP
+#include <omp.h>
+int orc_threads(void) { return omp_get_max_threads(); }
+#else
+int orc_threads(void) { return 1; }
+#endif
 int orc_ngp(orc_scene *s) { return s->ngp; }
 
 int orc_globals(orc_scene *s, double *out, int n)

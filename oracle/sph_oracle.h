@@ -82,6 +82,8 @@ int         orc_phase(orc_scene *s, int phase);
    the reference's stop test (t - dt >= endtime) fired. */
 int         orc_run(orc_scene *s, int nsteps, int *steps_done, int *done);
 int         orc_np(orc_scene *s);
+/* OpenMP threads the per-particle loops use (1 without OpenMP) */
+int orc_threads(void);
 int         orc_ngp(orc_scene *s);
 /* Globals: [dt, dtold, ptime, critts, timestep, hmax,
              cmm_min0..2, cmm_max0..2, ngp, rbR0..2, rbV0..2, rbW0..2, rbA0..2, rbAL0..2] */
