@@ -22,16 +22,20 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-updates/s on 1 GPU vs plain FP64 GPU path; p50/p99 scene latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# MI355X's published vector FP32 rate (157.3 TF) counts packed FP32 FMAs; the
+# library is compiled with packed FP32 turned off, so its FP32 VALU ceiling is
+# one FMA per lane per clock, the same 78.6 TF as FP64
+FP32_PEAK_TFLOPS = 78.6
 
-FP32_PEAK_TFLOPS = 157.3
 
-
-def alg_bytes_3d(group, arm, k):
+def alg_bytes_3d(group, arm, k, compact=False):
     """Algorithmic bytes per particle for one launch of a timed kernel group,
     ndim = 3: every field the launch reads or writes counted once, its outputs
     included (the neighbour list: 4 bytes per entry); the neighbours' re-reads
-    of each other's records are not algorithmic.  k = mean list length."""
-    rec_s, rec_m = (32, 40) if arm == "fp32c" else (64, 80)
+    of each other's records are not algorithmic.  k = mean list length;
+    compact: the FP32c momentum record is the 32-byte form (one mass per
+    material, linear EOS) rather than the 40-byte one."""
+    rec_s, rec_m = ((32, 32 if compact else 40) if arm == "fp32c" else (64, 80))
     if group == "k_neighbours":
         # read x, info, m, rho, v; write nnbr, list, xs, continuity record
         return 24 + 4 + 8 + 8 + 24 + 4 + 4 * k + 24 + rec_s
@@ -240,9 +244,11 @@ def main():
     for a in arms.values():
         a["scene"].close()
 
-    dom = max((g for g in kern if alg_bytes_3d(g, main_arm, kmean)), key=lambda g: kern[g][0])
+    # the library's rule for the compact FP32c momentum record
+    compact = deck.massopt != 1 and all(m.eos == 29 for m in deck.materials)
+    dom = max((g for g in kern if alg_bytes_3d(g, main_arm, kmean, compact)), key=lambda g: kern[g][0])
     avg_ms = kern[dom][0]
-    bytes_alg = alg_bytes_3d(dom, main_arm, kmean) * N
+    bytes_alg = alg_bytes_3d(dom, main_arm, kmean, compact) * N
     achieved = bytes_alg / (avg_ms * 1e-3) / 1e9 if avg_ms else 0.0
     traffic = None
     tf_path = os.path.join(ROOT, "profiles", "traffic_dam_break.json")
@@ -266,8 +272,9 @@ def main():
             tf = flops / (ms * 1e-3) / 1e12
             valu.setdefault(a, {})[g] = {"achieved_tflops": round(tf, 3), "peak": peak,
                                          "frac": round(tf / peak, 4)}
-    hbm = {a: {g: round(alg_bytes_3d(g, a, kmean) * N / (v[0] * 1e-3) / 1e9, 1)
-               for g, v in arms[a]["kern"].items() if v[1] and alg_bytes_3d(g, a, kmean)} for a in arms}
+    hbm = {a: {g: round(alg_bytes_3d(g, a, kmean, compact) * N / (v[0] * 1e-3) / 1e9, 1)
+               for g, v in arms[a]["kern"].items() if v[1] and alg_bytes_3d(g, a, kmean, compact)}
+           for a in arms}
     kernels = {a: {k: round(v[0], 4) for k, v in arms[a]["kern"].items() if v[1]} for a in arms}
 
     cpu = None
