@@ -729,13 +729,18 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
 // Cell-centric neighbour search (single-scene path).  One wave per occupied
 // cell: the stencil's candidates, in the traversal order of the walk above
 // (cells x -> y -> z, images newest first, then the cell's slots), are staged
-// as slot indices in LDS; lanes then take 64 candidates at a time (coalesced
-// position loads) and every target of the cell is broadcast in turn: a ballot
-// of the pairs inside the cut-off and a prefix count append them to the
-// target's list in candidate order, i.e. in the reference's order.
+// as slot indices in LDS.  Up to kNbrBlk chunks of 64 candidates are then
+// held in registers (lane = candidate, coalesced position loads) and every
+// target of the cell is broadcast in turn against all of them: a ballot of
+// the pairs inside the cut-off and a prefix count append them to the target's
+// list in candidate order, i.e. in the reference's order.  The target's count
+// stays wave-uniform across its chunks.
 // ---------------------------------------------------------------------------
 constexpr int kCandCap = 1024;        // staged candidates per wave
 constexpr int kCellWaves = kBlock / 64;
+// candidate chunks held in registers per pass: 4 for the common one-material
+// search, 2 where per-candidate materials or distances take registers too
+template <bool ONEMAT, bool MIND> constexpr int nbr_blk() { return ONEMAT && !MIND ? 4 : 2; }
 
 __device__ __forceinline__ double readlane_d(double v, int l)
 {
@@ -745,54 +750,58 @@ __device__ __forceinline__ double readlane_d(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
-// Targets t0 .. t1-1 of a target group against one staged chunk of
-// candidates (lane = candidate): pairs inside the cut-off are appended to each
-// target's list in candidate order.  SELF: the target's own slot lies in this
-// chunk, at lane self0 + t.
-template <int D, bool ONEMAT, bool SELF, bool MIND>
-__device__ __forceinline__ void nbr_chunk_targets(const Params &P, const Arrays &A, int t0, int t1, int tg, int lane,
-                                                  const double (&xi)[D], uint32_t fi, const double (&xc)[D],
-                                                  int matc, int cand, unsigned long long valid, int self0,
-                                                  double cut2_0, unsigned int cstride32, unsigned int maxnbr,
-                                                  int &nn, double &mind)
+// Target t of a target group against the nb (<= NB) resident chunks
+// that start at staged index kb: candidates inside the cut-off are appended
+// to the target's list in candidate order.  selfk: the target's own staged
+// index relative to kb.  List offsets are 32-bit (the launcher keeps lists of
+// 2^32 entries or more on the per-particle walk).
+template <int D, bool ONEMAT, bool MIND, int NB>
+__device__ __forceinline__ void nbr_target_block(const Params &P, const Arrays &A, int t, int tg, int lane, int kb,
+                                                 int nb, int ncand, int selfk, const double (&xi)[D], uint32_t fi,
+                                                 const int (&cv)[NB], const double (&xcv)[NB][D],
+                                                 const int (&matcv)[NB], double cut2_0,
+                                                 unsigned int cstride32, unsigned int maxnbr, int &nn, double &mind)
 {
-    int *lst = A.nbr + (size_t)(tg + t0) * kNbrChunk;
-    unsigned long long selfm = SELF ? (1ull << (self0 + t0)) : 0ull;
-    for (int t = t0; t < t1; t++, lst += kNbrChunk) {
-        double d2 = 0.0;
+    double xt[D];
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            const double dd = readlane_d(xi[d], t) - xc[d];
-            d2 += dd * dd;
-        }
-        double cut2 = cut2_0;
-        if (!ONEMAT) {
-            const int matt = info_mat((uint32_t)__builtin_amdgcn_readlane((int)fi, t));
-            cut2 = P.pc[matt][matc].cut2;
-        }
-        unsigned long long pm = __builtin_amdgcn_ballot_w64(d2 < cut2) & valid;
-        if (SELF) {
-            pm &= ~selfm;
-            selfm <<= 1;
-        }
-        if (pm) {
-            const unsigned int nnt = (unsigned int)__builtin_amdgcn_readlane(nn, t);
-            const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
-            const bool pass = (pm >> lane) & 1ull;
-            if (pass && pos < maxnbr) lst[(size_t)(pos >> 3) * cstride32 + (pos & 7u)] = cand;
-            const int nnew = (int)(nnt + (unsigned)__popcll(pm));
-            nn = lane == t ? nnew : nn;
-            if (MIND) {
-                double r1 = pass ? sqrt(d2) : 1.0e20;
+    for (int d = 0; d < D; d++) xt[d] = readlane_d(xi[d], t);
+    unsigned int nnt = (unsigned int)__builtin_amdgcn_readlane(nn, t);
+    int *lst = A.nbr + (size_t)(tg + t) * kNbrChunk;
+    const int matt = ONEMAT ? 0 : info_mat((uint32_t)__builtin_amdgcn_readlane((int)fi, t));
+    double r1 = 1.0e20;
 #pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) {
-                    const double y = __shfl_xor(r1, o, 64);
-                    r1 = y < r1 ? y : r1;
-                }
-                if (lane == t) mind = r1 < mind ? r1 : mind;
+    for (int c = 0; c < NB; c++) {
+        if (c < nb) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                const double dd = xt[d] - xcv[c][d];
+                d2 += dd * dd;
             }
+            const double cut2 = ONEMAT ? cut2_0 : P.pc[matt][matcv[c]].cut2;
+            const int k = c * 64 + lane;
+            const bool acc = d2 < cut2 && kb + k < ncand && k != selfk;
+            const unsigned long long pm = __builtin_amdgcn_ballot_w64(acc);
+            if (acc) {
+                const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
+                if (pos < maxnbr) lst[(pos >> 3) * cstride32 + (pos & 7u)] = cv[c];
+                if (MIND) {
+                    const double r = sqrt(d2);
+                    r1 = r < r1 ? r : r1;
+                }
+            }
+            nnt += (unsigned int)__popcll(pm);
         }
+    }
+    nn = lane == t ? (int)nnt : nn;
+    if (MIND) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const double y = __shfl_xor(r1, o, 64);
+            r1 = y < r1 ? y : r1;
+        }
+        if (lane == t) mind = r1 < mind ? r1 : mind;
     }
 }
 
@@ -930,36 +939,24 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             double mind = 1.0e20;
             const int mat0 = info_mat((uint32_t)__builtin_amdgcn_readfirstlane((int)fi));
             const double cut2_0 = P.pc[mat0][mat0].cut2;
-            // the next chunk's candidates are loaded while this chunk is tested
-            int cand_n = lane < ncand ? stage[lane] : tg;      // lanes past the list read a valid slot
-            double xc_n[D];
+            const int self0 = own + (tg - c0);          // staged index of target 0
+            constexpr int NB = nbr_blk<ONEMAT, MIND>();
+            for (int kb = 0; kb < ncand; kb += 64 * NB) {
+                int cv[NB];
+                double xcv[NB][D];
+                int matcv[NB];
 #pragma unroll
-            for (int d = 0; d < D; d++) xc_n[d] = A.x[d][cand_n];
-            for (int k0 = 0; k0 < ncand; k0 += 64) {
-                const int cand = cand_n;
-                double xc[D];
+                for (int c = 0; c < NB; c++) {
+                    const int k = kb + c * 64 + lane;
+                    cv[c] = k < ncand ? stage[k] : tg;      // lanes past the list read a valid slot
 #pragma unroll
-                for (int d = 0; d < D; d++) xc[d] = xc_n[d];
-                if (k0 + 64 < ncand) {
-                    const int kn = k0 + 64 + lane;
-                    cand_n = kn < ncand ? stage[kn] : tg;
-#pragma unroll
-                    for (int d = 0; d < D; d++) xc_n[d] = A.x[d][cand_n];
+                    for (int d = 0; d < D; d++) xcv[c][d] = A.x[d][cv[c]];
+                    matcv[c] = ONEMAT ? 0 : info_mat(A.info[cv[c]]);
                 }
-                const int matc = ONEMAT ? 0 : info_mat(A.info[cand]);
-                // lanes holding a candidate, and the lane of target t itself
-                // (staged at own + (tg + t - c0)): wave-uniform masks
-                const int nv = ncand - k0;
-                const unsigned long long valid = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
-                const int self0 = own + (tg - c0) - k0;   // lane of target 0 in this chunk
-                // targets whose own slot is in this chunk: t in [ta, tb)
-                const int ta = min(max(-self0, 0), nt), tb = min(max(64 - self0, 0), nt);
-                nbr_chunk_targets<D, ONEMAT, false, MIND>(P, A, 0, ta, tg, lane, xi, fi, xc, matc, cand, valid,
-                                                          self0, cut2_0, cstride32, maxnbr, nn, mind);
-                nbr_chunk_targets<D, ONEMAT, true, MIND>(P, A, ta, tb, tg, lane, xi, fi, xc, matc, cand, valid,
-                                                         self0, cut2_0, cstride32, maxnbr, nn, mind);
-                nbr_chunk_targets<D, ONEMAT, false, MIND>(P, A, tb, nt, tg, lane, xi, fi, xc, matc, cand, valid,
-                                                          self0, cut2_0, cstride32, maxnbr, nn, mind);
+                const int nb = min(NB, (ncand - kb + 63) >> 6);
+                for (int t = 0; t < nt; t++)
+                    nbr_target_block<D, ONEMAT, MIND, NB>(P, A, t, tg, lane, kb, nb, ncand, self0 + t - kb, xi, fi, cv,
+                                                      xcv, matcv, cut2_0, cstride32, maxnbr, nn, mind);
             }
             if (tv) {
                 if (nn > P.maxnbr) {
@@ -1864,6 +1861,12 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots)
 {
+    if ((unsigned long long)P.Ncap * (unsigned long long)nbr_round(P.maxnbr) > 0xffffffffull) {
+        // the cell kernel's list offsets are 32-bit
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A,
+                                              shift));
+        return;
+    }
     const dim3 grid(cell_blocks(P.N, cell_slots)), blk(kBlock);
     if (P.nmat == 1 && !P.want_mindist) {
         DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true, false>), grid, blk, 0, st, P, A, shift));

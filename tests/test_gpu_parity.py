@@ -154,6 +154,77 @@ def test_dense_cells(hfac, steps):
     compare_state(g, o, RTOL, exact=True, what=f"dense cells h={hfac}dx after {steps} steps")
 
 
+def _two_material_dam(tcrit_opt):
+    """The small dam break with the fluid's far half (y) given a second
+    material whose h is 1.25 times the first: pairs of unlike particles use
+    the mean h, so the search has three cut-offs."""
+    import dataclasses
+    deck = scenes.dam_break(nx_fluid=10, ny=8, nz_fluid=10, nx_tank=20, nz_tank=14, maxnbr=200)
+    m1 = deck.materials[0]
+    dx3 = deck.meta["dx"] ** 3
+    second = np.zeros(deck.np, bool)
+    second[:deck.n_fluid] = deck.x[:deck.n_fluid, 1] > 0.5 * deck.x[:deck.n_fluid, 1].max()
+    n2 = int(second.sum())
+    m2 = dataclasses.replace(m1, h=1.25 * m1.h, mass=m1.rho * dx3 * n2)
+    deck.materials = [dataclasses.replace(m1, mass=m1.rho * dx3 * (deck.np - n2)), m2]
+    deck.mat_id = np.where(second, 2, 1).astype(np.int32)
+    deck.tcrit_opt = tcrit_opt
+    return deck
+
+
+def _with_tcrit(deck, tcrit_opt):
+    deck.tcrit_opt = tcrit_opt
+    return deck
+
+
+CELL_KERNEL_CASES = {
+    # critical step from the smallest neighbour distance (tcrit option 2)
+    "dam_break_mindist": lambda: _with_tcrit(small_scenes()["dam_break"](), 2),
+    "cylinder_mindist": lambda: _with_tcrit(small_scenes()["cylinder"](), 2),
+    # per-pair cut-offs
+    "dam_break_two_h": lambda: _two_material_dam(3),
+    "dam_break_two_h_mindist": lambda: _two_material_dam(2),
+}
+
+
+@pytest.mark.parametrize("name", list(CELL_KERNEL_CASES))
+def test_cell_kernel_variants(name):
+    """The neighbour search's template variants (one or several cut-offs, with
+    or without the minimum distance) against the oracle: lists identical,
+    then 30 fused steps (bit-identical without a rigid body, dt included)."""
+    deck = CELL_KERNEL_CASES[name]()
+    o = orc.Oracle(deck)
+    g = gpu_scene(deck)
+    nn_o, ids_o = o.neighbours()
+    nn_g, ids_g = g.neighbours()
+    assert np.array_equal(nn_o, nn_g)
+    for i in range(deck.np):
+        assert np.array_equal(ids_o[i, :nn_o[i]], ids_g[i, :nn_g[i]]), i
+    assert g.globals()["dt"] == o.globals()["dt"]
+    assert o.run(30) == g.Solution(30) == 30
+    compare_state(g, o, RTOL, exact=not deck.rigid_body, what=name)
+    go, gg = o.globals(), g.globals()
+    for k in ("dt", "critts"):
+        # critts: with tcrit option 2 the smallest mindist / (c + |v|)
+        assert abs(gg[k] - go[k]) <= RTOL * go[k], (k, gg[k], go[k])
+        if not deck.rigid_body:
+            assert gg[k] == go[k], (k, gg[k], go[k])
+
+
+def test_fp32c_two_materials():
+    """FP32c arm with two materials of different h (compact momentum records
+    with a mass per material) against the FP64 arm."""
+    from sph07_amd.native import Scene, ARM_FP32C, F_V
+    deck = _two_material_dam(3)
+    g64, g32 = Scene(deck), Scene(deck, ARM_FP32C)
+    assert g64.Solution(60) == g32.Solution(60) == 60
+    moving = deck.dispbc == 0
+    v64, v32 = g64.field(F_V)[moving], g32.field(F_V)[moving]
+    err = np.linalg.norm(v32 - v64) / np.linalg.norm(v64)
+    print(f"two materials: fp32c rel L2 velocity error {err:.3e}")
+    assert err <= 1e-5, err
+
+
 def test_neighbour_overflow_is_reported():
     """maxnbr too small: the reference's 'Neighbour list exceeded maximum.'"""
     from sph07_amd.native import SphError
