@@ -8,8 +8,7 @@
 // One thread per particle.  Kernels read the device control block and return
 // at once when a previous kernel of the step flagged an error or the stop
 // test fired, so a queued sequence of steps is a no-op after either.
-#include "sph_kernels.hpp"
-#include "sph_rec32.hpp"
+#include "sph_cells.hpp"
 
 #include <stdlib.h>
 
@@ -41,17 +40,6 @@ __device__ __forceinline__ void set_err(Ctl *ctl, int code, int who)
 // ---------------------------------------------------------------------------
 // wave / block primitives (64-lane wavefronts)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int wave_incl_scan(int v)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
 // Exclusive scan over the block; *total receives the block sum.
 __device__ __forceinline__ int block_excl_scan(int v, int *total)
 {
@@ -204,19 +192,6 @@ __device__ __forceinline__ void grid_setup_body(const Params &P, Ctl *ctl)
 
     ctl->nbucket = (int)ncode + 1;
     for (int d = 0; d < 3; d++) ctl->cmask[d] = msk[d];
-}
-
-// deposits the low bits of v on the set bits of m (cell coordinate -> code bits)
-__device__ __forceinline__ int code_dep(int v, unsigned int m)
-{
-    int r = 0;
-    while (m) {
-        const unsigned int low = m & (0u - m);
-        if (v & 1) r |= (int)low;
-        v >>= 1;
-        m ^= low;
-    }
-    return r;
 }
 
 __device__ __forceinline__ int cell_code(const Ctl *ctl, int i, int j, int k)
@@ -736,7 +711,6 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
 // list in candidate order, i.e. in the reference's order.  The target's count
 // stays wave-uniform across its chunks.
 // ---------------------------------------------------------------------------
-constexpr int kCandCap = 1024;        // staged candidates per wave
 constexpr int kCellWaves = kBlock / 64;
 // candidate chunks held in registers per pass: 4 for the common one-material
 // search, 2 where per-candidate materials or distances take registers too
@@ -816,7 +790,6 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
     const int j0 = (int)((long long)P.N * wave / nwave), j1 = (int)((long long)P.N * (wave + 1) / nwave);
     const int ncode = ctl->ncode;
     const unsigned int m0 = ctl->cmask[0], m1 = ctl->cmask[1], m2 = ctl->cmask[2];
-    const int gl1 = ctl->glim[1], gl2 = ctl->glim[2];
     const unsigned int cstride32 = (unsigned int)P.Ncap * kNbrChunk;   // distance between list chunks
     const unsigned int maxnbr = (unsigned int)P.maxnbr;
     int jn = j0;
@@ -827,57 +800,14 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         jn = __builtin_amdgcn_readfirstlane(A.cell_start[q + 1]);
         if (head < j0) continue;                            // cell owned by the previous wave
         const int bc[3] = {code_ext(q, m0), code_ext(q, m1), code_ext(q, m2)};
-        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            lo[d] = bc[d] - 1 > 0 ? bc[d] - 1 : 0;
-            hi[d] = bc[d] + 1 < ctl->glim[d] - 1 ? bc[d] + 1 : ctl->glim[d] - 1;
-        }
-        // Stage the candidate slots in traversal order.  One lane per stencil
-        // cell (x -> y -> z, z fastest) loads its images and slot range in a
-        // single round of loads; a wave scan gives each cell's offset in the
-        // staged list (images first, newest first, then the slots).
-        int sr = 0, ss = 0, sz = 0;
-        bool inst;
-        if (D == 3) {
-            inst = lane < 27;
-            sr = lane / 9 - 1; ss = (lane / 3) % 3 - 1; sz = lane % 3 - 1;
-        } else if (D == 2) {
-            inst = lane < 9;
-            sr = lane / 3 - 1; ss = lane % 3 - 1;
-        } else {
-            inst = lane < 3;
-            sr = lane - 1;
-        }
-        const int cr = bc[0] + sr, cs = bc[1] + ss, cz = bc[2] + sz;
-        inst = inst && cr >= 0 && cr < ctl->glim[0] && cs >= 0 && cs < gl1 && cz >= 0 && cz < gl2;
-        int ng = 0, g0 = 0, s0 = 0, len = 0;
-        if (inst) {
-            const int qc = code_dep(cr, m0) | code_dep(cs, m1) | code_dep(cz, m2);
-            s0 = A.cell_start[qc];
-            len = A.cell_start[qc + 1] - s0;
-            if (P.periodic) {
-                const int c = gl2 * (gl1 * cr + cs) + cz;
-                ng = A.gcount[c];
-                if (ng) g0 = A.gfirst[c];
-            }
-        }
-        const int cnt = ng + len;
-        const int incl = wave_incl_scan(cnt);
-        const int off = incl - cnt;
-        int ncand = __builtin_amdgcn_readlane(incl, 63);
-        int own = __builtin_amdgcn_readlane(off + ng, D == 3 ? 13 : (D == 2 ? 4 : 1));
+        int own;
+        int ncand = stage_cell<D>(P, A, ctl, bc, lane, stage, own);
         const int c0 = head, c1 = jn;
         if (ncand > kCandCap) {
             // left to k_nbr_ovf (per-particle walk)
             if (lane == 0) A.ovf[atomicAdd(&ctl->novf, 1)] = q;
             continue;
         }
-        for (int u = 0; u < ng; u++) stage[off + u] = P.N + g0 + ng - 1 - u;
-        for (int u = 0; u < len; u++) stage[off + ng + u] = s0 + u;
-        // staged slots are read back by other lanes of the same wave (LDS
-        // operations of one wave complete in order)
-        __builtin_amdgcn_wave_barrier();
         if (kNbrBoxFilter) {
             // Drop the candidates farther than the largest cut-off from the
             // cell's box (parts of the edge and corner cells): no target of

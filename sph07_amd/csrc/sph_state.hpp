@@ -27,6 +27,8 @@ constexpr int kBlock = 256;
 // so a particle's consecutive entries share one 32-byte sector (its writes
 // merge in L2) and a wave reads a chunk of 64 lists in whole lines.
 constexpr int kNbrChunk = 8;
+// candidates one wave stages for a cell of the cell-centric neighbour search
+constexpr int kCandCap = 1024;
 constexpr int kBboxSlots = 64, kBboxStride = 16, kCritSlot = 8;
 inline __host__ __device__ size_t nbr_slot(int k, int i, int ncap)
 {

@@ -225,6 +225,28 @@ def test_fp32c_two_materials():
     assert err <= 1e-5, err
 
 
+@pytest.mark.parametrize("hfac,steps", [(3.0, 40), (4.5, 30), (6.0, 20)])
+def test_fp32c_dense_cells(hfac, steps):
+    """The FP32c sweeps over long lists (h = 3 dx: ~110 neighbours), lists
+    from the cell kernel's two target groups per cell (h = 4.5 dx) and from
+    the per-particle walk of oversized stencils (h = 6 dx): against the FP64
+    arm, neighbour counts shared."""
+    from sph07_amd.native import Scene, ARM_FP32C, F_V, F_RHO
+    deck = scenes.cavity(n=24, hfac=hfac, maxnbr=700)
+    g64, g32 = Scene(deck), Scene(deck, ARM_FP32C)
+    assert g64.Solution(steps) == g32.Solution(steps) == steps
+    moving = deck.dispbc == 0
+    v64, v32 = g64.field(F_V)[moving], g32.field(F_V)[moving]
+    err = np.linalg.norm(v32 - v64) / np.linalg.norm(v64)
+    print(f"h = {hfac} dx: fp32c rel L2 velocity error {err:.3e}")
+    assert err <= 1e-5, err
+    r64, r32 = g64.field(F_RHO), g32.field(F_RHO)
+    assert np.abs(r32 - r64).max() / np.abs(r64).max() <= 1e-6
+    nn64, _ = g64.neighbours()
+    nn32, _ = g32.neighbours()
+    assert np.array_equal(nn64, nn32)
+
+
 def test_neighbour_overflow_is_reported():
     """maxnbr too small: the reference's 'Neighbour list exceeded maximum.'"""
     from sph07_amd.native import SphError
