@@ -31,15 +31,26 @@ __device__ __forceinline__ int code_dep(int v, unsigned int m)
     return r;
 }
 
-// Stages the candidate slots of the stencil of the cell at coordinates bc in
-// stage[] in the traversal order of the reference's walk: cells x -> y -> z,
-// in each cell its periodic images newest first, then its slots.  One lane
-// per stencil cell loads the cell's range in a single round of loads; a wave
-// scan gives each cell's offset.  Returns the candidate count; above kCandCap
-// nothing is staged.  own: staged index of the cell's own first slot.
+// Code bits of axis m of the cell with code q stepped by -1, 0, +1 along
+// that axis (Morton increment and decrement on the axis's bits; the caller
+// keeps the coordinate inside the grid).
+__device__ __forceinline__ unsigned int code_step(unsigned int q, unsigned int m, int step)
+{
+    const unsigned int c = q & m;
+    const unsigned int dn = (c - 1u) & m, up = ((c | ~m) + 1u) & m;
+    return step < 0 ? dn : (step > 0 ? up : c);
+}
+
+// Stages the candidate slots of the stencil of the cell with code q (at
+// coordinates bc) in stage[] in the traversal order of the reference's walk:
+// cells x -> y -> z, in each cell its periodic images newest first, then its
+// slots.  One lane per stencil cell loads the cell's range in a single round
+// of loads; a wave scan gives each cell's offset.  Returns the candidate
+// count; above kCandCap nothing is staged.  own: staged index of the cell's
+// own first slot.
 template <int D>
-__device__ __forceinline__ int stage_cell(const Params &P, const Arrays &A, const Ctl *ctl, const int (&bc)[3],
-                                          int lane, int *stage, int &own)
+__device__ __forceinline__ int stage_cell(const Params &P, const Arrays &A, const Ctl *ctl, int q,
+                                          const int (&bc)[3], int lane, int *stage, int &own)
 {
     const unsigned int m0 = ctl->cmask[0], m1 = ctl->cmask[1], m2 = ctl->cmask[2];
     const int gl1 = ctl->glim[1], gl2 = ctl->glim[2];
@@ -59,7 +70,8 @@ __device__ __forceinline__ int stage_cell(const Params &P, const Arrays &A, cons
     inst = inst && cr >= 0 && cr < ctl->glim[0] && cs >= 0 && cs < gl1 && cz >= 0 && cz < gl2;
     int ng = 0, g0 = 0, s0 = 0, len = 0;
     if (inst) {
-        const int qc = code_dep(cr, m0) | code_dep(cs, m1) | code_dep(cz, m2);
+        const unsigned int qu = (unsigned int)q;
+        const int qc = (int)(code_step(qu, m0, sr) | code_step(qu, m1, ss) | code_step(qu, m2, sz));
         s0 = A.cell_start[qc];
         len = A.cell_start[qc + 1] - s0;
         if (P.periodic) {

@@ -801,7 +801,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         if (head < j0) continue;                            // cell owned by the previous wave
         const int bc[3] = {code_ext(q, m0), code_ext(q, m1), code_ext(q, m2)};
         int own;
-        int ncand = stage_cell<D>(P, A, ctl, bc, lane, stage, own);
+        int ncand = stage_cell<D>(P, A, ctl, q, bc, lane, stage, own);
         const int c0 = head, c1 = jn;
         if (ncand > kCandCap) {
             // left to k_nbr_ovf (per-particle walk)
