@@ -36,6 +36,36 @@ def test_kuglica_deck_two_colours():
     assert d.materials[1].surface_tension == 2e-07
 
 
+# the reference's decks kept as parity fixtures: what their cards state
+# (ndim, particles, materials, surface tension, rigid body, periodic boundary card, EOS)
+FIXTURE_DECKS = {
+    "stenosis.mcm": (2, 1891, 1, 0, 0, 1, [28]),
+    "rbcell.mcm": (2, 1891, 2, 1, 0, 1, [29, 29]),
+    "cylinders.mcm.gz": (2, 9044, 1, 0, 0, 0, [28]),
+    "flowpast.mcm.gz": (2, 10201, 1, 0, 0, 1, [29]),
+    "decuzzi.mcm.gz": (2, 18411, 2, 0, 1, 1, [29, 29]),
+    "rbcell3d.mcm.gz": (3, 43005, 2, 1, 0, 1, [29, 29]),
+}
+
+
+@pytest.mark.parametrize("name", list(FIXTURE_DECKS))
+def test_fixture_decks_parse(name, tmp_path):
+    import gzip
+    path = os.path.join(GOLD, name)
+    if name.endswith(".gz"):
+        out = tmp_path / name[:-3]
+        with gzip.open(path, "rb") as f:
+            out.write_bytes(f.read())
+        path = str(out)
+    d = read_mcm(path)
+    nd, n, nmat, st, rb, bnd, eos = FIXTURE_DECKS[name]
+    assert (d.ndim, d.np, len(d.materials), d.surface_tension, d.rigid_body, d.boundary) == (nd, n, nmat, st, rb,
+                                                                                             bnd)
+    assert [m.eos for m in d.materials] == eos
+    assert d.disctype == 2 and d.tcrit_opt == 3
+    assert set(np.unique(d.mat_id)) == set(range(1, nmat + 1))
+
+
 @pytest.mark.parametrize("gen", ["poiseuille", "cavity", "droplet", "cylinder"])
 def test_write_read_round_trip(tmp_path, gen):
     kw = {"poiseuille": dict(nx=12, ny=12), "cavity": dict(n=10), "droplet": dict(n_box=16),
