@@ -152,6 +152,40 @@ int  sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr);
 /* Blocks until all work queued on the context's stream has finished. */
 int  sph_synchronize(void *ctx);
 
+/* --- MNeighbourVars surface (MNeighbourVars.h:13-28): the grid and the
+   linked lists of the last MSolution::Neighbours / GhostSetup, as the
+   reference holds them.  Ids are the reference's: real particles by 1-based
+   deck number, periodic image g as -g, SPH_LINKEDLISTEND ends a list. --- */
+#define SPH_LINKEDLISTEND (-1000000)   /* global.h:13 */
+/* sph_gridmin, sph_gridsize, sph_gridlim after InitGrid (MNeighbourVars.cpp:18-56);
+   any pointer may be NULL. */
+int  sph_grid_info(void *ctx, double *gridmin, double *gridsize, int32_t *gridlim);
+/* MNeighbourVars::GetGridValue (MNeighbourVars.cpp:70-73): the head of cell
+   (i,j,k)'s list: its newest image, else its highest-numbered real particle,
+   else SPH_LINKEDLISTEND.  SPH_ERR_OUT_OF_GRID outside the grid. */
+int  sph_get_grid_value(void *ctx, int i, int j, int k, int32_t *value);
+/* The whole sph_llgrid (ncell >= gridlim[0]*gridlim[1]*gridlim[2] entries,
+   index gridlim[2]*(gridlim[1]*i+j)+k), the llpointer of every real particle
+   (real_next[np], indexed by 0-based deck number; MSolution.cpp:135-144) and
+   of every image (ghost_next[ngp]) with the image's source particle
+   (ghost_par[ngp], MGhostParticleData.cpp:16,30-31).  Any output may be NULL
+   (ghost_next and ghost_par together); ngp is the capacity of the image arrays. */
+int  sph_get_linked_list(void *ctx, int32_t *llgrid, long long ncell, int32_t *real_next,
+                         int32_t *ghost_next, int32_t *ghost_par, int ngp);
+/* MNeighbourVars::GetBoxCoords (MNeighbourVars.cpp:83-89): cell coordinates
+   of x[3] on the current grid, by truncation, unclamped. */
+int  sph_get_box_coords(void *ctx, const double *x, int32_t *boxcrd);
+/* MNeighbourVars::GetLoopLimits (MNeighbourVars.cpp:91-103): looplim[0][m]
+   and looplim[1][m] (row-major 2x3) clamped to the grid, 0 beyond ndim. */
+int  sph_get_loop_limits(void *ctx, int ndim, const int32_t *boxcrd, int32_t *looplim);
+
+/* --- MKernel surface: MKernelBSpline::Kernel and ::GradW, the static forms
+   (MKernelBSpline.cpp:98-187; MKernel.h:12-13), evaluated on the device for
+   n pairs.  xi, xj: n*3 coordinates (host memory); outputs w[n], dwdx[n*3]
+   (0 beyond ndim), dwdr[n]; any output may be NULL.  ndim 1-3, havg > 0. --- */
+int  sph_kernel_eval(int ndim, double havg, int n, const double *xi, const double *xj,
+                     double *w, double *dwdx, double *dwdr);
+
 /* --- batched scenes (no reference counterpart: the reference runs one scene
    per process, MSolution.h:16-22 singletons).  Many independent scenes of one
    ndim and arm are advanced together: every step is one sequence of kernel

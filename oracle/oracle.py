@@ -59,6 +59,8 @@ def lib():
         L.orc_kernel_w.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.orc_kernel_gradw.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_int, ctypes.c_double]
+        L.orc_kernel_dwdr.restype = ctypes.c_double
+        L.orc_kernel_dwdr.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -116,6 +118,28 @@ class Oracle:
         self._check(lib().orc_get_field(self._h, f, out.ctypes.data))
         return out
 
+    def linked_list(self) -> dict:
+        """Grid and linked lists of the last SetupLinkedList + GhostSetup."""
+        L = lib()
+        L.orc_get_linked_list.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_longlong] + \
+            [ctypes.c_void_p] * 3 + [ctypes.c_int]
+        gmin, gsize, glim = np.zeros(3), np.zeros(3), np.zeros(3, np.int32)
+        L.orc_get_linked_list(self._h, gmin.ctypes.data, gsize.ctypes.data, glim.ctypes.data,
+                              None, 0, None, None, None, 0)
+        ngp = L.orc_ngp(self._h)
+        ncell = int(np.prod(glim.astype(np.int64)))
+        grid = np.zeros(ncell, np.int32)
+        real_next = np.zeros(self.np, np.int32)
+        gnext = np.zeros(max(ngp, 1), np.int32)
+        gpar = np.zeros(max(ngp, 1), np.int32)
+        rc = L.orc_get_linked_list(self._h, None, None, None, grid.ctypes.data, ncell,
+                                   real_next.ctypes.data, gnext.ctypes.data, gpar.ctypes.data,
+                                   max(ngp, 1))
+        assert rc == 0
+        return {"gridmin": gmin, "gridsize": gsize, "gridlim": glim,
+                "llgrid": grid.reshape(tuple(int(g) for g in glim)), "real_next": real_next,
+                "ghost_next": gnext[:ngp], "ghost_par": gpar[:ngp]}
+
     def neighbours(self):
         n = self.np
         nn = np.zeros(n, np.int32)
@@ -136,6 +160,14 @@ def kernel_gradw(xi, xj, ndim, havg) -> np.ndarray:
     out = np.zeros(3)
     lib().orc_kernel_gradw(out.ctypes.data, xi.ctypes.data, xj.ctypes.data, ndim, havg)
     return out
+
+
+def kernel_dwdr(xi, xj, ndim, havg) -> float:
+    """dWdr of MKernelBSpline::GradW (static): the radial derivative dW/dr
+    (0 at r = 0)."""
+    xi = np.ascontiguousarray(xi, np.float64)
+    xj = np.ascontiguousarray(xj, np.float64)
+    return lib().orc_kernel_dwdr(xi.ctypes.data, xj.ctypes.data, ndim, havg)
 
 
 def threads() -> int:

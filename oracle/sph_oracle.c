@@ -101,7 +101,17 @@ static int rigid_p(const orc_scene *s, int mat)
 }
 
 /* ---- kernel: MKernelBSpline::GradW (static), MKernelBSpline.cpp:137-187 ---- */
+static void orc_gradw_r(double dwdx[3], double *dwdr, const double *xi, const double *xj, int ndim,
+                        double havg);
 static void orc_gradw(double dwdx[3], const double *xi, const double *xj, int ndim, double havg)
+{
+    double dwdr;
+    orc_gradw_r(dwdx, &dwdr, xi, xj, ndim, havg);
+}
+
+/* the full static GradW: dWdx and dWdr (dW/havg, 0 at r = 0, :174-185) */
+static void orc_gradw_r(double dwdx[3], double *dwdr, const double *xi, const double *xj, int ndim,
+                        double havg)
 {
     double const1 = 0.0, const2, xij[3], z, rad, dW;
     int n;
@@ -126,6 +136,7 @@ static void orc_gradw(double dwdx[3], const double *xi, const double *xj, int nd
         dW = 0.0;
     for (n = 0; n < ndim; n++)
         dwdx[n] = (rad == 0.0) ? 0.0 : dW * xij[n] / (rad * havg);
+    *dwdr = (rad == 0.0) ? 0.0 : dW / havg;
 }
 
 /* MKernelBSpline::Kernel (static), MKernelBSpline.cpp:98-135 */
@@ -158,6 +169,12 @@ double orc_kernel_w(const double *xi, const double *xj, int ndim, double havg)
 void orc_kernel_gradw(double *dwdx, const double *xi, const double *xj, int ndim, double havg)
 {
     orc_gradw(dwdx, xi, xj, ndim, havg);
+}
+double orc_kernel_dwdr(const double *xi, const double *xj, int ndim, double havg)
+{
+    double dwdx[3], dwdr;
+    orc_gradw_r(dwdx, &dwdr, xi, xj, ndim, havg);
+    return dwdr;
 }
 
 /* ---- neighbour record access: id>0 real, id<0 ghost (MSolution.cpp:187-188) ---- */
@@ -986,6 +1003,32 @@ int orc_get_neighbours(orc_scene *s, int32_t *nnbr, int64_t *canon, int maxnbr)
             }
             canon[(size_t)(i - 1) * maxnbr + k] = cid;
         }
+    }
+    return 0;
+}
+
+/* The linked-list state of the last SetupLinkedList + GhostSetup
+   (MNeighbourVars.cpp:57-80, MSolution.cpp:135-144, MGhostParticleData.cpp:16,30-31):
+   grid geometry, sph_llgrid, llpointer of the real particles (by 0-based
+   deck number) and of the images, and each image's source particle (1-based).
+   Any output may be NULL.  Returns 1 if an array is too small. */
+int orc_get_linked_list(orc_scene *s, double *gmin, double *gsize, int32_t *glim, int32_t *llgrid,
+                        long long ncell, int32_t *real_next, int32_t *ghost_next, int32_t *ghost_par, int ngp)
+{
+    const long long nc = (long long)s->glim[0] * s->glim[1] * s->glim[2];
+    for (int m = 0; m < 3; m++) {
+        if (gmin) gmin[m] = s->gmin[m];
+        if (gsize) gsize[m] = s->gsize[m];
+        if (glim) glim[m] = s->glim[m];
+    }
+    if ((llgrid && ncell < nc) || ((ghost_next || ghost_par) && ngp < s->ngp)) return 1;
+    if (llgrid)
+        for (long long c = 0; c < nc; c++) llgrid[c] = s->grid[c];
+    if (real_next)
+        for (int i = 1; i <= s->np; i++) real_next[i - 1] = s->llp[i];
+    for (int g = 1; g <= s->ngp; g++) {
+        if (ghost_next) ghost_next[g - 1] = s->g[g].llp;
+        if (ghost_par) ghost_par[g - 1] = s->g[g].par;
     }
     return 0;
 }

@@ -97,6 +97,19 @@ int         orc_get_field(orc_scene *s, int field, double *out);
    j (0-based); a periodic image of real j shifted by (sx,sy,sz) in {-1,0,1}^3
    is encoded as -(1 + j*27 + (sx+1)*9 + (sy+1)*3 + (sz+1)). */
 int         orc_get_neighbours(orc_scene *s, int32_t *nnbr, int64_t *canon, int maxnbr);
+/* MKernelBSpline statics (MKernelBSpline.cpp:98-187): W, grad W (3 entries,
+   0 beyond ndim) and dW/dr of one pair. */
+double      orc_kernel_w(const double *xi, const double *xj, int ndim, double havg);
+void        orc_kernel_gradw(double *dwdx, const double *xi, const double *xj, int ndim, double havg);
+double      orc_kernel_dwdr(const double *xi, const double *xj, int ndim, double havg);
+/* Linked-list state of the last SetupLinkedList + GhostSetup: grid geometry,
+   sph_llgrid (ncell >= glim product), llpointer of the real particles (np, by
+   0-based deck number; reference ids: 1-based reals, -g images, -1000000
+   end) and of the images (ngp), each image's 1-based source particle.
+   Any output may be NULL; returns 1 if an array is too small. */
+int         orc_get_linked_list(orc_scene *s, double *gmin, double *gsize, int32_t *glim,
+                                int32_t *llgrid, long long ncell, int32_t *real_next,
+                                int32_t *ghost_next, int32_t *ghost_par, int ngp);
 
 #ifdef __cplusplus
 }

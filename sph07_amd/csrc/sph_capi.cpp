@@ -24,6 +24,21 @@ namespace {
 
 constexpr double kPI = 3.141592654;   // global.h:12
 
+// B-spline normalisations for the averaged smoothing length havg, in the
+// reference's operation order (MKernelBSpline.cpp:145-156); eps2 = 0.01 havg^2
+// as in the Morris term (MSolution.cpp:592).  cut2 is left to the caller.
+PairConst kernel_const(int ndim, double havg)
+{
+    PairConst pc{};
+    pc.havg = havg;
+    if (ndim == 3) pc.const1 = 1.0 / (kPI * (havg * havg * havg));
+    else if (ndim == 2) pc.const1 = 10.0 / (7.0 * kPI * (havg * havg));
+    else pc.const1 = 2.0 / (3.0 * havg);
+    pc.const2 = 0.25 * pc.const1;
+    pc.eps2 = 0.01 * (havg * havg);
+    return pc;
+}
+
 // Kernel groups timed with HIP events on the context's stream.
 enum KId { K_BIN, K_IMAGES, K_NBR, K_STRAIN, K_NORMALS, K_ADVANCE, K_MOMENTUM, K_INTEGRATE,
            K_FINISH, K_COUNT };
@@ -507,13 +522,7 @@ int build_context(Context *c, const sph_scene_desc *d)
     for (int a = 1; a <= d->nummat; a++)
         for (int b = 1; b <= d->nummat; b++) {
             PairConst &pc = P.pc[a][b];
-            const double havg = 0.5 * (P.mat[a].h + P.mat[b].h);
-            pc.havg = havg;
-            if (D == 3) pc.const1 = 1.0 / (kPI * (havg * havg * havg));
-            else if (D == 2) pc.const1 = 10.0 / (7.0 * kPI * (havg * havg));
-            else pc.const1 = 2.0 / (3.0 * havg);
-            pc.const2 = 0.25 * pc.const1;
-            pc.eps2 = 0.01 * (havg * havg);
+            pc = kernel_const(D, 0.5 * (P.mat[a].h + P.mat[b].h));
             const double hav = 2.0 * (0.5 * (P.mat[a].h + P.mat[b].h));
             pc.cut2 = hav * hav;
         }
@@ -1199,6 +1208,138 @@ int sph_get_neighbours(void *ctx, int32_t *nnbr, int64_t *canon, int maxnbr)
     for (int i = 0; i < N; i++)
         for (int k = 0; k < nnbr[i] && k < mn; k++) canon[(size_t)i * maxnbr + k] = tmp[(size_t)i * mn + k];
     return SPH_OK;
+}
+
+// ---- MNeighbourVars surface ------------------------------------------------
+
+int sph_grid_info(void *ctx, double *gridmin, double *gridsize, int32_t *gridlim)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return SPH_ERROR;
+    if (sync_ctl(c)) return SPH_ERR_HIP;
+    for (int m = 0; m < 3; m++) {
+        if (gridmin) gridmin[m] = c->hctl->gmin[m];
+        if (gridsize) gridsize[m] = c->hctl->gsize[m];
+        if (gridlim) gridlim[m] = c->hctl->glim[m];
+    }
+    return SPH_OK;
+}
+
+int sph_get_linked_list(void *ctx, int32_t *llgrid, long long ncell, int32_t *real_next, int32_t *ghost_next,
+                        int32_t *ghost_par, int ngp)
+{
+    Context *c = ctx_of(ctx);
+    if (!c) return SPH_ERROR;
+    if (sync_ctl(c)) return SPH_ERR_HIP;
+    const Ctl &h = *c->hctl;
+    const long long n_cell = (long long)h.glim[0] * h.glim[1] * h.glim[2];
+    const int n_img = c->P.periodic ? h.ngp : 0;
+    if (llgrid && ncell < n_cell) {
+        c->err = "sph_get_linked_list: llgrid holds fewer entries than the grid has cells";
+        return SPH_ERROR;
+    }
+    if ((ghost_next || ghost_par) && ngp < n_img) {
+        c->err = "sph_get_linked_list: image arrays hold fewer entries than there are images";
+        return SPH_ERROR;
+    }
+    if ((ghost_next == nullptr) != (ghost_par == nullptr)) {
+        c->err = "sph_get_linked_list: ghost_next and ghost_par go together";
+        return SPH_ERROR;
+    }
+    const int N = c->N;
+    const size_t bytes = (llgrid ? (size_t)n_cell : 0) + (real_next ? (size_t)N : 0) +
+                         (ghost_next ? 2 * (size_t)n_img : 0);
+    int *buf = nullptr;
+    HIPCHK(c, hipMalloc((void **)&buf, (bytes ? bytes : 1) * sizeof(int)));
+    int *d_grid = llgrid ? buf : nullptr;
+    int *d_real = real_next ? buf + (llgrid ? n_cell : 0) : nullptr;
+    int *d_gnext = ghost_next ? buf + (llgrid ? n_cell : 0) + (real_next ? N : 0) : nullptr;
+    int *d_gpar = ghost_next ? d_gnext + n_img : nullptr;
+    launch_linked_list(c->P, cur(c), d_grid, d_real, d_gnext, d_gpar, n_cell, c->st);
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess && llgrid) e = hipMemcpy(llgrid, d_grid, (size_t)n_cell * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && real_next) e = hipMemcpy(real_next, d_real, (size_t)N * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && ghost_next && n_img) {
+        e = hipMemcpy(ghost_next, d_gnext, (size_t)n_img * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(ghost_par, d_gpar, (size_t)n_img * 4, hipMemcpyDeviceToHost);
+    }
+    hipFree(buf);
+    HIPCHK(c, e);
+    return SPH_OK;
+}
+
+int sph_get_grid_value(void *ctx, int i, int j, int k, int32_t *value)
+{
+    Context *c = ctx_of(ctx);
+    if (!c || !value) return SPH_ERROR;
+    if (sync_ctl(c)) return SPH_ERR_HIP;
+    const Ctl &h = *c->hctl;
+    if (i < 0 || j < 0 || k < 0 || i >= h.glim[0] || j >= h.glim[1] || k >= h.glim[2]) {
+        c->err = "GetGridValue: cell outside the grid";
+        return SPH_ERR_OUT_OF_GRID;
+    }
+    std::vector<int32_t> grid((size_t)h.glim[0] * h.glim[1] * h.glim[2]);
+    const int rc = sph_get_linked_list(ctx, grid.data(), (long long)grid.size(), nullptr, nullptr, nullptr, 0);
+    if (rc) return rc;
+    *value = grid[(size_t)h.glim[2] * ((size_t)h.glim[1] * i + j) + k];
+    return SPH_OK;
+}
+
+int sph_get_box_coords(void *ctx, const double *x, int32_t *boxcrd)
+{
+    Context *c = ctx_of(ctx);
+    if (!c || !x || !boxcrd) return SPH_ERROR;
+    if (sync_ctl(c)) return SPH_ERR_HIP;
+    // RealVector temp = x - gridmin; boxcrd(i) = gridsize(i) * temp(i), truncated
+    for (int m = 0; m < 3; m++) boxcrd[m] = (int32_t)(c->hctl->gsize[m] * (x[m] - c->hctl->gmin[m]));
+    return SPH_OK;
+}
+
+int sph_get_loop_limits(void *ctx, int ndim, const int32_t *boxcrd, int32_t *looplim)
+{
+    Context *c = ctx_of(ctx);
+    if (!c || !boxcrd || !looplim) return SPH_ERROR;
+    if (sync_ctl(c)) return SPH_ERR_HIP;
+    for (int m = 0; m < 3; m++) {
+        looplim[m] = std::max(boxcrd[m] - 1, 0);
+        looplim[3 + m] = std::min(boxcrd[m] + 1, c->hctl->glim[m] - 1);
+    }
+    for (int m = ndim; m < 3; m++) looplim[m] = looplim[3 + m] = 0;
+    return SPH_OK;
+}
+
+// ---- MKernel surface -------------------------------------------------------
+
+int sph_kernel_eval(int ndim, double havg, int n, const double *xi, const double *xj, double *w, double *dwdx,
+                    double *dwdr)
+{
+    if (ndim < 1 || ndim > 3 || !(havg > 0.0) || n < 0 || (n > 0 && (!xi || !xj))) return SPH_ERROR;
+    if (n == 0) return SPH_OK;
+    const size_t v3 = 3 * (size_t)n * sizeof(double), v1 = (size_t)n * sizeof(double);
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, 2 * v3 + v3 + 2 * v1) != hipSuccess) return SPH_ERR_HIP;
+    double *dxi = d, *dxj = d + 3 * (size_t)n, *ddw = d + 6 * (size_t)n, *dw = d + 9 * (size_t)n,
+           *ddr = d + 10 * (size_t)n;
+    const PairConst pc = kernel_const(ndim, havg);
+    // a private stream: the call neither waits for nor delays the scenes' streams
+    hipStream_t st = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemcpyAsync(dxi, xi, v3, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dxj, xj, v3, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        launch_kernel_eval(ndim, pc, n, dxi, dxj, w ? dw : nullptr, dwdx ? ddw : nullptr, dwdr ? ddr : nullptr, st);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && w) e = hipMemcpyAsync(w, dw, v1, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && dwdx) e = hipMemcpyAsync(dwdx, ddw, v3, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && dwdr) e = hipMemcpyAsync(dwdr, ddr, v1, hipMemcpyDeviceToHost, st);
+    if (st) {
+        const hipError_t es = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = es;
+        hipStreamDestroy(st);
+    }
+    hipFree(d);
+    return e == hipSuccess ? SPH_OK : SPH_ERR_HIP;
 }
 
 }  // extern "C"

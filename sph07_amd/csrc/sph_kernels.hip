@@ -1870,6 +1870,113 @@ void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, 
     hipLaunchKernelGGL(k_canon, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, out, maxnbr);
 }
 
+// ---------------------------------------------------------------------------
+// The reference's linked-list state, rebuilt from the cell-sorted storage of
+// the last binning: sph_llgrid (MNeighbourVars.cpp:57-66, 70-80), the
+// llpointer of every real particle (MSolution.cpp:135-144) and of every
+// periodic image (MGhostParticleData.cpp:30-31).  Ids are the reference's:
+// real particles 1-based deck numbers, image g as -g, LINKEDLISTEND at the
+// end of a list.  A cell's list is its images newest first, then its real
+// particles by descending number; the images of one cell are consecutive in
+// creation order (one source cell feeds them).
+// ---------------------------------------------------------------------------
+constexpr int kLinkedListEnd = -1000000;   // global.h:13
+
+__global__ __launch_bounds__(kBlock) void k_linked_list(Params P, Arrays A, int *llgrid, int *real_next,
+                                                        int *ghost_next, int *ghost_par)
+{
+    const Ctl *ctl = A.ctl;
+    const long long gl1 = ctl->glim[1], gl2 = ctl->glim[2];
+    const long long ncell = (long long)ctl->glim[0] * gl1 * gl2;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long long lin = t0; lin < ncell; lin += stride) {
+        const int k = (int)(lin % gl2), j = (int)((lin / gl2) % gl1), i = (int)(lin / (gl2 * gl1));
+        const int q = cell_code(ctl, i, j, k);
+        const int s0 = A.cell_start[q], s1 = A.cell_start[q + 1];
+        const int real_head = s0 < s1 ? A.orig[s0] + 1 : kLinkedListEnd;
+        int head = real_head;
+        const int cnt = P.periodic ? A.gcount[lin] : 0;
+        if (cnt > 0) {
+            const int g0 = A.gfirst[lin];          // image g0 + r has id -(g0 + r + 1)
+            head = -(g0 + cnt);
+            if (ghost_next)
+                for (int r = 0; r < cnt; r++) {
+                    ghost_next[g0 + r] = r > 0 ? -(g0 + r) : real_head;
+                    ghost_par[g0 + r] = A.orig[A.gsrc[g0 + r]] + 1;
+                }
+        }
+        if (llgrid) llgrid[lin] = head;
+    }
+    if (real_next)
+        for (long long s = t0; s < P.N; s += stride) {
+            const int q = A.occ[s];
+            int nx = kLinkedListEnd;                // inactive particles end their list at once
+            if (q < ctl->ncode && s + 1 < A.cell_start[q + 1]) nx = A.orig[s + 1] + 1;
+            real_next[A.orig[s]] = nx;
+        }
+}
+
+void launch_linked_list(const Params &P, const Arrays &A, int *llgrid, int *real_next, int *ghost_next,
+                        int *ghost_par, long long ncell, hipStream_t st)
+{
+    const long long items = ncell > P.N ? ncell : P.N;
+    long long nb = (items + kBlock - 1) / kBlock;
+    nb = nb < 1 ? 1 : (nb > 65536 ? 65536 : nb);
+    hipLaunchKernelGGL(k_linked_list, dim3((unsigned)nb), dim3(kBlock), 0, st, P, A, llgrid, real_next, ghost_next,
+                       ghost_par);
+}
+
+// ---------------------------------------------------------------------------
+// MKernelBSpline::Kernel and ::GradW, the static forms
+// (MKernelBSpline.cpp:98-187), for n pairs: xi, xj are n*3 coordinates, w n
+// values, dwdx n*3, dwdr n (any output may be null).  The normalisations come
+// in pc, computed on the host with the reference's operation order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_kernel_eval(int ndim, PairConst pc, int n, const double *xi,
+                                                        const double *xj, double *w, double *dwdx, double *dwdr)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double xij[3] = {0.0, 0.0, 0.0}, rad = 0.0;
+    for (int d = 0; d < ndim; d++) {
+        xij[d] = xi[3 * (size_t)p + d] - xj[3 * (size_t)p + d];
+        rad += xij[d] * xij[d];
+    }
+    rad = sqrt(rad);
+    const double z = rad / pc.havg;
+    if (w) {
+        double W;
+        if (z < 1.0)
+            W = pc.const1 * (((0.75 * z - 1.5) * z) * z + 1.0);
+        else if (z < 2.0)
+            W = pc.const2 * ((2.0 - z) * (2.0 - z) * (2.0 - z));
+        else
+            W = 0.0;
+        w[p] = W;
+    }
+    if (dwdx || dwdr) {
+        double dW;
+        if (z < 1.0)
+            dW = pc.const1 * ((2.25 * z - 3.0) * z);
+        else if (z < 2.0)
+            dW = -3.0 * pc.const2 * ((2.0 - z) * (2.0 - z));
+        else
+            dW = 0.0;
+        if (dwdx)
+            for (int d = 0; d < 3; d++)
+                dwdx[3 * (size_t)p + d] = (d >= ndim || rad == 0.0) ? 0.0 : dW * xij[d] / (rad * pc.havg);
+        if (dwdr) dwdr[p] = rad == 0.0 ? 0.0 : dW / pc.havg;
+    }
+}
+
+void launch_kernel_eval(int ndim, const PairConst &pc, int n, const double *xi, const double *xj, double *w,
+                        double *dwdx, double *dwdr, hipStream_t st)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_kernel_eval, dim3(nblk(n)), dim3(kBlock), 0, st, ndim, pc, n, xi, xj, w, dwdx, dwdr);
+}
+
 // One batched step over the scenes of a BatchView (all of one ndim and arm).
 void launch_batch_step(const BatchView &B, hipStream_t st)
 {

@@ -91,6 +91,13 @@ void launch_rigid_reduce(const Params &P, const Arrays &A, int rigid_kin, hipStr
 void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st);
 void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st);
 void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st);
+// the reference's linked-list arrays (sph_llgrid, llpointer of reals and
+// images, image sources) from the last binning; any output may be null
+void launch_linked_list(const Params &P, const Arrays &A, int *llgrid, int *real_next, int *ghost_next,
+                        int *ghost_par, long long ncell, hipStream_t st);
+// MKernelBSpline::Kernel / GradW statics over n pairs (device buffers)
+void launch_kernel_eval(int ndim, const PairConst &pc, int n, const double *xi, const double *xj, double *w,
+                        double *dwdx, double *dwdr, hipStream_t st);
 
 // Device tables of one batched launch: per-scene parameter and array blocks,
 // the scene list, and block maps (scene, local block) over particles and

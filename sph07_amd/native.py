@@ -234,6 +234,85 @@ class Scene:
         self._chk(lib().sph_get_neighbours(self._h, nn.ctypes.data, canon.ctypes.data, self.maxnbr))
         return nn, canon
 
+    # --- MNeighbourVars surface (MNeighbourVars.h:13-28) ---
+    def grid_info(self) -> dict:
+        """sph_gridmin, sph_gridsize, sph_gridlim of the last InitGrid."""
+        gmin, gsize = np.zeros(3), np.zeros(3)
+        glim = np.zeros(3, np.int32)
+        self._chk(_surface().sph_grid_info(self._h, gmin.ctypes.data, gsize.ctypes.data,
+                                           glim.ctypes.data))
+        return {"gridmin": gmin, "gridsize": gsize, "gridlim": glim}
+
+    def linked_list(self) -> dict:
+        """The reference's linked lists: llgrid (gridlim[0] x gridlim[1] x
+        gridlim[2], cell heads), real_next (llpointer per deck particle),
+        ghost_next / ghost_par (per image).  Reference ids: 1-based reals, -g
+        for image g, LINKEDLISTEND at a list's end."""
+        glim = self.grid_info()["gridlim"]
+        ngp = self.globals()["ngp"]
+        ncell = int(np.prod(glim.astype(np.int64)))
+        grid = np.zeros(ncell, np.int32)
+        real_next = np.zeros(self.np, np.int32)
+        gnext = np.zeros(max(ngp, 1), np.int32)
+        gpar = np.zeros(max(ngp, 1), np.int32)
+        self._chk(_surface().sph_get_linked_list(self._h, grid.ctypes.data, ncell,
+                                                 real_next.ctypes.data, gnext.ctypes.data,
+                                                 gpar.ctypes.data, max(ngp, 1)))
+        return {"llgrid": grid.reshape(tuple(int(g) for g in glim)), "real_next": real_next,
+                "ghost_next": gnext[:ngp], "ghost_par": gpar[:ngp]}
+
+    def GetGridValue(self, i: int, j: int, k: int) -> int:
+        out = ctypes.c_int32(0)
+        self._chk(_surface().sph_get_grid_value(self._h, i, j, k, ctypes.byref(out)))
+        return out.value
+
+    def GetBoxCoords(self, x) -> np.ndarray:
+        xv = np.zeros(3)
+        xv[:len(x)] = x
+        out = np.zeros(3, np.int32)
+        self._chk(_surface().sph_get_box_coords(self._h, xv.ctypes.data, out.ctypes.data))
+        return out
+
+    def GetLoopLimits(self, ndim: int, boxcrd) -> np.ndarray:
+        b = np.ascontiguousarray(boxcrd, np.int32)
+        out = np.zeros((2, 3), np.int32)
+        self._chk(_surface().sph_get_loop_limits(self._h, ndim, b.ctypes.data, out.ctypes.data))
+        return out
+
+
+LINKEDLISTEND = -1000000   # global.h:13
+
+
+def _surface():
+    """Argument types of the MNeighbourVars / MKernel entry points."""
+    L = lib()
+    if not getattr(L, "_surface_typed", False):
+        vp, ip = ctypes.c_void_p, ctypes.c_int
+        L.sph_grid_info.argtypes = [vp, vp, vp, vp]
+        L.sph_get_linked_list.argtypes = [vp, vp, ctypes.c_longlong, vp, vp, vp, ip]
+        L.sph_get_grid_value.argtypes = [vp, ip, ip, ip, ctypes.POINTER(ctypes.c_int32)]
+        L.sph_get_box_coords.argtypes = [vp, vp, vp]
+        L.sph_get_loop_limits.argtypes = [vp, ip, vp, vp]
+        L.sph_kernel_eval.argtypes = [ip, ctypes.c_double, ip, vp, vp, vp, vp, vp]
+        L._surface_typed = True
+    return L
+
+
+def kernel_eval(ndim: int, havg: float, xi, xj):
+    """MKernelBSpline::Kernel and ::GradW (statics, MKernelBSpline.cpp:98-187)
+    evaluated on the GPU for every row pair of xi, xj (n x 3): returns
+    (W[n], dWdx[n, 3], dWdr[n])."""
+    xi = np.ascontiguousarray(xi, np.float64).reshape(-1, 3)
+    xj = np.ascontiguousarray(xj, np.float64).reshape(-1, 3)
+    assert xi.shape == xj.shape
+    n = xi.shape[0]
+    w, dwdx, dwdr = np.zeros(n), np.zeros((n, 3)), np.zeros(n)
+    rc = _surface().sph_kernel_eval(ndim, havg, n, xi.ctypes.data, xj.ctypes.data, w.ctypes.data,
+                                    dwdx.ctypes.data, dwdr.ctypes.data)
+    if rc != 0:
+        raise SphError(rc, "sph_kernel_eval failed")
+    return w, dwdx, dwdr
+
 
 def read_mcm(path: str) -> Deck:
     """Reads a .mcm deck with the library's reader (sph_deck_read) into a Deck."""

@@ -47,3 +47,31 @@ def test_no_cpu_fallback_in_product():
             if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
                 txt = open(os.path.join(root, f)).read()
                 assert not pat.search(txt), f
+
+
+def test_cpp_surface_header_compiles(tmp_path):
+    """include/sph07_amd.hpp (the reference's class names over the C ABI)
+    compiles as C++17 and links against the library; nothing is run."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "use_surface.cpp"
+    src.write_text(
+        '#include "sph07_amd.hpp"\n'
+        "int main() {\n"
+        "  sph07_amd::MSimulationInit init;\n"
+        "  sph07_amd::MSolution s(init.Context(), 2);\n"
+        "  sph07_amd::MKernelBSpline *k = s.GenerateKernel(1.0);\n"
+        "  int b[3] = {0, 0, 0}; int32_t lim[2][3];\n"
+        "  s.NeighbourVars().GetLoopLimits(2, b, lim);\n"
+        "  delete k;\n"
+        "  return 0;\n"
+        "}\n")
+    exe = tmp_path / "use_surface"
+    r = subprocess.run([gxx, "-std=c++17", "-I", os.path.join(root, "include"), str(src), "-o", str(exe),
+                        "-L", os.path.dirname(native.LIB_PATH), "-lsph07_amd"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -90,3 +90,27 @@ def test_gradient_direction():
     xj = np.array([0.5, 0.2, 0.0])
     g = orc.kernel_gradw(xi, xj, 2, 1.0)
     assert np.dot(g[:2], (xj - xi)[:2]) > 0
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_dwdr_is_radial_derivative(ndim):
+    """dWdr of GradW (MKernelBSpline.cpp:174-185) is dW/dr: central
+    differences of W along the separation, and |grad W| = |dW/dr|."""
+    rng = np.random.default_rng(2)
+    h = 0.8
+    xi = np.zeros(3)
+    for _ in range(100):
+        r = rng.uniform(0.0, 2.2 * h)
+        if r < 1e-3 or abs(r / h - 1) < 1e-3 or abs(r / h - 2) < 1e-3:
+            continue
+        u = np.zeros(3)
+        u[:ndim] = rng.normal(size=ndim)
+        u /= np.linalg.norm(u)
+        xj = r * u
+        dr = orc.kernel_dwdr(xi, xj, ndim, h)
+        eps = 1e-6
+        fd = (orc.kernel_w(xi, (r + eps) * u, ndim, h) - orc.kernel_w(xi, (r - eps) * u, ndim, h)) / (2 * eps)
+        assert abs(dr - fd) <= 1e-6 * max(1.0, abs(fd)), (ndim, r / h, dr, fd)
+        g = orc.kernel_gradw(xi, xj, ndim, h)
+        assert abs(np.linalg.norm(g) - abs(dr)) <= 1e-12 * max(1.0, abs(dr))
+    assert orc.kernel_dwdr(xi, xi, ndim, h) == 0.0

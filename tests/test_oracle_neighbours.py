@@ -153,3 +153,80 @@ def test_neighbour_overflow():
     d = scenes.poiseuille(nx=10, ny=10, maxnbr=6)
     with pytest.raises(orc.OracleError, match="exceeded maximum"):
         orc.Oracle(d)
+
+
+def expected_lists(x, active, h, ndim, L=None):
+    """Linked lists a head-insertion binning leaves (MSolution.cpp:116-148,
+    811-918), recomputed with numpy: cell of every active real particle by
+    truncation, reals by descending number; with an x-periodic box of length
+    L, images of interior layer 1 go to layer lim-1 and those of layer lim-2
+    to layer 0, created walking each source cell's list."""
+    gs = 1.0 / (2.0 * h)
+    gsize = np.full(3, gs)
+    gmin = np.zeros(3)
+    glim = np.ones(3, np.int64)
+    for a in range(ndim):
+        if a == 0 and L is not None:
+            glim[0] = int(gs * L)
+            gsize[0] = glim[0] / L
+            gmin[0] = 0.0 - 1.0 / gsize[0]
+            glim[0] += 2
+        else:
+            gmin[a] = x[active, a].min() - 0.01 * h
+            glim[a] = int(gs * (x[active, a].max() - gmin[a])) + 1
+    bc = (gsize * (x - gmin)).astype(np.int64)
+    reals = {}
+    for i in np.nonzero(active)[0]:
+        reals.setdefault(tuple(bc[i]), []).append(i + 1)
+    for c in reals:
+        reals[c].sort(reverse=True)
+    images = {}
+    ng = 0
+    if L is not None:
+        # passes: min side (source layer 1), then max side (layer lim-2);
+        # source cells k -> j -> i ascending
+        for src_layer, dst_layer in ((1, glim[0] - 1), (glim[0] - 2, 0)):
+            for k in range(glim[2]):
+                for j in range(glim[1]):
+                    for p in reals.get((src_layer, j, k), []):
+                        ng += 1
+                        images.setdefault((dst_layer, j, k), []).append((ng, p))
+    return gmin, gsize, glim, reals, images, ng
+
+
+def walk(ll, cell):
+    head = int(ll["llgrid"][cell])
+    seq = []
+    while head != -1000000:
+        seq.append(head)
+        head = int(ll["real_next"][head - 1]) if head > 0 else int(ll["ghost_next"][-head - 1])
+    return seq
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_linked_list_export_matches_binning(periodic):
+    """The oracle's sph_llgrid / llpointer arrays are the lists a brute-force
+    binning gives: every active particle once, in its truncated cell, images
+    newest first, reals by descending number."""
+    x = jittered_box(n=16, jitter=0.25, seed=11)
+    h = 1.2
+    L = 16.0 if periodic else None
+    d = deck_from_points(x, h, boundary=1 if periodic else 0)
+    if periodic:
+        d.boundary_code[:, 0] = 2
+        d.boundary_x[0, 0], d.boundary_x[1, 0] = 0.0, L
+    o = orc.Oracle(d)
+    o.phase(orc.PH_NEIGHBOURS)
+    ll = o.linked_list()
+    xs = o.field(orc.F_X)
+    active = o.field(orc.F_ACTIVE) > 0
+    gmin, gsize, glim, reals, images, ng = expected_lists(xs, active, h, 2, L)
+    assert np.array_equal(ll["gridlim"], glim)
+    assert np.array_equal(ll["gridmin"], gmin) and np.array_equal(ll["gridsize"], gsize)
+    assert len(ll["ghost_next"]) == ng and (ng > 0) == periodic
+    for cell in np.ndindex(*ll["llgrid"].shape):
+        want = [-g for g, _ in reversed(images.get(cell, []))] + reals.get(cell, [])
+        assert walk(ll, cell) == want, cell
+    for cell, imgs in images.items():
+        for g, p in imgs:
+            assert ll["ghost_par"][g - 1] == p
