@@ -167,6 +167,10 @@ int alloc_particles(Context *c, int gcap)
         c->err = "too many particles and images for one context";
         return 1;
     }
+    if (c->arm == SPH_ARM_FP32C && ncap * 32 > 0xffffffffull) {   // 32-bit record offsets
+        c->err = "the FP32c arm takes at most 2^27 particles and images per context";
+        return 1;
+    }
     std::vector<void *> &L = c->part_allocs;
     P.gcap = gcap;
     P.Ncap = (int)ncap;
@@ -200,13 +204,11 @@ int alloc_particles(Context *c, int gcap)
     rc |= dalloc(c, L, &A.rmB, ncap);
     rc |= dalloc(c, L, &A.rmC, ncap);
     if (c->arm == SPH_ARM_FP32C) {
-        rc |= dalloc(c, L, &A.fsA, ncap);
-        rc |= dalloc(c, L, &A.fsB, ncap);
-        rc |= dalloc(c, L, &A.fmA, ncap);
-        rc |= dalloc(c, L, &A.fmB, ncap);
+        rc |= dalloc(c, L, &A.fs, 2 * ncap);
+        rc |= dalloc(c, L, &A.fm, 2 * ncap);
         rc |= dalloc(c, L, &A.fmC, ncap);
     } else {
-        A.fsA = A.fsB = A.fmA = A.fmB = nullptr;
+        A.fs = A.fm = nullptr;
         A.fmC = nullptr;
     }
     rc |= dalloc(c, L, &A.rho_new, ncap);
@@ -239,7 +241,7 @@ int alloc_particles(Context *c, int gcap)
         B.nn[d] = A.nn[d];
     }
     B.rsA = A.rsA; B.rsB = A.rsB; B.rmA = A.rmA; B.rmB = A.rmB; B.rmC = A.rmC;
-    B.fsA = A.fsA; B.fsB = A.fsB; B.fmA = A.fmA; B.fmB = A.fmB; B.fmC = A.fmC;
+    B.fs = A.fs; B.fm = A.fm; B.fmC = A.fmC;
     B.rho_new = A.rho_new; B.p = A.p; B.prho2 = A.prho2; B.vol = A.vol;
     B.trd = A.trd; B.mindist = A.mindist; B.critts_p = A.critts_p; B.rhoold = A.rhoold;
     B.Nf = A.Nf; B.kappa = A.kappa; B.nnbr = A.nnbr; B.nbr = A.nbr;

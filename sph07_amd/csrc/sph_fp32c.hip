@@ -31,6 +31,7 @@ namespace sph {
 #endif
 constexpr int kStrain32Batch = SPH_STRAIN32_BATCH, kMomentum32Batch = SPH_MOMENTUM32_BATCH;
 
+
 // TwoSum: s + e == a + b exactly (Knuth, TAOCP vol. 2, 4.2.2)
 __device__ __forceinline__ void two_sum_acc(float &s, float &c, float x)
 {
@@ -42,7 +43,7 @@ __device__ __forceinline__ void two_sum_acc(float &s, float &c, float x)
 }
 
 struct PairConst32 {
-    float ih, c1, c2, eps2, havg;
+    float ih, c1ih, eps2, havg;
 };
 
 __device__ __forceinline__ PairConst32 pc32(const PairConst &pc)
@@ -50,25 +51,23 @@ __device__ __forceinline__ PairConst32 pc32(const PairConst &pc)
     PairConst32 q;
     q.havg = (float)pc.havg;
     q.ih = (float)(1.0 / pc.havg);
-    q.c1 = (float)pc.const1;
-    q.c2 = (float)pc.const2;
+    q.c1ih = (float)(pc.const1 / pc.havg);
     q.eps2 = (float)pc.eps2;
     return q;
 }
 
-// dW/dr / r (so that grad W = f * dx) in FP32 from r^2
+// dW/dr / r in FP32 from r^2 (so that grad W = f * dx).  The B-spline
+// derivative (MKernelBSpline.cpp:166-172) in branch-free form:
+//   dW/dz = c1 (3 (1-z)+^2 - 0.75 (2-z)+^2),
+// which is c1 (2.25 z^2 - 3 z) for z < 1, -3 c2 (2-z)^2 for 1 <= z < 2
+// (c2 = c1 / 4) and 0 beyond; f = 0 at r = 0 as in the reference.
 __device__ __forceinline__ float gradw_factor(float r2, const PairConst32 &q)
 {
     const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     const float z = r2 * rinv * q.ih;
-    float dW;
-    if (z < 1.0f)
-        dW = q.c1 * ((2.25f * z - 3.0f) * z);
-    else if (z < 2.0f)
-        dW = -3.0f * q.c2 * ((2.0f - z) * (2.0f - z));
-    else
-        dW = 0.0f;
-    return dW * rinv * q.ih;
+    const float a = fmaxf(2.0f - z, 0.0f), b = fmaxf(1.0f - z, 0.0f);
+    const float dwz = 3.0f * (b * b) - 0.75f * (a * a);
+    return dwz * (rinv * q.c1ih);
 }
 
 __device__ __forceinline__ double eos_p32(const Mat &mt, double rho)
@@ -118,7 +117,7 @@ __device__ __forceinline__ double wave_min32(double v)
 
 // continuity sweep + RhoUpdate/EOS + critical step (fused mode of k_strain);
 // G neighbour records in flight per lane
-template <int D, int G = kStrain32Batch>
+template <int D, int G = kStrain32Batch, bool ONEMAT = false>
 __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, const int bid)
 {
     Ctl *ctl = A.ctl;
@@ -133,8 +132,9 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
         const double dt = ctl->dt;
         double rho_i = A.rho[i];
         if (act) {
-            float cs[D], ri[D], vi32[D], gs[D], gc[D];
-            const float4 fa = A.fsA[i], fb = A.fsB[i];
+            float cs[D], ri[D], vi32[D];
+            float gs = 0.0f, gc = 0.0f;
+            const float4 fa = rec32_at<0>(A.fs, (uint32_t)i), fb = rec32_at<1>(A.fs, (uint32_t)i);
             const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
             const uint32_t pk_i = __float_as_uint(fb.w);
 #pragma unroll
@@ -142,16 +142,14 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
                 cs[d] = cell_size32(ctl, d);
                 ri[d] = ri3[d];
                 vi32[d] = vi3[d];
-                gs[d] = 0.0f;
-                gc[d] = 0.0f;
             }
             const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
             struct Rec { float4 a, b; };
-            auto load = [&](int j) { return Rec{A.fsA[j], A.fsB[j]}; };
-            for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+            auto load = [&](int j) { return Rec{rec32_at<0>(A.fs, (uint32_t)j), rec32_at<1>(A.fs, (uint32_t)j)}; };
+            for_each_nbr_batched<G, true>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
                 const float4 ra = rec.a, rb = rec.b;
                 const uint32_t pk_j = __float_as_uint(rb.w);
-                const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)]);
+                const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)]);
                 const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
                 float off[3];
                 cell_offs(pk_i, pk_j, off);
@@ -161,14 +159,14 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
                     dx[d] = off[d] * cs[d] + (ri[d] - rj[d]);
                     r2 += dx[d] * dx[d];
                 }
-                const float f = gradw_factor(r2, q) * ra.w;
+                // the pair's share of tr D, sum over d of (v_j - v_i)_d * vol_j * dW/dx_d,
+                // goes into one compensated accumulator
+                float dvx = 0.0f;
 #pragma unroll
-                for (int d = 0; d < D; d++)
-                    two_sum_acc(gs[d], gc[d], (vj[d] - vi32[d]) * (f * dx[d]));
+                for (int d = 0; d < D; d++) dvx += (vj[d] - vi32[d]) * dx[d];
+                two_sum_acc(gs, gc, (gradw_factor(r2, q) * ra.w) * dvx);
             });
-            double tr = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; d++) tr += (double)gs[d] + (double)gc[d];
+            const double tr = (double)gs + (double)gc;
             A.trd[i] = tr;
             double xi[D], vi[D], xr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -205,7 +203,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
 // momentum sweep: pressure gradient, Morris viscosity, Monaghan term
 // COMPACT: the 32-byte records of Params::mcompact (mass and p/rho^2 from the
 // material and rho - rho0; images, flagged in the packed word, have p = 0)
-template <int D, int G = kMomentum32Batch, bool COMPACT = false>
+template <int D, int G = kMomentum32Batch, bool COMPACT = false, bool ONEMAT = false>
 __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
@@ -225,7 +223,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const float q_i = (float)(mi.nu * rho_i64);
         const bool monaghan = mi.visc_type == 2;
         float cs[D], ri[D], vi[D], as[D], ac[D];
-        const float4 fa = A.fmA[i], fb = A.fmB[i];
+        const float4 fa = rec32_at<0>(A.fm, (uint32_t)i), fb = rec32_at<1>(A.fm, (uint32_t)i);
         const uint32_t pk_i = COMPACT ? __float_as_uint(fb.w) : __float_as_uint(A.fmC[i].y);
         const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
 #pragma unroll
@@ -240,28 +238,33 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         struct Rec { float4 a, b; float2 c; };
         auto load = [&](int j) {
             Rec r;
-            r.a = A.fmA[j];
-            r.b = A.fmB[j];
+            r.a = rec32_at<0>(A.fm, (uint32_t)j);
+            r.b = rec32_at<1>(A.fm, (uint32_t)j);
             if (!COMPACT) r.c = A.fmC[j];
             return r;
         };
         const float rho0_i = (float)mi.rho0, c2_i = (float)(mi.c * mi.c), m_i = P.mass32[mat_i];
+        const float irho_i = (float)(1.0 / rho_i64);
+        // one material: the partner's constants are the particle's own
+        const float nu_i = (float)mi.nu;
+        const float acbar_i = (float)(mi.alpha * mi.c);     // alpha_bar * c_bar for equal materials
         for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
             const uint32_t fj = COMPACT ? __float_as_uint(rb.w) : __float_as_uint(rec.c.y);
-            const int mat_j = P.nmat == 1 ? mat_i : rec_mat(fj);
-            const PairConst32 q = P.nmat == 1 ? q0 : pc32(P.pc[mat_i][mat_j]);
-            float mj, rho_j, prho2_j;
+            const int mat_j = ONEMAT ? mat_i : rec_mat(fj);
+            const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][mat_j]);
+            float mj, rho_j, prho2_j, ir_j;
             if (COMPACT) {
-                const float rho0_j = P.nmat == 1 ? rho0_i : (float)P.mat[mat_j].rho0;
-                const float c2_j = P.nmat == 1 ? c2_i : (float)(P.mat[mat_j].c * P.mat[mat_j].c);
-                mj = P.nmat == 1 ? m_i : P.mass32[mat_j];
+                const float rho0_j = ONEMAT ? rho0_i : (float)P.mat[mat_j].rho0;
+                const float c2_j = ONEMAT ? c2_i : (float)(P.mat[mat_j].c * P.mat[mat_j].c);
+                mj = ONEMAT ? m_i : P.mass32[mat_j];
                 rho_j = rho0_j + ra.w;
-                const float ir = __builtin_amdgcn_rcpf(rho_j);
-                prho2_j = (fj & kRecGhost) ? 0.0f : c2_j * ra.w * ir * ir;
+                ir_j = __builtin_amdgcn_rcpf(rho_j);
+                prho2_j = (fj & kRecGhost) ? 0.0f : c2_j * ra.w * ir_j * ir_j;
             } else {
                 mj = ra.w;
                 rho_j = rb.w;
+                ir_j = __builtin_amdgcn_rcpf(rho_j);
                 prho2_j = rec.c.x;
             }
             const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
@@ -277,19 +280,23 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             }
             const float f = gradw_factor(r2, q);            // grad W = f * dx
             float sdw = -mj * (prho2_j + pr_i);             // coefficient of grad W
-            float tv = 0.0f;                                // coefficient of (v_i - v_j)
-            if (!(rig_i && (fj & kRecRigid))) {
-                const float qj = (float)P.mat[mat_j].nu * rho_j;
-                const float inv = __builtin_amdgcn_rcpf(rho_i * rho_j * (r2 + q.eps2));
-                tv = mj * (q_i + qj) * (f * r2) * inv;
-                if (monaghan && vr < 0.0f) {
+            // viscous terms, skipped between two rigid particles; evaluated
+            // for every pair and selected (no divergent branch per pair)
+            const bool visc = !(rig_i && (fj & kRecRigid));
+            const float ir2e = __builtin_amdgcn_rcpf(r2 + q.eps2);
+            const float qj = (ONEMAT ? nu_i : (float)P.mat[mat_j].nu) * rho_j;
+            // Morris: m_j (mu_i + mu_j) (r . grad W) / (rho_i rho_j (r^2 + 0.01 h^2))
+            const float tv = visc ? mj * (q_i + qj) * (f * r2) * (irho_i * ir_j * ir2e) : 0.0f;
+            if (monaghan) {
+                // Monaghan: -m_j Pi_ij grad W, Pi_ij = -alpha cbar mu / rho_bar for approaching pairs
+                float acbar = acbar_i;
+                if (!ONEMAT) {
                     const Mat &mj_ = P.mat[mat_j];
-                    const float alpha = (float)(0.5 * (mi.alpha + mj_.alpha));
-                    const float cbar = (float)(0.5 * (mi.c + mj_.c));
-                    const float mu = q.havg * vr * __builtin_amdgcn_rcpf(r2 + q.eps2);
-                    const float pij = -alpha * cbar * mu * __builtin_amdgcn_rcpf(0.5f * (rho_i + rho_j));
-                    sdw += -mj * pij;
+                    acbar = (float)(0.5 * (mi.alpha + mj_.alpha)) * (float)(0.5 * (mi.c + mj_.c));
                 }
+                const float mu = q.havg * vr * ir2e;
+                const float pterm = mj * (acbar * mu) * (2.0f * __builtin_amdgcn_rcpf(rho_i + rho_j));
+                sdw += (visc && vr < 0.0f) ? pterm : 0.0f;
             }
             const float sf = sdw * f;
 #pragma unroll
@@ -307,26 +314,34 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
     for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
 }
 
-template <int D, int G> __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
+template <int D, int G, bool ONEMAT> __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) strain32_body<D, G>(P, A, b);
+    if (b < nb) strain32_body<D, G, ONEMAT>(P, A, b);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    strain32_body<D>(Pt[e.x], At[e.x], e.y);
+    if (Pt[e.x].nmat == 1) strain32_body<D, kStrain32Batch, true>(Pt[e.x], At[e.x], e.y);
+    else strain32_body<D>(Pt[e.x], At[e.x], e.y);
 }
-template <int D, int G, bool COMPACT> __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
+template <int D, int G, bool COMPACT, bool ONEMAT>
+__global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) momentum32_body<D, G, COMPACT>(P, A, b);
+    if (b < nb) momentum32_body<D, G, COMPACT, ONEMAT>(P, A, b);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    if (Pt[e.x].mcompact) momentum32_body<D, kMomentum32Batch, true>(Pt[e.x], At[e.x], e.y);
-    else momentum32_body<D, kMomentum32Batch, false>(Pt[e.x], At[e.x], e.y);
+    const bool one = Pt[e.x].nmat == 1;
+    if (Pt[e.x].mcompact) {
+        if (one) momentum32_body<D, kMomentum32Batch, true, true>(Pt[e.x], At[e.x], e.y);
+        else momentum32_body<D, kMomentum32Batch, true, false>(Pt[e.x], At[e.x], e.y);
+    } else {
+        if (one) momentum32_body<D, kMomentum32Batch, false, true>(Pt[e.x], At[e.x], e.y);
+        else momentum32_body<D, kMomentum32Batch, false, false>(Pt[e.x], At[e.x], e.y);
+    }
 }
 
 void launch_batch_strain32(const BatchView &B, hipStream_t st)
@@ -349,33 +364,46 @@ void launch_batch_momentum32(const BatchView &B, hipStream_t st)
 
 static inline int nblk32(int n) { return (n + kBlock - 1) / kBlock; }
 
-template <int G>
-static void strain32_g(const Params &P, const Arrays &A, hipStream_t st)
+template <int G, bool ONEMAT>
+static void strain32_gm(const Params &P, const Arrays &A, hipStream_t st)
 {
     const dim3 grid(xcd_grid(nblk32(P.N))), blk(kBlock);
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL((k_strain32<1, G>), grid, blk, 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL((k_strain32<2, G>), grid, blk, 0, st, P, A); break;
-    default: hipLaunchKernelGGL((k_strain32<3, G>), grid, blk, 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_strain32<1, G, ONEMAT>), grid, blk, 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL((k_strain32<2, G, ONEMAT>), grid, blk, 0, st, P, A); break;
+    default: hipLaunchKernelGGL((k_strain32<3, G, ONEMAT>), grid, blk, 0, st, P, A); break;
     }
 }
 
-template <int G, bool C>
+// one-material scenes take the form with every material constant hoisted
+template <int G>
+static void strain32_g(const Params &P, const Arrays &A, hipStream_t st)
+{
+    if (P.nmat == 1) strain32_gm<G, true>(P, A, st);
+    else strain32_gm<G, false>(P, A, st);
+}
+
+template <int G, bool C, bool ONEMAT>
 static void momentum32_gc(const Params &P, const Arrays &A, hipStream_t st)
 {
     const dim3 grid(xcd_grid(nblk32(P.N))), blk(kBlock);
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL((k_momentum32<1, G, C>), grid, blk, 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL((k_momentum32<2, G, C>), grid, blk, 0, st, P, A); break;
-    default: hipLaunchKernelGGL((k_momentum32<3, G, C>), grid, blk, 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_momentum32<1, G, C, ONEMAT>), grid, blk, 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL((k_momentum32<2, G, C, ONEMAT>), grid, blk, 0, st, P, A); break;
+    default: hipLaunchKernelGGL((k_momentum32<3, G, C, ONEMAT>), grid, blk, 0, st, P, A); break;
     }
 }
 
 template <int G>
 static void momentum32_g(const Params &P, const Arrays &A, hipStream_t st)
 {
-    if (P.mcompact) momentum32_gc<G, true>(P, A, st);
-    else momentum32_gc<G, false>(P, A, st);
+    if (P.mcompact) {
+        if (P.nmat == 1) momentum32_gc<G, true, true>(P, A, st);
+        else momentum32_gc<G, true, false>(P, A, st);
+    } else {
+        if (P.nmat == 1) momentum32_gc<G, false, true>(P, A, st);
+        else momentum32_gc<G, false, false>(P, A, st);
+    }
 }
 
 void launch_strain32(const Params &P, const Arrays &A, hipStream_t st, int batch)

@@ -539,6 +539,9 @@ __device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, c
                                              int shift, const int *bc)
 {
     A.nnbr[i] = nn;
+    // pad the last chunk with the particle's own index: the batched sweeps
+    // load whole chunks without a per-entry bound
+    for (int k = nn; k & (kNbrChunk - 1); k++) A.nbr[nbr_slot(k, i, P.Ncap)] = i;
     if (P.want_mindist) A.mindist[i] = mind;
     if (shift) {
         const double dt = ctl->dt;

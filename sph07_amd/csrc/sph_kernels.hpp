@@ -38,9 +38,13 @@ __device__ __forceinline__ void for_each_nbr(const int *nbr, int i, int nn, int 
 
 // As for_each_nbr, with the record loads of G consecutive entries issued
 // before any is used: load(j) returns a neighbour's record, use(rec) consumes
-// it; entries past the list end load the particle's own record (a valid
-// address) and are not used.  The use order is the list order.
-template <int G, class Load, class Use>
+// it; the entries past the list end up to the chunk boundary hold the
+// particle's own index (padding written with the count), so they load a valid
+// record that is not used.  The use order is the list order.  LOADS_FIRST
+// pins the G loads ahead of every use with a scheduling barrier (the compiler
+// otherwise interleaves them with the arithmetic to save registers, which
+// leaves fewer gathers in flight).
+template <int G, bool LOADS_FIRST = false, class Load, class Use>
 __device__ __forceinline__ void for_each_nbr_batched(const int *nbr, int i, int nn, int ncap, Load &&load, Use &&use)
 {
     static_assert(kNbrChunk % G == 0, "batch divides the chunk");
@@ -54,13 +58,15 @@ __device__ __forceinline__ void for_each_nbr_batched(const int *nbr, int i, int 
             if (g0 >= m) break;
             decltype(load(0)) rec[G];
 #pragma unroll
-            for (int u = 0; u < G; u++) rec[u] = load(g0 + u < m ? e[g0 + u] : i);
+            for (int u = 0; u < G; u++) rec[u] = load(e[g0 + u]);   // entries past nn: padding (i)
+            if (LOADS_FIRST) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < G; u++)
                 if (g0 + u < m) use(rec[u]);
         }
     }
 }
+
 #endif
 
 // masks of launch_strain

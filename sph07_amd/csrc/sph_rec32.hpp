@@ -70,8 +70,8 @@ __device__ __forceinline__ void put_srec32(const Arrays &A, const Ctl *ctl, int 
     if (D > 1) { a.y = (float)(xs[1] - cell_origin(ctl, 1, bc[1])); b.y = (float)v[1]; }
     if (D > 2) { a.z = (float)(xs[2] - cell_origin(ctl, 2, bc[2])); b.z = (float)v[2]; }
     b.w = __uint_as_float(pack32(info, bc));
-    A.fsA[j] = a;
-    A.fsB[j] = b;
+    A.fs[2 * (size_t)j] = a;
+    A.fs[2 * (size_t)j + 1] = b;
 }
 
 // momentum record: {r[0..2], m}, {v[0..2], rho}, {p/rho^2, packed}; with
@@ -89,13 +89,22 @@ __device__ __forceinline__ void put_mrec32(const Params &P, const Arrays &A, con
     if (P.mcompact) {
         a.w = (float)(rho - P.mat[info & kMatMask].rho0);
         b.w = __uint_as_float(pack32(info, bc));
-        A.fmA[j] = a;
-        A.fmB[j] = b;
+        A.fm[2 * (size_t)j] = a;
+        A.fm[2 * (size_t)j + 1] = b;
         return;
     }
-    A.fmA[j] = a;
-    A.fmB[j] = b;
+    A.fm[2 * (size_t)j] = a;
+    A.fm[2 * (size_t)j + 1] = b;
     A.fmC[j] = make_float2((float)prho2, __uint_as_float(pack32(info, bc)));
+}
+
+// Half PART (0 or 1) of particle j's 32-byte record: a 32-bit byte offset from
+// the wave-uniform base, so the load takes the base in scalar registers and
+// one vector register of offset (the arm limits scenes to 2^27 slots).
+template <int PART>
+__device__ __forceinline__ float4 rec32_at(const float4 *base, uint32_t j)
+{
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + ((j << 5) + PART * 16u));
 }
 
 }  // namespace sph

@@ -122,10 +122,11 @@ struct Arrays {
     //   momentum:   rmA = {x[0..2], m}, rmB = {v[0..2], rho}, rmC = {p/rho^2, info}
     double4 *rsA, *rsB, *rmA, *rmB;
     double2 *rmC;
-    // FP32 records of the FP32-compensated arm (cell-relative positions):
-    //   continuity: fsA = {r[0..2], vol}, fsB = {v[0..2], packed}
-    //   momentum:   fmA = {r[0..2], m}, fmB = {v[0..2], rho}, fmC = {p/rho^2, packed}
-    float4 *fsA, *fsB, *fmA, *fmB;
+    // FP32 records of the FP32-compensated arm (cell-relative positions),
+    // two float4 per particle side by side (32 bytes, one sector):
+    //   continuity: fs[2j] = {r[0..2], vol}, fs[2j+1] = {v[0..2], packed}
+    //   momentum:   fm[2j] = {r[0..2], m}, fm[2j+1] = {v[0..2], rho}, fmC[j] = {p/rho^2, packed}
+    float4 *fs, *fm;
     float2 *fmC;
 
     uint32_t *info;
