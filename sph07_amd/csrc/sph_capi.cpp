@@ -163,7 +163,7 @@ int alloc_particles(Context *c, int gcap)
     Params &P = c->P;
     Arrays &A = c->AA[0], &B = c->AA[1];
     const size_t ncap = (size_t)c->N + gcap;
-    if (ncap * kNbrChunk > 0xffffffffull) {     // neighbour-list chunk stride is 32-bit
+    if (ncap > (1ull << 30)) {                  // neighbour-list chunk stride is 32-bit
         c->err = "too many particles and images for one context";
         return 1;
     }
@@ -174,6 +174,12 @@ int alloc_particles(Context *c, int gcap)
     std::vector<void *> &L = c->part_allocs;
     P.gcap = gcap;
     P.Ncap = (int)ncap;
+    P.nstride = 1;
+    P.nshift = 5;                              // log2(kNbrChunk * sizeof(int))
+    while ((size_t)P.nstride < ncap) {
+        P.nstride <<= 1;
+        P.nshift++;
+    }
     int rc = 0;
     for (int d = 0; d < 3; d++) {
         A.x[d] = A.xs[d] = A.v[d] = A.a[d] = A.nrm[d] = A.nn[d] = nullptr;
@@ -226,7 +232,7 @@ int alloc_particles(Context *c, int gcap)
         A.Nf = A.kappa = nullptr;
     }
     rc |= dalloc(c, L, &A.nnbr, c->N);
-    rc |= dalloc(c, L, &A.nbr, (size_t)nbr_round(c->maxnbr) * ncap);
+    rc |= dalloc(c, L, &A.nbr, (size_t)nbr_round(c->maxnbr) * (size_t)P.nstride);
     rc |= dalloc(c, L, &A.cell_of, c->N);
     rc |= dalloc(c, L, &A.occ, (size_t)c->N + 1);
     rc |= dalloc(c, L, &A.ovf, (size_t)c->N + 1);
@@ -1598,7 +1604,8 @@ long long sph_scene_footprint(const sph_scene_desc *d)
     // per slot (reals and images): x, v in both buffer sets, xs, rho and m in
     // both sets, info in both sets, the two gather records, rho_new/p/prho2/vol,
     // the neighbour list
-    const long long per_slot = 32 * D + 8 * D + 32 + 8 + 144 + 32 + 4 * maxnbr;
+    // (the list is counted twice: its chunk stride is a power of two >= slots)
+    const long long per_slot = 32 * D + 8 * D + 32 + 8 + 144 + 32 + 8 * maxnbr;
     // per real particle: a in both sets, orig, trd/mindist/critts, nnbr, sort
     // scratch, CSF fields
     const long long per_real = 16 * D + 8 + 24 + 4 + 12 + (d->surface_tension ? 16 * D + 16 : 0);

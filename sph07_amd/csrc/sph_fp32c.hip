@@ -146,7 +146,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
             const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
             struct Rec { float4 a, b; };
             auto load = [&](int j) { return Rec{rec32_at<0>(A.fs, (uint32_t)j), rec32_at<1>(A.fs, (uint32_t)j)}; };
-            for_each_nbr_batched<G, true>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+            for_each_nbr_batched<G, true>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
                 const float4 ra = rec.a, rb = rec.b;
                 const uint32_t pk_j = __float_as_uint(rb.w);
                 const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)]);
@@ -248,7 +248,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         // one material: the partner's constants are the particle's own
         const float nu_i = (float)mi.nu;
         const float acbar_i = (float)(mi.alpha * mi.c);     // alpha_bar * c_bar for equal materials
-        for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+        for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
             const uint32_t fj = COMPACT ? __float_as_uint(rb.w) : __float_as_uint(rec.c.y);
             const int mat_j = ONEMAT ? mat_i : rec_mat(fj);

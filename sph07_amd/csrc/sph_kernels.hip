@@ -541,7 +541,7 @@ __device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, c
     A.nnbr[i] = nn;
     // pad the last chunk with the particle's own index: the batched sweeps
     // load whole chunks without a per-entry bound
-    for (int k = nn; k & (kNbrChunk - 1); k++) A.nbr[nbr_slot(k, i, P.Ncap)] = i;
+    for (int k = nn; k & (kNbrChunk - 1); k++) A.nbr[nbr_slot(k, i, P.nstride)] = i;
     if (P.want_mindist) A.mindist[i] = mind;
     if (shift) {
         const double dt = ctl->dt;
@@ -681,7 +681,7 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
                     mind = mind < r1 ? mind : r1;
                 }
                 if (nn >= P.maxnbr) { ok = false; break; }
-                A.nbr[nbr_slot(nn, i, P.Ncap)] = cand;
+                A.nbr[nbr_slot(nn, i, P.nstride)] = cand;
                 nn++;
             }
         }
@@ -730,20 +730,24 @@ __device__ __forceinline__ double readlane_d(double v, int l)
 // Target t of a target group against the nb (<= NB) resident chunks
 // that start at staged index kb: candidates inside the cut-off are appended
 // to the target's list in candidate order.  selfk: the target's own staged
-// index relative to kb.  List offsets are 32-bit (the launcher keeps lists of
-// 2^32 entries or more on the per-particle walk).
+// index relative to kb.  Entry pos of the list of slot i is entry
+// i * 8 + ((pos / 8) << cshift) + pos % 8 of the list array, with
+// 1 << cshift = Params::nstride * 8 (a shift, not a multiply); its byte offset
+// is 32-bit (the launcher keeps larger lists on the per-particle walk), so the
+// store takes the list base in scalar registers.
 template <int D, bool ONEMAT, bool MIND, int NB>
 __device__ __forceinline__ void nbr_target_block(const Params &P, const Arrays &A, int t, int tg, int lane, int kb,
                                                  int nb, int ncand, int selfk, const double (&xi)[D], uint32_t fi,
                                                  const int (&cv)[NB], const double (&xcv)[NB][D],
                                                  const int (&matcv)[NB], double cut2_0,
-                                                 unsigned int cstride32, unsigned int maxnbr, int &nn, double &mind)
+                                                 unsigned int cshift, unsigned int maxnbr, int &nn, double &mind)
 {
     double xt[D];
 #pragma unroll
     for (int d = 0; d < D; d++) xt[d] = readlane_d(xi[d], t);
     unsigned int nnt = (unsigned int)__builtin_amdgcn_readlane(nn, t);
-    int *lst = A.nbr + (size_t)(tg + t) * kNbrChunk;
+    char *lst = reinterpret_cast<char *>(A.nbr);
+    const unsigned int tbase = (unsigned int)(tg + t) * kNbrChunk;
     const int matt = ONEMAT ? 0 : info_mat((uint32_t)__builtin_amdgcn_readlane((int)fi, t));
     double r1 = 1.0e20;
 #pragma unroll
@@ -762,7 +766,8 @@ __device__ __forceinline__ void nbr_target_block(const Params &P, const Arrays &
             if (acc) {
                 const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
-                if (pos < maxnbr) lst[(pos >> 3) * cstride32 + (pos & 7u)] = cv[c];
+                if (pos < maxnbr)
+                    *reinterpret_cast<int *>(lst + ((tbase + ((pos >> 3) << cshift) + (pos & 7u)) << 2)) = cv[c];
                 if (MIND) {
                     const double r = sqrt(d2);
                     r1 = r < r1 ? r : r1;
@@ -793,8 +798,8 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
     const int j0 = (int)((long long)P.N * wave / nwave), j1 = (int)((long long)P.N * (wave + 1) / nwave);
     const int ncode = ctl->ncode;
     const unsigned int m0 = ctl->cmask[0], m1 = ctl->cmask[1], m2 = ctl->cmask[2];
-    const unsigned int cstride32 = (unsigned int)P.Ncap * kNbrChunk;   // distance between list chunks
     const unsigned int maxnbr = (unsigned int)P.maxnbr;
+    const unsigned int cshift = (unsigned int)P.nshift - 2u;       // log2(nstride * kNbrChunk)
     int jn = j0;
     while (jn < j1) {
         const int q = __builtin_amdgcn_readfirstlane(A.occ[jn]);
@@ -889,7 +894,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
                 const int nb = min(NB, (ncand - kb + 63) >> 6);
                 for (int t = 0; t < nt; t++)
                     nbr_target_block<D, ONEMAT, MIND, NB>(P, A, t, tg, lane, kb, nb, ncand, self0 + t - kb, xi, fi, cv,
-                                                      xcv, matcv, cut2_0, cstride32, maxnbr, nn, mind);
+                                                      xcv, matcv, cut2_0, cshift, maxnbr, nn, mind);
             }
             if (tv) {
                 if (nn > P.maxnbr) {
@@ -1014,7 +1019,7 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
             }
             struct Rec { double4 a, b; };
             auto load = [&](int j) { return Rec{A.rsA[j], A.rsB[j]}; };
-            for_each_nbr_batched<kStrainBatch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+            for_each_nbr_batched<kStrainBatch>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
                 const double4 ra = rec.a, rb = rec.b;
                 const double volj = ra.w;
                 const PairConst &pc = P.pc[mat_i][P.nmat == 1 ? mat_i : info_mat((uint32_t)rb.w)];
@@ -1082,7 +1087,7 @@ __device__ __forceinline__ void normals_body(const Params &P, const Arrays &A, c
         xi[d] = A.x[d][i];
         nrm[d] = 0.0;
     }
-    for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
+    for_each_nbr(A.nbr, i, A.nnbr[i], P.nstride, [&](int j) {
         const uint32_t fj = A.info[j];
         const double volj = A.m[j] / A.rho[j];
         const PairConst &pc = P.pc[mat_i][info_mat(fj)];
@@ -1125,7 +1130,7 @@ __device__ __forceinline__ void curvature_body(const Params &P, const Arrays &A,
     }
     const double Ni = A.Nf[i];
     double div = 0.0;
-    for_each_nbr(A.nbr, i, A.nnbr[i], P.Ncap, [&](int j) {
+    for_each_nbr(A.nbr, i, A.nnbr[i], P.nstride, [&](int j) {
         const uint32_t fj = A.info[j];
         const bool img = fj & kGhost;        // images: N = 0, normals zero
         const double volj = A.m[j] / A.rho[j];
@@ -1205,7 +1210,7 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
         }
         struct Rec { double4 a, b; double2 c; };
         auto load = [&](int j) { return Rec{A.rmA[j], A.rmB[j], A.rmC[j]}; };
-        for_each_nbr_batched<kMomentumBatch>(A.nbr, i, A.nnbr[i], P.Ncap, load, [&](const Rec &rec) {
+        for_each_nbr_batched<kMomentumBatch>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
             const double4 ra = rec.a, rb = rec.b;
             const double2 rc = rec.c;
             const uint32_t fj = (uint32_t)rc.y;
@@ -1626,7 +1631,10 @@ template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P
 // The grid is a multiple of 8 blocks; each wave takes an equal share of the
 // slots.  At most 77 VGPRs (6 waves per SIMD) without spills.
 template <int D, bool ONEMAT, bool MIND>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_nbr_cells(Params P, Arrays A, int shift)
+#ifndef SPH_NBR_WAVES
+#define SPH_NBR_WAVES 6
+#endif
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES, 8))) void k_nbr_cells(Params P, Arrays A, int shift)
 {
     __shared__ int stage[kCellWaves][kCandCap];
     const int nb = (int)gridDim.x;
@@ -1730,7 +1738,7 @@ __global__ void k_canon(Params P, Arrays A, long long *out, int maxnbr)
     if (i >= P.N) return;
     const int nn = A.nnbr[i];
     for (int k = 0; k < nn && k < maxnbr; k++) {
-        const int j = A.nbr[nbr_slot(k, i, P.Ncap)];
+        const int j = A.nbr[nbr_slot(k, i, P.nstride)];
         long long cid;
         if (j < P.N) cid = A.orig[j];
         else {
@@ -1794,8 +1802,8 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots)
 {
-    if ((unsigned long long)P.Ncap * (unsigned long long)nbr_round(P.maxnbr) > 0xffffffffull) {
-        // the cell kernel's list offsets are 32-bit
+    if ((unsigned long long)P.nstride * (unsigned long long)nbr_round(P.maxnbr) * 4ull > 0xffffffffull) {
+        // the cell kernel's list byte offsets are 32-bit
         DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A,
                                               shift));
         return;

@@ -23,7 +23,8 @@ constexpr int kMaxMat = 8;
 constexpr int kBlock = 256;
 
 // Neighbour lists are stored in chunks of kNbrChunk entries: entry k of
-// particle i sits at ((k / kNbrChunk) * Ncap + i) * kNbrChunk + k % kNbrChunk,
+// particle i sits at ((k / kNbrChunk) * nstride + i) * kNbrChunk + k % kNbrChunk
+// (nstride: Params::nstride, a power of two, so a chunk offset is a shift),
 // so a particle's consecutive entries share one 32-byte sector (its writes
 // merge in L2) and a wave reads a chunk of 64 lists in whole lines.
 constexpr int kNbrChunk = 8;
@@ -87,6 +88,9 @@ struct PairConst {
 
 struct Params {
     int N, Ncap;          // real particles, stride of per-particle arrays (N + Gcap)
+    // neighbour lists: chunk stride in particles, the power of two >= Ncap,
+    // and log2 of the chunk stride in bytes (nstride * kNbrChunk * 4)
+    int nstride, nshift;
     int gcap;             // image capacity
     int ndim, tcrit_opt, maxnbr;
     int st, rigid, baseaccel, periodic;
