@@ -1399,8 +1399,9 @@ struct Batch {
     Arrays *dA = nullptr, *dA2 = nullptr;   // per slot: current / other buffer set
     int cap_slots = 0;
     int *dscenes = nullptr;
-    int2 *dpmap = nullptr, *dtmap = nullptr;
-    int cap_scenes = 0, cap_pmap = 0, cap_tmap = 0;
+    int2 *dpmap = nullptr, *dtmap = nullptr, *dwmap = nullptr;
+    int4 *dcmap = nullptr;
+    int cap_scenes = 0, cap_pmap = 0, cap_tmap = 0, cap_cmap = 0, cap_wmap = 0;
     Ctl *dctl = nullptr, *hctl = nullptr; // gathered control blocks
     int cap_ctl = 0;
     BatchView view{};
@@ -1452,12 +1453,19 @@ int batch_rebuild(Batch *b)
     // Arrays table has the same capacity as the Params table
     static_assert(sizeof(Arrays) > 0, "");
     std::vector<int> scenes;
-    std::vector<int2> pmap, tmap;
+    std::vector<int2> pmap, tmap, wmap;
+    std::vector<int4> cmap;
     int any_p = 0, any_st = 0, any_r = 0, mask = 0;
     for (int s : b->live) {
         Context *c = b->slots[s];
         scenes.push_back(s);
         for (int lb = 0; lb < (c->N + kBlock - 1) / kBlock; lb++) pmap.push_back(make_int2(s, lb));
+        if (cell_lists_fit(c->P)) {
+            const int nb = batch_cell_blocks(c->N, kBatchCellSlots);
+            for (int lb = 0; lb < nb; lb++) cmap.push_back(make_int4(s, lb, nb, 0));
+        } else {
+            for (int lb = 0; lb < (c->N + kBlock - 1) / kBlock; lb++) wmap.push_back(make_int2(s, lb));
+        }
         for (int t = 0; t < (c->P.ncell_cap + kScanTileItems - 1) / kScanTileItems; t++) tmap.push_back(make_int2(s, t));
         any_p |= c->P.periodic;
         any_st |= c->P.st;
@@ -1471,6 +1479,8 @@ int batch_rebuild(Batch *b)
     if (grow_dev(b, &b->dscenes, &b->cap_scenes, (int)scenes.size() + 1)) return SPH_ERR_HIP;
     if (grow_dev(b, &b->dpmap, &b->cap_pmap, (int)pmap.size() + 1)) return SPH_ERR_HIP;
     if (grow_dev(b, &b->dtmap, &b->cap_tmap, (int)tmap.size() + 1)) return SPH_ERR_HIP;
+    if (grow_dev(b, &b->dcmap, &b->cap_cmap, (int)cmap.size() + 1)) return SPH_ERR_HIP;
+    if (grow_dev(b, &b->dwmap, &b->cap_wmap, (int)wmap.size() + 1)) return SPH_ERR_HIP;
     if (grow_dev(b, &b->dctl, &b->cap_ctl, nslot + 1)) return SPH_ERR_HIP;
     if (b->hctl) hipHostFree(b->hctl);
     BCHK(b, hipHostMalloc((void **)&b->hctl, (size_t)b->cap_ctl * sizeof(Ctl), hipHostMallocDefault));
@@ -1497,6 +1507,10 @@ int batch_rebuild(Batch *b)
         BCHK(b, hipMemcpy(b->dscenes, scenes.data(), scenes.size() * sizeof(int), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->dpmap, pmap.data(), pmap.size() * sizeof(int2), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->dtmap, tmap.data(), tmap.size() * sizeof(int2), hipMemcpyHostToDevice));
+        if (!cmap.empty())
+            BCHK(b, hipMemcpy(b->dcmap, cmap.data(), cmap.size() * sizeof(int4), hipMemcpyHostToDevice));
+        if (!wmap.empty())
+            BCHK(b, hipMemcpy(b->dwmap, wmap.data(), wmap.size() * sizeof(int2), hipMemcpyHostToDevice));
     }
     BatchView &v = b->view;
     v.P = b->dP;
@@ -1508,6 +1522,10 @@ int batch_rebuild(Batch *b)
     v.nscene = (int)scenes.size();
     v.npart_blocks = (int)pmap.size();
     v.ntile_blocks = (int)tmap.size();
+    v.cmap = b->dcmap;
+    v.wmap = b->dwmap;
+    v.ncell_blocks = (int)cmap.size();
+    v.nwalk_blocks = (int)wmap.size();
     v.zero_nb = 64;
     v.slab_nb = 8;
     v.ndim = b->ndim;
@@ -1753,6 +1771,8 @@ void sph_batch_destroy(void *batch)
     hipFree(b->dscenes);
     hipFree(b->dpmap);
     hipFree(b->dtmap);
+    hipFree(b->dcmap);
+    hipFree(b->dwmap);
     hipFree(b->dctl);
     if (b->hctl) hipHostFree(b->hctl);
     hipStreamDestroy(b->st);

@@ -1630,10 +1630,10 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Para
 template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { XCD_BID(P); neighbours_body<D>(P, A, shift, bid_); }
 // The grid is a multiple of 8 blocks; each wave takes an equal share of the
 // slots.  At most 77 VGPRs (6 waves per SIMD) without spills.
-template <int D, bool ONEMAT, bool MIND>
 #ifndef SPH_NBR_WAVES
 #define SPH_NBR_WAVES 6
 #endif
+template <int D, bool ONEMAT, bool MIND>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES, 8))) void k_nbr_cells(Params P, Arrays A, int shift)
 {
     __shared__ int stage[kCellWaves][kCandCap];
@@ -1641,6 +1641,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_
     const int b = xcd_block(nb);
     const int wv = (int)(threadIdx.x >> 6);
     nbr_cells_body<D, ONEMAT, MIND>(P, A, shift, b * kCellWaves + wv, nb * kCellWaves, stage[wv]);
+}
+
+// Batched cell kernel: block b serves scene m.x, as its block m.y of m.z
+// (the scene's waves split its slots as in k_nbr_cells); the variant is
+// chosen per scene (uniform per block).
+template <int D>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES, 8))) void kb_nbr_cells(const Params *Pt, const Arrays *At, const int4 *map, int shift)
+{
+    __shared__ int stage[kCellWaves][kCandCap];
+    const int4 m = map[blockIdx.x];
+    SCENE_OF(m.x);
+    const int wv = (int)(threadIdx.x >> 6);
+    const int w = m.y * kCellWaves + wv, nw = m.z * kCellWaves;
+    if (Pb.nmat == 1 && !Pb.want_mindist) nbr_cells_body<D, true, false>(Pb, Ab, shift, w, nw, stage[wv]);
+    else if (Pb.nmat == 1) nbr_cells_body<D, true, true>(Pb, Ab, shift, w, nw, stage[wv]);
+    else if (!Pb.want_mindist) nbr_cells_body<D, false, false>(Pb, Ab, shift, w, nw, stage[wv]);
+    else nbr_cells_body<D, false, true>(Pb, Ab, shift, w, nw, stage[wv]);
 }
 
 // grid of the cell kernel: about `slots` slots per wave (a few cells), so
@@ -1664,6 +1681,21 @@ template <int D> __global__ __launch_bounds__(kBlock) void k_nbr_ovf(Params P, A
         const int q = A.ovf[w];
         const int c0 = A.cell_start[q], c1 = A.cell_start[q + 1];
         for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(P, A, shift, i);
+    }
+}
+// overflow cells of the batched cell kernel: kOvfBlocksPerScene blocks per scene
+constexpr int kOvfBlocksPerScene = 8;
+template <int D> __global__ __launch_bounds__(kBlock) void kb_nbr_ovf(const Params *Pt, const Arrays *At, const int *sc, int shift)
+{
+    const int s = sc[blockIdx.x / kOvfBlocksPerScene], part = (int)blockIdx.x % kOvfBlocksPerScene;
+    SCENE_OF(s);
+    const Ctl *ctl = Ab.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int n = ctl->novf;
+    for (int w = part; w < n; w += kOvfBlocksPerScene) {
+        const int q = Ab.ovf[w];
+        const int c0 = Ab.cell_start[q], c1 = Ab.cell_start[q + 1];
+        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(Pb, Ab, shift, i);
     }
 }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *Pt, const Arrays *At, const int2 *map, int shift)
@@ -1800,9 +1832,21 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
         }
 }
 
+bool cell_lists_fit(const Params &P)
+{
+    return (unsigned long long)P.nstride * (unsigned long long)nbr_round(P.maxnbr) * 4ull <= 0xffffffffull;
+}
+
+int batch_cell_blocks(int n, int slots)
+{
+    const long long per = (long long)slots * kCellWaves;
+    const long long nb = ((long long)n + per - 1) / per;
+    return (int)(nb < 1 ? 1 : nb);
+}
+
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots)
 {
-    if ((unsigned long long)P.nstride * (unsigned long long)nbr_round(P.maxnbr) * 4ull > 0xffffffffull) {
+    if (!cell_lists_fit(P)) {
         // the cell kernel's list byte offsets are 32-bit
         DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A,
                                               shift));
@@ -2012,7 +2056,12 @@ void launch_batch_step(const BatchView &B, hipStream_t st)
                 hipLaunchKernelGGL(kb_ghost_emit, dim3(ns * B.slab_nb), dim3(kBlock), 0, st, B.P, A2, B.scenes, B.slab_nb, m, side);
             }
     }
-    DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_neighbours<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap, 1));
+    if (B.ncell_blocks > 0) {
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_cells<DD>, dim3(B.ncell_blocks), dim3(kBlock), 0, st, B.P, A2, B.cmap, 1));
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_ovf<DD>, dim3(ns * kOvfBlocksPerScene), dim3(kBlock), 0, st, B.P, A2, B.scenes, 1));
+    }
+    if (B.nwalk_blocks > 0)
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_neighbours<DD>, dim3(B.nwalk_blocks), dim3(kBlock), 0, st, B.P, A2, B.wmap, 1));
     if (B.arm == 1) launch_batch_strain32(B, st);
     else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_strain<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap, M_STRAIN | M_CONST | M_CRIT));
     if (B.any_st) {

@@ -115,7 +115,12 @@ struct BatchView {
     const int *scenes;       // scene indices into P/A, nscene entries
     const int2 *pmap;        // npart_blocks entries
     const int2 *tmap;        // ntile_blocks entries (scan tiles over cells)
-    int nscene, npart_blocks, ntile_blocks;
+    // neighbour search: the cell kernel's blocks (scene, block, scene's blocks)
+    // and, for scenes whose lists exceed its 32-bit offsets, particle blocks
+    // of the per-particle walk
+    const int4 *cmap;
+    const int2 *wmap;
+    int nscene, npart_blocks, ntile_blocks, ncell_blocks, nwalk_blocks;
     int zero_nb, slab_nb;    // fixed blocks per scene for grid-stride kernels
     int ndim, arm;
     int any_periodic, any_st, any_rigid, pass_mask;
@@ -123,6 +128,12 @@ struct BatchView {
 constexpr int kScanTileItems = kBlock * 8;
 
 void launch_batch_step(const BatchView &B, hipStream_t st);
+// slots per wave of the batched cell kernel
+constexpr int kBatchCellSlots = 64;
+// blocks of the cell kernel for one scene of n particles
+int batch_cell_blocks(int n, int slots);
+// whether a scene's lists fit the cell kernel's 32-bit offsets
+bool cell_lists_fit(const Params &P);
 void launch_gather_ctl(const Arrays *At, const int *sc, int ns, Ctl *out, hipStream_t st);
 void launch_batch_strain32(const BatchView &B, hipStream_t st);
 void launch_batch_momentum32(const BatchView &B, hipStream_t st);
