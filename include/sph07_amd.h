@@ -212,6 +212,13 @@ int  sph_batch_live(void *batch);
 int  sph_batch_finished(void *batch, int *slots, int max);
 /* Device memory a scene of this deck will hold, in bytes (admission control). */
 long long sph_scene_footprint(const sph_scene_desc *desc);
+/* Launch shapes of a batch (timing only, never results): shape[0] = slots
+   per wave of the batched cell kernel, shape[1] = cell storage order (0
+   Morton, 1 row-major), shape[2] = 1 once fixed.  A batch of 65536 live
+   particles or more times each candidate on one 8-step graph of its own run
+   and keeps the cheapest per particle-step.  reps == 0 fixes the current
+   shapes (no tuning); reps < 0 only reports.  shape holds 3 entries. */
+int  sph_batch_autotune(void *batch, int reps, int32_t *shape);
 
 /* --- .mcm deck reader (MSimulationData::GetInput, MSimulationData.cpp:186-1068) --- */
 /* Reads a deck into an owned descriptor; *deck is freed with sph_deck_free. */
@@ -226,12 +233,17 @@ void sph_deck_free(void *deck);
 /* --- autotuner (no reference counterpart) ---
    Launch shapes that change timing but never results: shape[0] = slots per
    wave of the cell-centric neighbour search, shape[1] / shape[2] = neighbour
-   records in flight per lane in the FP32c continuity / momentum sweeps.
+   records in flight per lane in the FP32c continuity / momentum sweeps,
+   shape[3] = cell storage order (0 Morton, 1 row-major).  shape holds 4
+   entries.
    reps > 0 times every alternative reps times on the scene's current state
    and keeps the fastest (scenes of 65536 particles or more are tuned this way
    on their first sph_solution call); reps == 0 keeps the current shapes and
    turns the automatic tuning off; reps < 0 only reports.  shape may be NULL. */
 int  sph_autotune(void *ctx, int reps, int32_t *shape);
+/* Sets the four launch shapes of sph_autotune explicitly (slots per wave
+   4-4096, batches 2/4/8, order 0/1) and turns the automatic tuning off. */
+int  sph_set_launch_shape(void *ctx, const int32_t *shape);
 
 /* --- measurement (no reference counterpart: the reference only prints total
    wall seconds, MSimulationData.cpp:110-122) --- */

@@ -943,34 +943,19 @@ int build_graph(Context *c)
 }
 
 // Autotuner: times the alternatives of each launch shape on the scene's
-// current state and keeps the fastest.  It begins a step (sort and images,
-// repairing capacities as the step loop does) and re-runs kernels that are
-// idempotent on one step's inputs: the neighbour search (every slot
-// count), the continuity sweep (whose rewrite of the positions is undone
-// from a saved copy) and the momentum sweep.  The following step re-sorts
-// already sorted particles, so results are those of an untuned run.
+// current state and keeps the fastest.  For each cell storage order (Morton,
+// row-major) it begins a step (sort and images, repairing capacities as the
+// step loop does) and re-runs kernels that are idempotent on one step's
+// inputs: the neighbour search (every slot count), the continuity sweep
+// (whose rewrite of the positions is undone from a saved copy) and the
+// momentum sweep.  The order with the smallest sum of the three fastest
+// kernels wins.  Neither the order nor a shape changes a result (the lists
+// are in the reference's traversal order whatever the storage order), and
+// the following step re-sorts, so results are those of an untuned run.
 int autotune(Context *c, int reps)
 {
-    const Params &P = c->P;
+    Params &P = c->P;
     hipStream_t st = c->st;
-    for (int guard = 0;; guard++) {
-        if (sync_ctl(c)) return SPH_ERR_HIP;
-        if (c->hctl->done || c->hctl->err) break;
-        sort_step(c);
-        if (P.periodic) launch_ghost_setup(P, cur(c), st);
-        const int rc = check_device(c);
-        if (rc > 0) return rc;
-        if (rc == 0) break;
-        if (guard > 16) {
-            c->err = "ERROR: capacity repair did not converge.";
-            return SPH_ERROR;
-        }
-    }
-    if (c->hctl->done || c->hctl->err) {
-        c->tuned = true;
-        return SPH_OK;
-    }
-    const Arrays &A = cur(c);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -982,49 +967,80 @@ int autotune(Context *c, int reps)
         if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
         return ms;
     };
-    Tune best = c->tune;
-    float tbest = 1e30f;
-    for (int slots : {16, 32, 64, 128}) {
-        const float t = time_ms([&] {
-            hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
-            launch_neighbours(P, A, 1, st, slots);
-        });
-        if (t < tbest) { tbest = t; best.cell_slots = slots; }
+    auto finish = [&](int rc) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+    };
+    const size_t n = (size_t)c->N;
+    double *save = nullptr;
+    if (hipMalloc((void **)&save, n * sizeof(double) * c->ndim) != hipSuccess) {
+        c->err = "ERROR: autotuner allocation failed.";
+        return finish(SPH_ERR_HIP);
     }
-    // the chosen shape runs last, so the lists are from it (all shapes agree)
-    hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
-    launch_neighbours(P, A, 1, st, best.cell_slots);
+    Tune best = c->tune;
+    float best_total = 1e30f;
     int rc = SPH_OK;
-    if (c->arm == SPH_ARM_FP32C) {
-        const size_t n = (size_t)c->N;
-        double *save = nullptr;
-        if (hipMalloc((void **)&save, n * sizeof(double) * c->ndim) != hipSuccess) rc = SPH_ERR_HIP;
-        if (rc == SPH_OK) {
-            for (int d = 0; d < c->ndim; d++)
-                hipMemcpyAsync(save + d * n, A.x[d], n * sizeof(double), hipMemcpyDeviceToDevice, st);
-            tbest = 1e30f;
+    bool stopped = false;
+    for (int order : {0, 1}) {
+        P.corder = order;
+        for (int guard = 0;; guard++) {
+            if (sync_ctl(c)) { rc = SPH_ERR_HIP; break; }
+            if (c->hctl->done || c->hctl->err) { stopped = true; break; }
+            sort_step(c);
+            if (P.periodic) launch_ghost_setup(P, cur(c), st);
+            const int r = check_device(c);
+            if (r > 0) { rc = r; break; }
+            if (r == 0) break;
+            if (guard > 16) {
+                c->err = "ERROR: capacity repair did not converge.";
+                rc = SPH_ERROR;
+                break;
+            }
+        }
+        if (rc || stopped) break;
+        const Arrays &A = cur(c);
+        Tune t = c->tune;
+        t.corder = order;
+        float tn = 1e30f;
+        for (int slots : {16, 32, 64, 128}) {
+            const float ms = time_ms([&] {
+                hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
+                launch_neighbours(P, A, 1, st, slots);
+            });
+            if (ms < tn) { tn = ms; t.cell_slots = slots; }
+        }
+        hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
+        launch_neighbours(P, A, 1, st, t.cell_slots);     // lists of this order for the sweeps
+        for (int d = 0; d < c->ndim; d++)
+            hipMemcpyAsync(save + d * n, A.x[d], n * sizeof(double), hipMemcpyDeviceToDevice, st);
+        float ts = 1e30f, tm = 1e30f;
+        if (c->arm == SPH_ARM_FP32C) {
             for (int g : {4, 8}) {
-                const float t = time_ms([&] { launch_strain32(P, A, st, g); });
-                if (t < tbest) { tbest = t; best.strain32_batch = g; }
+                const float ms = time_ms([&] { launch_strain32(P, A, st, g); });
+                if (ms < ts) { ts = ms; t.strain32_batch = g; }
             }
-            tbest = 1e30f;
             for (int g : {2, 4, 8}) {
-                const float t = time_ms([&] { launch_momentum32(P, A, st, g); });
-                if (t < tbest) { tbest = t; best.momentum32_batch = g; }
+                const float ms = time_ms([&] { launch_momentum32(P, A, st, g); });
+                if (ms < tm) { tm = ms; t.momentum32_batch = g; }
             }
-            for (int d = 0; d < c->ndim; d++)
-                hipMemcpyAsync(A.x[d], save + d * n, n * sizeof(double), hipMemcpyDeviceToDevice, st);
-            hipStreamSynchronize(st);
-            hipFree(save);
+        } else {
+            ts = time_ms([&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st); });
+            tm = time_ms([&] { launch_momentum(P, A, 0, st); });
+        }
+        for (int d = 0; d < c->ndim; d++)
+            hipMemcpyAsync(A.x[d], save + d * n, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+        if (hipStreamSynchronize(st) != hipSuccess) { rc = SPH_ERR_HIP; break; }
+        if (tn + ts + tm < best_total) {
+            best_total = tn + ts + tm;
+            best = t;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    if (rc) {
-        c->err = "ERROR: autotuner allocation failed.";
-        return rc;
-    }
+    hipFree(save);
+    finish(0);
+    if (rc) return rc;
     c->tune = best;
+    P.corder = best.corder;
     c->tuned = true;
     drop_graph(c);
     rc = check_device(c);
@@ -1073,6 +1089,27 @@ int sph_solution(void *ctx, int max_steps, int *steps_done)
     return SPH_OK;
 }
 
+int sph_set_launch_shape(void *ctx, const int32_t *shape)
+{
+    Context *c = ctx_of(ctx);
+    if (!c || !shape) return SPH_ERROR;
+    const int slots = shape[0], gs = shape[1], gm = shape[2], order = shape[3];
+    if (slots < 4 || slots > 4096 || (gs != 2 && gs != 4 && gs != 8) || (gm != 2 && gm != 4 && gm != 8) ||
+        (order != 0 && order != 1)) {
+        c->err = "sph_set_launch_shape: shape out of range";
+        return SPH_ERROR;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->tune.cell_slots = slots;
+    c->tune.strain32_batch = gs;
+    c->tune.momentum32_batch = gm;
+    c->tune.corder = order;
+    c->P.corder = order;
+    c->tuned = true;
+    drop_graph(c);
+    return SPH_OK;
+}
+
 int sph_autotune(void *ctx, int reps, int32_t *shape)
 {
     Context *c = ctx_of(ctx);
@@ -1083,6 +1120,7 @@ int sph_autotune(void *ctx, int reps, int32_t *shape)
         shape[0] = c->tune.cell_slots;
         shape[1] = c->tune.strain32_batch;
         shape[2] = c->tune.momentum32_batch;
+        shape[3] = c->tune.corder;
     }
     return rc;
 }
@@ -1409,7 +1447,21 @@ struct Batch {
     hipGraphExec_t gexec2[2] = {nullptr, nullptr};   // captured steps, per starting parity
     int phase = 0;                                   // 1: tables swapped since the rebuild
     std::string err;
+    // launch shapes of the batch (timing only, never results): cell storage
+    // order of every scene and slots per wave of the cell kernel, chosen
+    // online (see sph_batch_step)
+    int corder = 0, cell_slots = kBatchCellSlots;
+    bool tuned = false;                  // shapes fixed (tuned, or tuning turned off)
+    int trial = 0;                       // next candidate to time
+    double trial_cost[8] = {0};          // ms per million particle-steps of each candidate
+    hipEvent_t ev[2] = {nullptr, nullptr};
 };
+
+// candidates of the batch tuner: (cell storage order, slots per wave)
+constexpr int kBatchCand[][2] = {{0, 64}, {1, 64}, {0, 32}, {1, 32}, {0, 128}, {1, 128}};
+constexpr int kBatchNCand = (int)(sizeof(kBatchCand) / sizeof(kBatchCand[0]));
+// batches of at least this many live particles tune their shapes online
+constexpr long long kBatchTuneMinN = 65536;
 
 #define BCHK(b, expr)                                                               \
     do {                                                                            \
@@ -1458,10 +1510,11 @@ int batch_rebuild(Batch *b)
     int any_p = 0, any_st = 0, any_r = 0, mask = 0;
     for (int s : b->live) {
         Context *c = b->slots[s];
+        c->P.corder = b->corder;           // the next step sorts in this order
         scenes.push_back(s);
         for (int lb = 0; lb < (c->N + kBlock - 1) / kBlock; lb++) pmap.push_back(make_int2(s, lb));
         if (cell_lists_fit(c->P)) {
-            const int nb = batch_cell_blocks(c->N, kBatchCellSlots);
+            const int nb = batch_cell_blocks(c->N, b->cell_slots);
             for (int lb = 0; lb < nb; lb++) cmap.push_back(make_int4(s, lb, nb, 0));
         } else {
             for (int lb = 0; lb < (c->N + kBlock - 1) / kBlock; lb++) wmap.push_back(make_int2(s, lb));
@@ -1645,6 +1698,7 @@ int sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int *
     if (rc) return rc;
     Context *c = ctx_of(ctx);
     c->hctl->step_limit = step_limit;
+    c->P.corder = b->corder;
     if (push_ctl(c)) {
         if (err && errlen > 0) snprintf(err, (size_t)errlen, "%s", c->err.c_str());
         sph_destroy(c);
@@ -1678,17 +1732,40 @@ int sph_batch_remove(void *batch, int slot)
 
 // Advances every live scene by up to nsteps steps; scenes stop on their own
 // step limit or stop test.  Returns the number of scenes still live in *live.
+// Online tuning: while a batch of at least kBatchTuneMinN live particles is
+// untuned, each step chunk that is one captured graph runs with the next
+// candidate shape and is timed with events; its cost per particle-step is
+// kept, and after the last candidate the cheapest is fixed.  Every candidate
+// gives the same results, so the tuning is done on the run's own steps.
+long long batch_particles(const Batch *b)
+{
+    long long n = 0;
+    for (int s : b->live) n += b->slots[s]->N;
+    return n;
+}
+
 int sph_batch_step(void *batch, int nsteps, int *live)
 {
     Batch *b = (Batch *)batch;
     constexpr int kBatchGraphSteps = 8;    // even: a graph starts and ends on the same parity
     int done_steps = 0;
     while (done_steps < nsteps && !b->live.empty()) {
+        const long long npart = batch_particles(b);
+        const bool tuning = !b->tuned && npart >= kBatchTuneMinN && nsteps - done_steps >= kBatchGraphSteps;
+        if (tuning && (b->corder != kBatchCand[b->trial][0] || b->cell_slots != kBatchCand[b->trial][1])) {
+            b->corder = kBatchCand[b->trial][0];
+            b->cell_slots = kBatchCand[b->trial][1];
+            b->dirty = true;
+        }
         if (b->dirty) {
             if (batch_rebuild(b)) return SPH_ERR_HIP;
             b->phase = 0;
         }
-        const int chunk = std::min(nsteps - done_steps, 32);
+        if (tuning && !b->ev[0]) {
+            BCHK(b, hipEventCreate(&b->ev[0]));
+            BCHK(b, hipEventCreate(&b->ev[1]));
+        }
+        const int chunk = tuning ? kBatchGraphSteps : std::min(nsteps - done_steps, 32);
         int k = 0;
         BatchView views[2] = {b->view, b->view};
         std::swap(views[1].A, views[1].A2);
@@ -1702,6 +1779,8 @@ int sph_batch_step(void *batch, int nsteps, int *live)
                 if (hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess) gx = nullptr;
                 hipGraphDestroy(g);
             }
+            // (timed after the capture, so a trial's cost is its steps only)
+            if (tuning) BCHK(b, hipEventRecord(b->ev[0], b->st));
             if (gx)
                 for (; k + kBatchGraphSteps <= chunk; k += kBatchGraphSteps) BCHK(b, hipGraphLaunch(gx, b->st));
         }
@@ -1717,9 +1796,40 @@ int sph_batch_step(void *batch, int nsteps, int *live)
             return SPH_ERR_HIP;
         }
         done_steps += chunk;
+        if (tuning) {
+            BCHK(b, hipEventRecord(b->ev[1], b->st));
+            BCHK(b, hipEventSynchronize(b->ev[1]));
+            float ms = 0.f;
+            BCHK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+            b->trial_cost[b->trial] = (double)ms * 1e6 / ((double)npart * chunk);
+            if (++b->trial == kBatchNCand) {
+                int best = 0;
+                for (int k = 1; k < kBatchNCand; k++)
+                    if (b->trial_cost[k] < b->trial_cost[best]) best = k;
+                b->corder = kBatchCand[best][0];
+                b->cell_slots = kBatchCand[best][1];
+                b->tuned = true;
+                b->dirty = true;
+            }
+        }
         if (batch_check(b)) return SPH_ERR_HIP;
     }
     if (live) *live = (int)b->live.size();
+    return SPH_OK;
+}
+
+// Reports the batch's launch shapes; reps == 0 fixes the current ones and
+// turns the online tuning off, reps < 0 only reports.
+int sph_batch_autotune(void *batch, int reps, int32_t *shape)
+{
+    Batch *b = (Batch *)batch;
+    if (!b) return SPH_ERROR;
+    if (reps == 0) b->tuned = true;
+    if (shape) {
+        shape[0] = b->cell_slots;
+        shape[1] = b->corder;
+        shape[2] = b->tuned ? 1 : 0;
+    }
     return SPH_OK;
 }
 
@@ -1773,6 +1883,8 @@ void sph_batch_destroy(void *batch)
     hipFree(b->dtmap);
     hipFree(b->dcmap);
     hipFree(b->dwmap);
+    for (hipEvent_t e : b->ev)
+        if (e) hipEventDestroy(e);
     hipFree(b->dctl);
     if (b->hctl) hipHostFree(b->hctl);
     hipStreamDestroy(b->st);

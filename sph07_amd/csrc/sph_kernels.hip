@@ -166,19 +166,26 @@ __device__ __forceinline__ void grid_setup_body(const Params &P, Ctl *ctl)
         }
     }
     for (int i = 0; i < 3; i++) ncell *= (ctl->glim[i] > 0 ? ctl->glim[i] : 0);
-    // Storage order of the cells: a bit-interleaved (Morton) code with the
-    // innermost traversal axis in the lowest bit of every level and each axis
-    // contributing ceil(log2(glim)) bits, so that a contiguous range of sorted
-    // particles is a compact region of space.  The traversal order of the
-    // neighbour search is unaffected (it walks cells by coordinates).
+    // Storage order of the cells, as the bits of a cell code that each axis
+    // owns (each axis contributing ceil(log2(glim)) bits).  Morton: the bits
+    // interleaved, the innermost traversal axis in the lowest bit of every
+    // level, so that a contiguous range of sorted particles is a compact
+    // region of space.  Row-major: z in the lowest bits, then y, then x.  The
+    // traversal order of the neighbour search is unaffected (it walks cells
+    // by coordinates), so results do not depend on the choice.
     int mb[3] = {0, 0, 0};
     for (int d = 0; d < 3; d++)
         while (mb[d] < 31 && (1LL << mb[d]) < (long long)ctl->glim[d]) mb[d]++;
     unsigned int msk[3] = {0u, 0u, 0u};
     int pos = 0;
-    for (int l = 0; l < 31 && pos <= 30; l++)
+    if (P.corder == 1) {
         for (int e = 2; e >= 0; e--)
-            if (l < mb[e] && pos <= 30) msk[e] |= 1u << pos++;
+            for (int l = 0; l < mb[e] && pos <= 30; l++) msk[e] |= 1u << pos++;
+    } else {
+        for (int l = 0; l < 31 && pos <= 30; l++)
+            for (int e = 2; e >= 0; e--)
+                if (l < mb[e] && pos <= 30) msk[e] |= 1u << pos++;
+    }
     const long long ncode = pos <= 29 ? (1LL << pos) : 0x7fffffffLL;
     const long long need = ncode > ncell ? ncode : ncell;
     if (ncell <= 0 || pos > 29 || need + 2 > P.ncell_cap) {

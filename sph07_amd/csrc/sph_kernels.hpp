@@ -83,6 +83,7 @@ struct Tune {
     int cell_slots = 32;       // slots per wave of the cell-centric neighbour kernel
     int strain32_batch = 8;    // neighbour records in flight per lane, FP32c continuity
     int momentum32_batch = 4;  // the same, FP32c momentum
+    int corder = 0;            // cell storage order: 0 Morton, 1 row-major
 };
 
 void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots = 32);

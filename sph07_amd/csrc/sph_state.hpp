@@ -98,6 +98,9 @@ struct Params {
     int ncell_cap;
     int want_mindist;
     int arm;              // 0 FP64, 1 FP32-compensated (selects the records the fused step writes)
+    // storage order of the cells: 0 Morton (bit-interleaved code), 1 row-major
+    // (z fastest); results do not depend on it, only the memory locality
+    int corder;
     // FP32c momentum record in 32 bytes: every material has one particle mass
     // (mass32) and the linear EOS, so m and p/rho^2 follow from the material
     // and the stored rho - rho0

@@ -212,9 +212,19 @@ class Scene:
         shapes (no automatic tuning), < 0 only reports them."""
         L = lib()
         L.sph_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
-        shape = (ctypes.c_int32 * 3)()
+        shape = (ctypes.c_int32 * 4)()
         self._chk(L.sph_autotune(self._h, reps, shape))
-        return {"cell_slots": shape[0], "strain32_batch": shape[1], "momentum32_batch": shape[2]}
+        return {"cell_slots": shape[0], "strain32_batch": shape[1], "momentum32_batch": shape[2],
+                "cell_order": "row-major" if shape[3] == 1 else "morton"}
+
+    def set_launch_shape(self, cell_slots: int = 64, strain32_batch: int = 8, momentum32_batch: int = 4,
+                         cell_order: str = "morton"):
+        """sph_set_launch_shape: fixes the launch shapes (no automatic tuning)."""
+        L = lib()
+        L.sph_set_launch_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+        shape = (ctypes.c_int32 * 4)(cell_slots, strain32_batch, momentum32_batch,
+                                     1 if cell_order == "row-major" else 0)
+        self._chk(L.sph_set_launch_shape(self._h, shape))
 
     def timing(self) -> dict:
         """{group name: (total ms, launches)} from HIP events on the scene's stream."""
@@ -448,6 +458,16 @@ class Batch:
     @property
     def live(self) -> int:
         return lib().sph_batch_live(self._h)
+
+    def autotune(self, reps: int = -1) -> dict:
+        """sph_batch_autotune: 0 fixes the current shapes (no online tuning),
+        < 0 only reports."""
+        L = lib()
+        L.sph_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
+        shape = (ctypes.c_int32 * 3)()
+        self._chk(L.sph_batch_autotune(self._h, reps, shape))
+        return {"cell_slots": shape[0], "cell_order": "row-major" if shape[1] == 1 else "morton",
+                "tuned": bool(shape[2])}
 
     def finished(self):
         L = lib()

@@ -61,3 +61,33 @@ def test_slots_are_reused_and_scenes_join_mid_run():
         pass
     assert b.status(s3)["timestep"] == 6
     b.close()
+
+
+@pytest.mark.parametrize("arm", [ARM_FP64, ARM_FP32C])
+def test_batch_online_tuning_keeps_results(arm):
+    """A batch large enough to tune its launch shapes online (cell storage
+    order, slots per wave) ends bit-identical to the same batch with the
+    tuning turned off, and each scene to the scene alone."""
+    items = [(scenes.cavity(n=170), 70), (scenes.poiseuille(nx=160, ny=160, dx=1e-3 / 160), 64),
+             (scenes.droplet(n_box=120), 56)]
+    assert sum(d.np for d, _ in items) >= 65536
+    tuned, fixed = Batch(2, arm), Batch(2, arm)
+    fixed.autotune(0)
+    st = [tuned.add(d, lim) for d, lim in items]
+    sf = [fixed.add(d, lim) for d, lim in items]
+    for b in (tuned, fixed):
+        rounds = 0
+        while b.step(16) and rounds < 100:
+            rounds += 1
+        assert b.live == 0
+    shape = tuned.autotune(-1)
+    assert shape["tuned"] and shape["cell_slots"] in (32, 64, 128), shape
+    assert fixed.autotune(-1)["cell_slots"] == 64
+    for (d, lim), a, c in zip(items, st, sf):
+        for f in (F_X, F_V, F_RHO):
+            assert np.array_equal(tuned.field(a, f), fixed.field(c, f)), (d.name, f)
+    alone = Scene(items[0][0], arm)
+    assert alone.Solution(items[0][1]) == items[0][1]
+    assert np.array_equal(alone.field(F_V), tuned.field(st[0], F_V))
+    tuned.close()
+    fixed.close()

@@ -355,3 +355,23 @@ def test_autotune_changes_timing_not_results(arm_name):
     assert shape["cell_slots"] in (16, 32, 64, 128)
     for f in (F_X, F_V, F_RHO):
         assert np.array_equal(tuned.field(f), plain.field(f))
+
+
+@pytest.mark.parametrize("arm_name", ["fp64", "fp32c"])
+@pytest.mark.parametrize("name", ["dam_break", "poiseuille", "droplet"])
+def test_cell_order_changes_timing_not_results(arm_name, name):
+    """The cells stored in row-major instead of Morton order: the same bits
+    after 30 fused steps (the neighbour lists follow the traversal order, not
+    the storage order)."""
+    from sph07_amd.native import Scene, ARM_FP64, ARM_FP32C, F_X, F_V, F_RHO
+    arm = ARM_FP64 if arm_name == "fp64" else ARM_FP32C
+    deck = small_scenes()[name]()
+    morton, rowmajor = Scene(deck, arm), Scene(deck, arm)
+    morton.set_launch_shape(cell_order="morton")
+    rowmajor.set_launch_shape(cell_order="row-major", cell_slots=32)
+    assert morton.Solution(30) == rowmajor.Solution(30) == 30
+    for f in (F_X, F_V, F_RHO):
+        assert np.array_equal(morton.field(f), rowmajor.field(f)), f
+    nm, cm = morton.neighbours()
+    nr, cr = rowmajor.neighbours()
+    assert np.array_equal(nm, nr) and np.array_equal(cm, cr)
