@@ -228,7 +228,34 @@ const sph_scene_desc *sph_deck_desc(void *deck);
 /* Title card (:194-196) and status interval of control card 3 (:274-275). */
 const char *sph_deck_title(void *deck);
 int  sph_deck_status_interval(void *deck);
+/* Time between state plots and the state-output format of control card 3
+   (MSimulationData.cpp:274-275; format 2 = EnSight CASE, MOutput.cpp:45-67). */
+double sph_deck_state_time(void *deck);
+int  sph_deck_state_opt(void *deck);
 void sph_deck_free(void *deck);
+
+/* --- state output (MOutput, MOutput.cpp:39-720; host side, cold path) --- */
+/* Opens the EnSight CASE output of a scene under the base name `base` (the
+   reference uses the deck path): files base.case, base.geo, base_time.txt and
+   base_{sig,dis,vel,acc,eps,rho,egy,snd,mas,bnd,nbr,tmp}.ens.  The current
+   positions become xzero of the displacements.  mat_id: np 1-based materials. */
+int  sph_output_open(void *ctx, const char *base, int32_t ndim, int32_t nummat,
+                     const int32_t *mat_id, void **out);
+/* MOutput::StateOutput (:73-114): the geometry with the first state, the case
+   file with every state's time, one time step of each variable file.  Fields
+   the path does not carry are written as zero (efps, energy, temperature, the
+   boundary flag) or as -p on the diagonal (stress: no deviatoric part). */
+int  sph_output_state(void *out);
+/* MSolution::PlotDroppingCylinder (MSolution.cpp:1638-1668): x(0) and rho
+   of particles 1..np-1, "%13.5e%13.5e" per line. */
+int  sph_output_cylinder_plot(void *out, const char *path);
+/* Appends time, R, V, OMEGA of the rigid body to path (%13.5e each). */
+int  sph_output_rigid(void *out, const char *path);
+const char *sph_output_last_error(void *out);
+void sph_output_close(void *out);
+/* Globals and x, v, a, rho, p of every particle (deck order) as hexadecimal
+   floats: a full-precision state dump. */
+int  sph_write_state_dump(void *ctx, const char *path);
 
 /* --- autotuner (no reference counterpart) ---
    Launch shapes that change timing but never results: shape[0] = slots per

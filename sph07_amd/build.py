@@ -20,7 +20,7 @@ ARCH = os.environ.get("SPH07_ARCH", "gfx950")
 # the reference's operation order; the FP32-compensated sweeps may fuse
 # multiply-adds (TwoSum itself is additions only, so contraction cannot touch it).
 SOURCES = [("sph_kernels.hip", "off"), ("sph_fp32c.hip", "fast"), ("sph_capi.cpp", "off"),
-           ("mcm_reader.cpp", "off")]
+           ("mcm_reader.cpp", "off"), ("sph_output.cpp", "off")]
 EXE = os.path.join(HERE, "sph07_run")
 
 COMMON = [

@@ -25,6 +25,7 @@ struct Deck {
     int version = 0;
     int status_interval = 100;
     double stpltime = 0.0;
+    int state_opt = 0;
 };
 
 struct Reader {
@@ -72,9 +73,9 @@ int read_deck(Deck *dk, FILE *f)
     if (!rd.next(line)) return fail(dk, rd, "missing control card 2");
     sscanf(line.c_str(), "%10lf%10lf%10lf%10lf", &d.endtime, &d.tssf, &d.itss, &drelax);
     double thpltime = 0.0;
-    int state_opt = 0, thnode = 0, ntrans = 0, restart = 0, run_restart = 0;
+    int thnode = 0, ntrans = 0, restart = 0, run_restart = 0;
     if (!rd.next(line)) return fail(dk, rd, "missing control card 3");
-    sscanf(line.c_str(), "%10lf%5d%10lf%10d%5d%10d%5d%5d", &dk->stpltime, &state_opt, &thpltime,
+    sscanf(line.c_str(), "%10lf%5d%10lf%10d%5d%10d%5d%5d", &dk->stpltime, &dk->state_opt, &thpltime,
            &thnode, &ntrans, &dk->status_interval, &restart, &run_restart);
     int keep_files = 0, init_v_opt = 0;
     if (!rd.next(line)) return fail(dk, rd, "missing control card 4");
@@ -310,6 +311,16 @@ const char *sph_deck_title(void *deck)
 int sph_deck_status_interval(void *deck)
 {
     return deck ? ((Deck *)deck)->status_interval : 100;
+}
+
+double sph_deck_state_time(void *deck)
+{
+    return deck ? ((Deck *)deck)->stpltime : 0.0;
+}
+
+int sph_deck_state_opt(void *deck)
+{
+    return deck ? ((Deck *)deck)->state_opt : 0;
 }
 
 void sph_deck_free(void *deck)

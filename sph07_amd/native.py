@@ -54,7 +54,7 @@ def header_symbols():
     """Names of the functions declared in include/sph07_amd.h."""
     txt = open(HEADER).read()
     return sorted(set(re.findall(
-        r"^\s*(?:int|void|long long|const char|const sph_scene_desc)\s*\**\s*(sph_\w+)\s*\(",
+        r"^\s*(?:int|void|long long|double|const char|const sph_scene_desc)\s*\**\s*(sph_\w+)\s*\(",
         txt, re.M)))
 
 
@@ -288,6 +288,79 @@ class Scene:
         out = np.zeros((2, 3), np.int32)
         self._chk(_surface().sph_get_loop_limits(self._h, ndim, b.ctypes.data, out.ctypes.data))
         return out
+
+
+class StateOutput:
+    """The reference's state output of a scene (MOutput, EnSight CASE format;
+    sph_output_*): open it after creating the scene (the current positions are
+    the displacements' origin), then call state() at each output time."""
+
+    def __init__(self, scene: "Scene", base: str, deck: Deck):
+        L = lib()
+        vp = ctypes.c_void_p
+        L.sph_output_open.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, vp,
+                                      ctypes.POINTER(vp)]
+        for name in ("sph_output_state",):
+            getattr(L, name).argtypes = [vp]
+        L.sph_output_cylinder_plot.argtypes = [vp, ctypes.c_char_p]
+        L.sph_output_rigid.argtypes = [vp, ctypes.c_char_p]
+        L.sph_output_last_error.argtypes = [vp]
+        L.sph_output_last_error.restype = ctypes.c_char_p
+        L.sph_output_close.argtypes = [vp]
+        L.sph_output_close.restype = None
+        self._mat = np.ascontiguousarray(deck.mat_id, np.int32)
+        h = vp()
+        rc = L.sph_output_open(scene._h, base.encode(), deck.ndim, len(deck.materials),
+                               self._mat.ctypes.data, ctypes.byref(h))
+        if rc != 0:
+            raise SphError(rc, "sph_output_open failed")
+        self._h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SphError(rc, lib().sph_output_last_error(self._h).decode())
+
+    def state(self):
+        self._chk(lib().sph_output_state(self._h))
+
+    def cylinder_plot(self, path: str):
+        self._chk(lib().sph_output_cylinder_plot(self._h, path.encode()))
+
+    def rigid(self, path: str):
+        self._chk(lib().sph_output_rigid(self._h, path.encode()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sph_output_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def write_state_dump(scene: "Scene", path: str):
+    """sph_write_state_dump: globals and x, v, a, rho, p as hexadecimal floats."""
+    L = lib()
+    L.sph_write_state_dump.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    rc = L.sph_write_state_dump(scene._h, path.encode())
+    if rc != 0:
+        raise SphError(rc, "sph_write_state_dump failed")
+
+
+def read_state_dump(path: str) -> dict:
+    """Parses a state dump written by write_state_dump."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    assert lines[0] == "sph07_amd state dump"
+    head = lines[1].split()
+    out = {"np": int(head[1]), "timestep": int(head[3])}
+    g = lines[2].split()
+    for k in range(0, len(g), 2):
+        out[g[k]] = float.fromhex(g[k + 1])
+    vals = np.array([[float.fromhex(t) for t in ln.split()] for ln in lines[4:4 + out["np"]]])
+    out["x"], out["v"], out["a"] = vals[:, 0:3], vals[:, 3:6], vals[:, 6:9]
+    out["rho"], out["p"] = vals[:, 9], vals[:, 10]
+    return out
 
 
 LINKEDLISTEND = -1000000   # global.h:13
