@@ -17,9 +17,12 @@ LIB = os.path.join(HERE, "libsph07_amd.so")
 ARCH = os.environ.get("SPH07_ARCH", "gfx950")
 
 # (source, floating-point contraction): the FP64 arm and the host code keep
-# the reference's operation order; the FP32-compensated sweeps may fuse
-# multiply-adds (TwoSum itself is additions only, so contraction cannot touch it).
-SOURCES = [("sph_kernels.hip", "off"), ("sph_fp32c.hip", "fast"), ("sph_capi.cpp", "off"),
+# the reference's operation order; the FP32-compensated sweeps fuse the
+# multiply-adds written within one expression ("on": decided per source
+# expression, so every launch shape of a sweep computes the same bits; "fast"
+# lets the backend fuse across statements, differently per instantiation).
+# TwoSum itself is additions only, so contraction cannot touch it.
+SOURCES = [("sph_kernels.hip", "off"), ("sph_fp32c.hip", "on"), ("sph_capi.cpp", "off"),
            ("mcm_reader.cpp", "off"), ("sph_output.cpp", "off")]
 EXE = os.path.join(HERE, "sph07_run")
 

@@ -629,7 +629,9 @@ int create_impl(Context *c, const sph_scene_desc *d)
         }
         for (int m = 1; m <= d->nummat && uniform; m++)
             if (d->mat[m - 1].eos != 29) uniform = false;
-        P.mcompact = uniform ? 1 : 0;
+        // 2: one material and no rigid body, so the record needs no material
+        // or rigid bits and carries 1/rho and p/rho^2 directly
+        P.mcompact = uniform ? ((d->nummat == 1 && !P.rigid) ? 2 : 1) : 0;
         for (int m = 0; m <= kMaxMat; m++) P.mass32[m] = mref[m] > 0.0 ? (float)mref[m] : 0.0f;
     }
     P.n_rigid = (int)rigid_idx.size();

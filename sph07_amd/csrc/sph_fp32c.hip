@@ -108,6 +108,12 @@ __device__ __forceinline__ double crit_one32(const Params &P, const Arrays &A, c
     return ct;
 }
 
+// a wave-uniform float kept in a scalar register
+__device__ __forceinline__ float uniform_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 __device__ __forceinline__ double wave_min32(double v)
 {
 #pragma unroll
@@ -200,12 +206,14 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
     }
 }
 
-// momentum sweep: pressure gradient, Morris viscosity, Monaghan term
-// COMPACT: the 32-byte records of Params::mcompact (mass and p/rho^2 from the
-// material and rho - rho0; images, flagged in the packed word, have p = 0)
-template <int D, int G = kMomentum32Batch, bool COMPACT = false, bool ONEMAT = false>
+// momentum sweep: pressure gradient, Morris viscosity, Monaghan term.
+// KIND: the momentum record of Params::mcompact (sph_rec32.hpp): 0 full
+// (40 bytes), 1 {R, rho - rho0}, {v, packed}, 2 {R, 1/rho}, {v, p/rho^2}
+// (one material, no rigid body; implies ONEMAT).
+template <int D, int G = kMomentum32Batch, int KIND = 0, bool ONEMAT = false>
 __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A, const int bid)
 {
+    static_assert(KIND != 2 || ONEMAT, "the one-material record has no material bits");
     const Ctl *ctl = A.ctl;
     if (CTL_STOP32(ctl)) return;
     const int i = bid * kBlock + threadIdx.x;
@@ -222,13 +230,13 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const float rho_i = (float)rho_i64, pr_i = (float)A.prho2[i];
         const float q_i = (float)(mi.nu * rho_i64);
         const bool monaghan = mi.visc_type == 2;
-        float cs[D], ri[D], vi[D], as[D], ac[D];
+        float tcs[D], itcs[D], ri[D], vi[D], as[D], ac[D];
         const float4 fa = rec32_at<0>(A.fm, (uint32_t)i), fb = rec32_at<1>(A.fm, (uint32_t)i);
-        const uint32_t pk_i = COMPACT ? __float_as_uint(fb.w) : __float_as_uint(A.fmC[i].y);
         const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            cs[d] = cell_size32(ctl, d);
+            tcs[d] = uniform_f(2.0f * cell_size32(ctl, d));
+            itcs[d] = uniform_f(1.0f / tcs[d]);
             ri[d] = ri3[d];
             vi[d] = vi3[d];
             as[d] = 0.0f;
@@ -240,21 +248,26 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             Rec r;
             r.a = rec32_at<0>(A.fm, (uint32_t)j);
             r.b = rec32_at<1>(A.fm, (uint32_t)j);
-            if (!COMPACT) r.c = A.fmC[j];
+            if (KIND == 0) r.c = A.fmC[j];
             return r;
         };
         const float rho0_i = (float)mi.rho0, c2_i = (float)(mi.c * mi.c), m_i = P.mass32[mat_i];
         const float irho_i = (float)(1.0 / rho_i64);
         // one material: the partner's constants are the particle's own
         const float nu_i = (float)mi.nu;
+        const float mnu_i = m_i * nu_i;
         const float acbar_i = (float)(mi.alpha * mi.c);     // alpha_bar * c_bar for equal materials
         for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
-            const uint32_t fj = COMPACT ? __float_as_uint(rb.w) : __float_as_uint(rec.c.y);
+            const uint32_t fj = KIND == 1 ? __float_as_uint(rb.w) : (KIND == 0 ? __float_as_uint(rec.c.y) : 0u);
             const int mat_j = ONEMAT ? mat_i : rec_mat(fj);
             const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][mat_j]);
-            float mj, rho_j, prho2_j, ir_j;
-            if (COMPACT) {
+            float mj, rho_j = 0.0f, prho2_j, ir_j;
+            if (KIND == 2) {
+                mj = m_i;
+                ir_j = ra.w;
+                prho2_j = rb.w;
+            } else if (KIND == 1) {
                 const float rho0_j = ONEMAT ? rho0_i : (float)P.mat[mat_j].rho0;
                 const float c2_j = ONEMAT ? c2_i : (float)(P.mat[mat_j].c * P.mat[mat_j].c);
                 mj = ONEMAT ? m_i : P.mass32[mat_j];
@@ -268,12 +281,10 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
                 prho2_j = rec.c.x;
             }
             const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
-            float off[3];
-            cell_offs(pk_i, fj, off);
             float dx[D], dv[D], r2 = 0.0f, vr = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                dx[d] = off[d] * cs[d] + (ri[d] - rj[d]);
+                dx[d] = sep32(ri[d], rj[d], tcs[d], itcs[d]);
                 dv[d] = vi[d] - vj[d];
                 r2 += dx[d] * dx[d];
                 vr += dv[d] * dx[d];
@@ -282,11 +293,17 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             float sdw = -mj * (prho2_j + pr_i);             // coefficient of grad W
             // viscous terms, skipped between two rigid particles; evaluated
             // for every pair and selected (no divergent branch per pair)
-            const bool visc = !(rig_i && (fj & kRecRigid));
+            const bool visc = KIND == 2 || !(rig_i && (fj & kRecRigid));
             const float ir2e = __builtin_amdgcn_rcpf(r2 + q.eps2);
-            const float qj = (ONEMAT ? nu_i : (float)P.mat[mat_j].nu) * rho_j;
-            // Morris: m_j (mu_i + mu_j) (r . grad W) / (rho_i rho_j (r^2 + 0.01 h^2))
-            const float tv = visc ? mj * (q_i + qj) * (f * r2) * (irho_i * ir_j * ir2e) : 0.0f;
+            // Morris: m_j (mu_i + mu_j) (r . grad W) / (rho_i rho_j (r^2 + 0.01 h^2));
+            // with one material mu = nu rho, so (mu_i + mu_j) / (rho_i rho_j) = nu (1/rho_i + 1/rho_j)
+            float tv;
+            if (KIND == 2) {
+                tv = mnu_i * (irho_i + ir_j) * (f * r2) * ir2e;
+            } else {
+                const float qj = (ONEMAT ? nu_i : (float)P.mat[mat_j].nu) * rho_j;
+                tv = visc ? mj * (q_i + qj) * (f * r2) * (irho_i * ir_j * ir2e) : 0.0f;
+            }
             if (monaghan) {
                 // Monaghan: -m_j Pi_ij grad W, Pi_ij = -alpha cbar mu / rho_bar for approaching pairs
                 float acbar = acbar_i;
@@ -295,7 +312,10 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
                     acbar = (float)(0.5 * (mi.alpha + mj_.alpha)) * (float)(0.5 * (mi.c + mj_.c));
                 }
                 const float mu = q.havg * vr * ir2e;
-                const float pterm = mj * (acbar * mu) * (2.0f * __builtin_amdgcn_rcpf(rho_i + rho_j));
+                // 1 / rho_bar = 2 / (rho_i + rho_j) = 2 (1/rho_i)(1/rho_j) / (1/rho_i + 1/rho_j)
+                const float irbar = KIND == 2 ? 2.0f * irho_i * ir_j * __builtin_amdgcn_rcpf(irho_i + ir_j)
+                                              : 2.0f * __builtin_amdgcn_rcpf(rho_i + rho_j);
+                const float pterm = mj * (acbar * mu) * irbar;
                 sdw += (visc && vr < 0.0f) ? pterm : 0.0f;
             }
             const float sf = sdw * f;
@@ -325,22 +345,25 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Par
     if (Pt[e.x].nmat == 1) strain32_body<D, kStrain32Batch, true>(Pt[e.x], At[e.x], e.y);
     else strain32_body<D>(Pt[e.x], At[e.x], e.y);
 }
-template <int D, int G, bool COMPACT, bool ONEMAT>
+template <int D, int G, int KIND, bool ONEMAT>
 __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) momentum32_body<D, G, COMPACT, ONEMAT>(P, A, b);
+    if (b < nb) momentum32_body<D, G, KIND, ONEMAT>(P, A, b);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     const bool one = Pt[e.x].nmat == 1;
-    if (Pt[e.x].mcompact) {
-        if (one) momentum32_body<D, kMomentum32Batch, true, true>(Pt[e.x], At[e.x], e.y);
-        else momentum32_body<D, kMomentum32Batch, true, false>(Pt[e.x], At[e.x], e.y);
-    } else {
-        if (one) momentum32_body<D, kMomentum32Batch, false, true>(Pt[e.x], At[e.x], e.y);
-        else momentum32_body<D, kMomentum32Batch, false, false>(Pt[e.x], At[e.x], e.y);
+    switch (Pt[e.x].mcompact) {
+    case 2: momentum32_body<D, kMomentum32Batch, 2, true>(Pt[e.x], At[e.x], e.y); break;
+    case 1:
+        if (one) momentum32_body<D, kMomentum32Batch, 1, true>(Pt[e.x], At[e.x], e.y);
+        else momentum32_body<D, kMomentum32Batch, 1, false>(Pt[e.x], At[e.x], e.y);
+        break;
+    default:
+        if (one) momentum32_body<D, kMomentum32Batch, 0, true>(Pt[e.x], At[e.x], e.y);
+        else momentum32_body<D, kMomentum32Batch, 0, false>(Pt[e.x], At[e.x], e.y);
     }
 }
 
@@ -383,26 +406,29 @@ static void strain32_g(const Params &P, const Arrays &A, hipStream_t st)
     else strain32_gm<G, false>(P, A, st);
 }
 
-template <int G, bool C, bool ONEMAT>
+template <int G, int KIND, bool ONEMAT>
 static void momentum32_gc(const Params &P, const Arrays &A, hipStream_t st)
 {
     const dim3 grid(xcd_grid(nblk32(P.N))), blk(kBlock);
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL((k_momentum32<1, G, C, ONEMAT>), grid, blk, 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL((k_momentum32<2, G, C, ONEMAT>), grid, blk, 0, st, P, A); break;
-    default: hipLaunchKernelGGL((k_momentum32<3, G, C, ONEMAT>), grid, blk, 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_momentum32<1, G, KIND, ONEMAT>), grid, blk, 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL((k_momentum32<2, G, KIND, ONEMAT>), grid, blk, 0, st, P, A); break;
+    default: hipLaunchKernelGGL((k_momentum32<3, G, KIND, ONEMAT>), grid, blk, 0, st, P, A); break;
     }
 }
 
 template <int G>
 static void momentum32_g(const Params &P, const Arrays &A, hipStream_t st)
 {
-    if (P.mcompact) {
-        if (P.nmat == 1) momentum32_gc<G, true, true>(P, A, st);
-        else momentum32_gc<G, true, false>(P, A, st);
-    } else {
-        if (P.nmat == 1) momentum32_gc<G, false, true>(P, A, st);
-        else momentum32_gc<G, false, false>(P, A, st);
+    switch (P.mcompact) {
+    case 2: momentum32_gc<G, 2, true>(P, A, st); break;
+    case 1:
+        if (P.nmat == 1) momentum32_gc<G, 1, true>(P, A, st);
+        else momentum32_gc<G, 1, false>(P, A, st);
+        break;
+    default:
+        if (P.nmat == 1) momentum32_gc<G, 0, true>(P, A, st);
+        else momentum32_gc<G, 0, false>(P, A, st);
     }
 }
 

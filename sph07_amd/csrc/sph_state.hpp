@@ -101,9 +101,11 @@ struct Params {
     // storage order of the cells: 0 Morton (bit-interleaved code), 1 row-major
     // (z fastest); results do not depend on it, only the memory locality
     int corder;
-    // FP32c momentum record in 32 bytes: every material has one particle mass
-    // (mass32) and the linear EOS, so m and p/rho^2 follow from the material
-    // and the stored rho - rho0
+    // FP32c momentum record: 0 the 40-byte form; 1 32 bytes when every
+    // material has one particle mass (mass32) and the linear EOS, so m and
+    // p/rho^2 follow from the material and the stored rho - rho0; 2 32 bytes
+    // when besides there is one material and no rigid body: 1/rho and p/rho^2
+    // are stored (sph_rec32.hpp)
     int mcompact;
     float mass32[kMaxMat + 1];
     int btype[3];
