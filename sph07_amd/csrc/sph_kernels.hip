@@ -724,7 +724,10 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
 constexpr int kCellWaves = kBlock / 64;
 // candidate chunks held in registers per pass: 4 for the common one-material
 // search, 2 where per-candidate materials or distances take registers too
-template <bool ONEMAT, bool MIND> constexpr int nbr_blk() { return ONEMAT && !MIND ? 4 : 2; }
+#ifndef SPH_NBR_BLK
+#define SPH_NBR_BLK 4
+#endif
+template <bool ONEMAT, bool MIND> constexpr int nbr_blk() { return ONEMAT && !MIND ? SPH_NBR_BLK : 2; }
 
 __device__ __forceinline__ double readlane_d(double v, int l)
 {
@@ -825,14 +828,33 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         }
         if (kNbrBoxFilter) {
             // Drop the candidates farther than the largest cut-off from the
-            // cell's box (parts of the edge and corner cells): no target of
-            // the cell can reach them.  The margin is far above FP64 rounding
-            // of the box edges; order is kept (in-place compaction).
+            // box spanned by the cell's own particles (parts of the edge and
+            // corner cells, and of the faces where the particles do not fill
+            // the cell): no target of the cell can reach them.  The margin is
+            // far above FP64 rounding; order is kept (in-place compaction).
             double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                blo[d] = ctl->gmin[d] + bc[d] / ctl->gsize[d];
-                bhi[d] = ctl->gmin[d] + (bc[d] + 1) / ctl->gsize[d];
+                blo[d] = 1.0e300;
+                bhi[d] = -1.0e300;
+            }
+            for (int tg = c0; tg < c1; tg += 64) {
+                const int it = tg + lane < c1 ? tg + lane : c0;
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    const double xv = A.x[d][it];
+                    blo[d] = xv < blo[d] ? xv : blo[d];
+                    bhi[d] = xv > bhi[d] ? xv : bhi[d];
+                }
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    const double lo = __shfl_xor(blo[d], o, 64), hi = __shfl_xor(bhi[d], o, 64);
+                    blo[d] = lo < blo[d] ? lo : blo[d];
+                    bhi[d] = hi > bhi[d] ? hi : bhi[d];
+                }
             }
             const double reach2 = P.cut2max * (1.0 + 1.0e-9);
             int nkeep = 0, own_new = 0;
