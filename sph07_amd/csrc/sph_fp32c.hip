@@ -4,8 +4,10 @@
 // per-particle update stay FP64 (the same k_integrate / k_advance / rigid
 // kernels as the FP64 arm).  Only the bulk pair sums run in FP32: each
 // neighbour is one compact FP32 record (sph_rec32.hpp) holding its position
-// relative to its cell corner, so the pair separation is rebuilt as
-// off * cs32 + (r_i - r_j) without cancellation of large coordinates; the
+// relative to a cell corner, so the pair separation is rebuilt exactly from
+// small numbers (off * cs32 + (r_i - r_j) in the continuity sweep, R_i - R_j
+// less the nearest multiple of 2 cs32 in the momentum sweep) without
+// cancellation of large coordinates; the
 // kernel gradient, pressure, viscous and Monaghan terms are FP32 with hardware
 // reciprocal and reciprocal square root; each particle's sums are accumulated
 // with the TwoSum error-free transformation (Knuth 1969) into (sum,
