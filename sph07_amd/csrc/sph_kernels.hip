@@ -29,6 +29,10 @@ constexpr int kStrainBatch = SPH_STRAIN_BATCH, kMomentumBatch = SPH_MOMENTUM_BAT
 #define SPH_NBR_BOX_FILTER 1
 #endif
 constexpr bool kNbrBoxFilter = SPH_NBR_BOX_FILTER;   // cell-box filter of the staged candidates
+#ifndef SPH_NBR_FILT_BLK
+#define SPH_NBR_FILT_BLK 4
+#endif
+constexpr int kFiltBlk = SPH_NBR_FILT_BLK;           // box-filter chunks with loads in flight together
 
 #define CTL_STOP(ctl) ((ctl)->err != 0 || (ctl)->done != 0)
 
@@ -818,9 +822,17 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
         jn = __builtin_amdgcn_readfirstlane(A.cell_start[q + 1]);
         if (head < j0) continue;                            // cell owned by the previous wave
         const int bc[3] = {code_ext(q, m0), code_ext(q, m1), code_ext(q, m2)};
+        const int c0 = head, c1 = jn;
+        // the cell's own positions for the box filter, loaded ahead of the
+        // staging so that their latency overlaps it
+        double xo[D];
+        {
+            const int it = c0 + lane < c1 ? c0 + lane : c0;
+#pragma unroll
+            for (int d = 0; d < D; d++) xo[d] = A.x[d][it];
+        }
         int own;
         int ncand = stage_cell<D>(P, A, ctl, q, bc, lane, stage, own);
-        const int c0 = head, c1 = jn;
         if (ncand > kCandCap) {
             // left to k_nbr_ovf (per-particle walk)
             if (lane == 0) A.ovf[atomicAdd(&ctl->novf, 1)] = q;
@@ -835,10 +847,10 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                blo[d] = 1.0e300;
-                bhi[d] = -1.0e300;
+                blo[d] = xo[d];
+                bhi[d] = xo[d];
             }
-            for (int tg = c0; tg < c1; tg += 64) {
+            for (int tg = c0 + 64; tg < c1; tg += 64) {
                 const int it = tg + lane < c1 ? tg + lane : c0;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
@@ -858,36 +870,41 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             }
             const double reach2 = P.cut2max * (1.0 + 1.0e-9);
             int nkeep = 0, own_new = 0;
-            int j_n = lane < ncand ? stage[lane] : c0;
-            double xj_n[D];
+            // blocks of kFiltBlk chunks: every load of a block is issued
+            // before any is used.  The compacted writes of a block land below
+            // its first unread entry (a kept count never exceeds the entries
+            // read), and its reads precede its writes in program order.
+            for (int kb0 = 0; kb0 < ncand; kb0 += 64 * kFiltBlk) {
+                int jb[kFiltBlk];
+                double xb[kFiltBlk][D];
 #pragma unroll
-            for (int d = 0; d < D; d++) xj_n[d] = A.x[d][j_n];
-            for (int k0 = 0; k0 < ncand; k0 += 64) {
-                const int kk = k0 + lane;
-                const bool in = kk < ncand;
-                const int j = j_n;
-                double xjc[D];
+                for (int c = 0; c < kFiltBlk; c++) {
+                    const int kk = kb0 + c * 64 + lane;
+                    jb[c] = kk < ncand ? stage[kk] : c0;
 #pragma unroll
-                for (int d = 0; d < D; d++) xjc[d] = xj_n[d];
-                if (k0 + 64 < ncand) {          // next chunk's loads in flight during this one
-                    j_n = kk + 64 < ncand ? stage[kk + 64] : c0;
-#pragma unroll
-                    for (int d = 0; d < D; d++) xj_n[d] = A.x[d][j_n];
+                    for (int d = 0; d < D; d++) xb[c][d] = A.x[d][jb[c]];
                 }
-                double e2 = 0.0;
 #pragma unroll
-                for (int d = 0; d < D; d++) {
-                    const double xj = xjc[d];
-                    const double e = xj < blo[d] ? blo[d] - xj : (xj > bhi[d] ? xj - bhi[d] : 0.0);
-                    e2 += e * e;
+                for (int c = 0; c < kFiltBlk; c++) {
+                    const int k0 = kb0 + c * 64;
+                    if (k0 < ncand) {
+                        const bool in = k0 + lane < ncand;
+                        double e2 = 0.0;
+#pragma unroll
+                        for (int d = 0; d < D; d++) {
+                            const double xj = xb[c][d];
+                            const double e = xj < blo[d] ? blo[d] - xj : (xj > bhi[d] ? xj - bhi[d] : 0.0);
+                            e2 += e * e;
+                        }
+                        const unsigned long long km = __builtin_amdgcn_ballot_w64(in && e2 <= reach2);
+                        const int pos = nkeep + (int)__builtin_amdgcn_mbcnt_hi(
+                                                    (unsigned)(km >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
+                        if ((km >> lane) & 1ull) stage[pos] = jb[c];
+                        const int ol = own - k0;            // the cell's own particles are always kept
+                        if (ol >= 0 && ol < 64) own_new = nkeep + (int)__popcll(km & ((1ull << ol) - 1ull));
+                        nkeep += (int)__popcll(km);
+                    }
                 }
-                const unsigned long long km = __builtin_amdgcn_ballot_w64(in && e2 <= reach2);
-                const int pos = nkeep + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
-                                                                        __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
-                if ((km >> lane) & 1ull) stage[pos] = j;
-                const int ol = own - k0;            // the cell's own particles are always kept
-                if (ol >= 0 && ol < 64) own_new = nkeep + (int)__popcll(km & ((1ull << ol) - 1ull));
-                nkeep += (int)__popcll(km);
             }
             ncand = nkeep;
             own = own_new;
