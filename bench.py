@@ -100,6 +100,9 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for HBM counters)")
     ap.add_argument("--arms", default="fp64,fp32c", help="comma-separated arms to run")
+    ap.add_argument("--error-steps", type=int, default=1000,
+                    help="steps both arms have run (the timed ones included, the rest untimed) when the "
+                         "FP32c error against FP64 is measured")
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate, 1/s")
@@ -237,7 +240,12 @@ def main():
             if cnt:
                 print(f"bench:   {k:40s} {avg * 1e3:9.1f} us x {cnt}", file=sys.stderr, flush=True)
         arms[arm_name] = dict(scene=sc, elapsed=elapsed, t_init=t_init, kern=kern, rate=rate,
-                              cand=candidate_tests(sc.field(F_X), deck))
+                              cand=candidate_tests(sc.field(F_X), deck), pairs=float(sc.field(F_NNBR).sum()))
+        # untimed: on to the step count at which the arms are compared
+        ran = args.warmup + args.steps + args.profile_steps
+        if args.error_steps > ran:
+            assert sc.Solution(args.error_steps - ran) == args.error_steps - ran
+        arms[arm_name]["steps_run"] = max(ran, args.error_steps)
     # both arms advanced the same number of steps from the same deck
     moving = deck.dispbc == 0
     vel_err = rho_err = None
@@ -251,7 +259,7 @@ def main():
         # against the density deviation rho - rho0 (rho itself is ~rho0 everywhere)
         rho_err = float(np.linalg.norm(r32 - r64) / max(np.linalg.norm(r64 - rho0), 1e-300))
     main_arm = "fp32c" if "fp32c" in arms else "fp64"
-    pairs = float(arms[main_arm]["scene"].field(F_NNBR).sum())
+    pairs = arms[main_arm]["pairs"]
     kmean = pairs / N
     elapsed, kern, t_init = arms[main_arm]["elapsed"], arms[main_arm]["kern"], arms[main_arm]["t_init"]
     value = arms[main_arm]["rate"]
@@ -323,6 +331,7 @@ def main():
         "speedup_vs_fp64": value / arms["fp64"]["rate"] if "fp64" in arms else None,
         "fp32c_rel_l2_velocity_error_vs_fp64": vel_err,
         "fp32c_rel_l2_density_deviation_error_vs_fp64": rho_err,
+        "error_at_steps": arms[main_arm]["steps_run"],
         "neighbour_interactions_per_s": pairs * args.steps / elapsed,
         "init_s": round(t_init, 3),
         "kernels_avg_ms": kernels,
