@@ -29,6 +29,9 @@ constexpr int kStrainBatch = SPH_STRAIN_BATCH, kMomentumBatch = SPH_MOMENTUM_BAT
 #define SPH_NBR_BOX_FILTER 1
 #endif
 constexpr bool kNbrBoxFilter = SPH_NBR_BOX_FILTER;   // cell-box filter of the staged candidates
+#ifndef SPH_NBR_SMASK
+#define SPH_NBR_SMASK 1
+#endif
 #ifndef SPH_NBR_FILT_BLK
 #define SPH_NBR_FILT_BLK 4
 #endif
@@ -774,9 +777,19 @@ __device__ __forceinline__ void nbr_target_block(const Params &P, const Arrays &
                 d2 += dd * dd;
             }
             const double cut2 = ONEMAT ? cut2_0 : P.pc[matt][matcv[c]].cut2;
+#if SPH_NBR_SMASK
+            // lanes past the candidates and the target itself as a
+            // wave-uniform mask; the lane's own bit of the ballot is its test
+            const int nv = ncand - kb - c * 64, sl = selfk - c * 64;
+            const unsigned long long vm = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
+            const unsigned long long pm =
+                __builtin_amdgcn_ballot_w64(d2 < cut2) & vm & ~(sl >= 0 && sl < 64 ? 1ull << sl : 0ull);
+            const bool acc = __builtin_amdgcn_inverse_ballot_w64(pm);
+#else
             const int k = c * 64 + lane;
             const bool acc = d2 < cut2 && kb + k < ncand && k != selfk;
             const unsigned long long pm = __builtin_amdgcn_ballot_w64(acc);
+#endif
             if (acc) {
                 const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
@@ -888,18 +901,19 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
                 for (int c = 0; c < kFiltBlk; c++) {
                     const int k0 = kb0 + c * 64;
                     if (k0 < ncand) {
-                        const bool in = k0 + lane < ncand;
+                        const int nv = ncand - k0;          // lanes holding candidates
                         double e2 = 0.0;
 #pragma unroll
                         for (int d = 0; d < D; d++) {
-                            const double xj = xb[c][d];
-                            const double e = xj < blo[d] ? blo[d] - xj : (xj > bhi[d] ? xj - bhi[d] : 0.0);
+                            // distance outside the box on this axis (one term is positive at most)
+                            const double e = fmax(blo[d] - xb[c][d], 0.0) + fmax(xb[c][d] - bhi[d], 0.0);
                             e2 += e * e;
                         }
-                        const unsigned long long km = __builtin_amdgcn_ballot_w64(in && e2 <= reach2);
+                        const unsigned long long km = __builtin_amdgcn_ballot_w64(e2 <= reach2) &
+                                                      (nv >= 64 ? ~0ull : ((1ull << nv) - 1ull));
                         const int pos = nkeep + (int)__builtin_amdgcn_mbcnt_hi(
                                                     (unsigned)(km >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
-                        if ((km >> lane) & 1ull) stage[pos] = jb[c];
+                        if (__builtin_amdgcn_inverse_ballot_w64(km)) stage[pos] = jb[c];
                         const int ol = own - k0;            // the cell's own particles are always kept
                         if (ol >= 0 && ol < 64) own_new = nkeep + (int)__popcll(km & ((1ull << ol) - 1ull));
                         nkeep += (int)__popcll(km);
