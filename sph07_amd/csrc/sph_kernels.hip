@@ -29,6 +29,10 @@ constexpr int kStrainBatch = SPH_STRAIN_BATCH, kMomentumBatch = SPH_MOMENTUM_BAT
 #define SPH_NBR_BOX_FILTER 1
 #endif
 constexpr bool kNbrBoxFilter = SPH_NBR_BOX_FILTER;   // cell-box filter of the staged candidates
+#ifndef SPH_NBR_TPOS_LDS
+#define SPH_NBR_TPOS_LDS 1
+#endif
+constexpr bool kNbrTposLds = SPH_NBR_TPOS_LDS;       // targets' positions broadcast from LDS
 #ifndef SPH_NBR_SMASK
 #define SPH_NBR_SMASK 1
 #endif
@@ -757,11 +761,9 @@ __device__ __forceinline__ void nbr_target_block(const Params &P, const Arrays &
                                                  int nb, int ncand, int selfk, const double (&xi)[D], uint32_t fi,
                                                  const int (&cv)[NB], const double (&xcv)[NB][D],
                                                  const int (&matcv)[NB], double cut2_0,
-                                                 unsigned int cshift, unsigned int maxnbr, int &nn, double &mind)
+                                                 unsigned int cshift, unsigned int maxnbr, int &nn, double &mind,
+                                                 const double (&xt)[D])
 {
-    double xt[D];
-#pragma unroll
-    for (int d = 0; d < D; d++) xt[d] = readlane_d(xi[d], t);
     unsigned int nnt = (unsigned int)__builtin_amdgcn_readlane(nn, t);
     char *lst = reinterpret_cast<char *>(A.nbr);
     const unsigned int tbase = (unsigned int)(tg + t) * kNbrChunk;
@@ -816,7 +818,7 @@ __device__ __forceinline__ void nbr_target_block(const Params &P, const Arrays &
 
 template <int D, bool ONEMAT, bool MIND>
 __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A, int shift, int wave, int nwave,
-                                               int *stage)
+                                               int *stage, double *tpos)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
@@ -931,6 +933,12 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             double xi[D];
 #pragma unroll
             for (int d = 0; d < D; d++) xi[d] = A.x[d][it];
+            if (kNbrTposLds) {
+                // the targets' positions, read back as broadcasts by the target loop
+#pragma unroll
+                for (int d = 0; d < D; d++) tpos[d * 64 + lane] = xi[d];
+                __builtin_amdgcn_wave_barrier();
+            }
             const uint32_t fi = A.info[it];
             const double mi = A.m[it];
             int nn = 0;
@@ -952,9 +960,15 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
                     matcv[c] = ONEMAT ? 0 : info_mat(A.info[cv[c]]);
                 }
                 const int nb = min(NB, (ncand - kb + 63) >> 6);
-                for (int t = 0; t < nt; t++)
+                for (int t = 0; t < nt; t++) {
+                    // target t's position: a broadcast read from LDS (LDS
+                    // instructions, where readlane would take six VALU ones)
+                    double xt[D];
+#pragma unroll
+                    for (int d = 0; d < D; d++) xt[d] = kNbrTposLds ? tpos[d * 64 + t] : readlane_d(xi[d], t);
                     nbr_target_block<D, ONEMAT, MIND, NB>(P, A, t, tg, lane, kb, nb, ncand, self0 + t - kb, xi, fi, cv,
-                                                      xcv, matcv, cut2_0, cshift, maxnbr, nn, mind);
+                                                          xcv, matcv, cut2_0, cshift, maxnbr, nn, mind, xt);
+                }
             }
             if (tv) {
                 if (nn > P.maxnbr) {
@@ -1697,10 +1711,11 @@ template <int D, bool ONEMAT, bool MIND>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES, 8))) void k_nbr_cells(Params P, Arrays A, int shift)
 {
     __shared__ int stage[kCellWaves][kCandCap];
+    __shared__ double tpos[kCellWaves][kNbrTposLds ? 3 * 64 : 1];
     const int nb = (int)gridDim.x;
     const int b = xcd_block(nb);
     const int wv = (int)(threadIdx.x >> 6);
-    nbr_cells_body<D, ONEMAT, MIND>(P, A, shift, b * kCellWaves + wv, nb * kCellWaves, stage[wv]);
+    nbr_cells_body<D, ONEMAT, MIND>(P, A, shift, b * kCellWaves + wv, nb * kCellWaves, stage[wv], tpos[wv]);
 }
 
 // Batched cell kernel: block b serves scene m.x, as its block m.y of m.z
@@ -1710,14 +1725,15 @@ template <int D>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES, 8))) void kb_nbr_cells(const Params *Pt, const Arrays *At, const int4 *map, int shift)
 {
     __shared__ int stage[kCellWaves][kCandCap];
+    __shared__ double tpos[kCellWaves][kNbrTposLds ? 3 * 64 : 1];
     const int4 m = map[blockIdx.x];
     SCENE_OF(m.x);
     const int wv = (int)(threadIdx.x >> 6);
     const int w = m.y * kCellWaves + wv, nw = m.z * kCellWaves;
-    if (Pb.nmat == 1 && !Pb.want_mindist) nbr_cells_body<D, true, false>(Pb, Ab, shift, w, nw, stage[wv]);
-    else if (Pb.nmat == 1) nbr_cells_body<D, true, true>(Pb, Ab, shift, w, nw, stage[wv]);
-    else if (!Pb.want_mindist) nbr_cells_body<D, false, false>(Pb, Ab, shift, w, nw, stage[wv]);
-    else nbr_cells_body<D, false, true>(Pb, Ab, shift, w, nw, stage[wv]);
+    if (Pb.nmat == 1 && !Pb.want_mindist) nbr_cells_body<D, true, false>(Pb, Ab, shift, w, nw, stage[wv], tpos[wv]);
+    else if (Pb.nmat == 1) nbr_cells_body<D, true, true>(Pb, Ab, shift, w, nw, stage[wv], tpos[wv]);
+    else if (!Pb.want_mindist) nbr_cells_body<D, false, false>(Pb, Ab, shift, w, nw, stage[wv], tpos[wv]);
+    else nbr_cells_body<D, false, true>(Pb, Ab, shift, w, nw, stage[wv], tpos[wv]);
 }
 
 // grid of the cell kernel: about `slots` slots per wave (a few cells), so
