@@ -232,7 +232,7 @@ int alloc_particles(Context *c, int gcap)
         A.Nf = A.kappa = nullptr;
     }
     rc |= dalloc(c, L, &A.nnbr, c->N);
-    rc |= dalloc(c, L, &A.nbr, (size_t)nbr_round(c->maxnbr) * (size_t)P.nstride);
+    rc |= dalloc(c, L, &A.nbr, (size_t)nbr_rows(c->maxnbr) * (size_t)P.nstride);
     rc |= dalloc(c, L, &A.cell_of, c->N);
     rc |= dalloc(c, L, &A.occ, (size_t)c->N + 1);
     rc |= dalloc(c, L, &A.ovf, (size_t)c->N + 1);
@@ -1678,7 +1678,7 @@ long long sph_scene_footprint(const sph_scene_desc *d)
     // both sets, info in both sets, the two gather records, rho_new/p/prho2/vol,
     // the neighbour list
     // (the list is counted twice: its chunk stride is a power of two >= slots)
-    const long long per_slot = 32 * D + 8 * D + 32 + 8 + 144 + 32 + 8 * maxnbr;
+    const long long per_slot = 32 * D + 8 * D + 32 + 8 + 144 + 32 + 8 * ((long long)maxnbr + 16);
     // per real particle: a in both sets, orig, trd/mindist/critts, nnbr, sort
     // scratch, CSF fields
     const long long per_real = 16 * D + 8 + 24 + 4 + 12 + (d->surface_tension ? 16 * D + 16 : 0);

@@ -795,8 +795,10 @@ __device__ __forceinline__ void nbr_target_block(const Params &P, const Arrays &
             if (acc) {
                 const unsigned int pos = nnt + __builtin_amdgcn_mbcnt_hi((unsigned)(pm >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((unsigned)pm, 0u));
-                if (pos < maxnbr)
-                    *reinterpret_cast<int *>(lst + ((tbase + ((pos >> 3) << cshift) + (pos & 7u)) << 2)) = cv[c];
+                // entries past the list capacity (an overflow, reported by the
+                // caller) all land in the spare chunk row past it
+                const unsigned int ps = pos < maxnbr ? pos : maxnbr;
+                *reinterpret_cast<int *>(lst + ((tbase + ((ps >> 3) << cshift) + (ps & 7u)) << 2)) = cv[c];
                 if (MIND) {
                     const double r = sqrt(d2);
                     r1 = r < r1 ? r : r1;
@@ -827,7 +829,8 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
     const int j0 = (int)((long long)P.N * wave / nwave), j1 = (int)((long long)P.N * (wave + 1) / nwave);
     const int ncode = ctl->ncode;
     const unsigned int m0 = ctl->cmask[0], m1 = ctl->cmask[1], m2 = ctl->cmask[2];
-    const unsigned int maxnbr = (unsigned int)P.maxnbr;
+    // the spare row's first entry (lists hold nbr_round(maxnbr) + kNbrChunk entries)
+    const unsigned int maxnbr = (unsigned int)nbr_round(P.maxnbr);
     const unsigned int cshift = (unsigned int)P.nshift - 2u;       // log2(nstride * kNbrChunk)
     int jn = j0;
     while (jn < j1) {
@@ -1910,7 +1913,7 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
 
 bool cell_lists_fit(const Params &P)
 {
-    return (unsigned long long)P.nstride * (unsigned long long)nbr_round(P.maxnbr) * 4ull <= 0xffffffffull;
+    return (unsigned long long)P.nstride * (unsigned long long)nbr_rows(P.maxnbr) * 4ull <= 0xffffffffull;
 }
 
 int batch_cell_blocks(int n, int slots)

@@ -36,6 +36,9 @@ inline __host__ __device__ size_t nbr_slot(int k, int i, int ncap)
     return ((size_t)(k / kNbrChunk) * (size_t)ncap + (size_t)i) * kNbrChunk + (size_t)(k % kNbrChunk);
 }
 inline __host__ __device__ int nbr_round(int maxnbr) { return (maxnbr + kNbrChunk - 1) / kNbrChunk * kNbrChunk; }
+// entries per slot of the list memory: the rounded capacity and one spare
+// chunk, where the cell search stores entries past the capacity (an overflow)
+inline __host__ __device__ int nbr_rows(int maxnbr) { return nbr_round(maxnbr) + kNbrChunk; }
 
 // packed per-particle flags (info[])
 constexpr uint32_t kMatMask = 0xFFu;         // bits 0-7 material (1-based)
