@@ -232,7 +232,7 @@ __device__ __forceinline__ void zero_cells_body(const Params &P, const Arrays &A
 
 // MSolution::SetupLinkedList (MSolution.cpp:116-148): cell of each computable
 // particle (MNeighbourVars::GetBoxCoords :83-89, truncation, no clamp) and an
-// arbitrary slot inside the cell; the order inside a cell is fixed by k_bin_rank.
+// arbitrary slot inside the cell; the order inside a cell is fixed by k_permute.
 template <int D>
 __device__ __forceinline__ void bin_count_body(const Params &P, const Arrays &A, const int bid)
 {
@@ -288,25 +288,10 @@ __device__ __forceinline__ void bin_scatter_body(const Params &P, const Arrays &
     if (i >= P.N) return;
     const int c = A.cell_of[i];
     if (c < 0) return;
-    A.tmp[A.cell_start[c] + A.slot[i]] = i;
+    // the cell's members' deck numbers, the keys of their order
+    A.tmp[A.cell_start[c] + A.slot[i]] = A.orig[i];
 }
 
-// Order inside a cell: descending deck number, the order head insertion
-// leaves in the reference's linked lists (MSolution.cpp:143-144).  The
-// destination slot of particle i in the sorted storage goes to slot[i].
-__device__ __forceinline__ void bin_rank_body(const Params &P, const Arrays &A, const int bid)
-{
-    if (CTL_STOP(A.ctl)) return;
-    const int i = bid * kBlock + threadIdx.x;
-    if (i >= P.N) return;
-    const int c = A.cell_of[i];
-    if (c < 0) return;
-    const int s = A.cell_start[c], e = A.cell_start[c + 1];
-    const int oi = A.orig[i];
-    int r = 0;
-    for (int t = s; t < e; t++) r += A.orig[A.tmp[t]] > oi;
-    A.slot[i] = s + r;
-}
 
 // Moves every persistent per-particle field to its sorted slot (src -> dst
 // buffer set).  After an error or the stop test the move is the identity, so
@@ -317,7 +302,22 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const bool moving = !CTL_STOP(S.ctl);
-    const int j = moving ? S.slot[i] : i;
+    // Order inside a cell: descending deck number, the order head insertion
+    // leaves in the reference's linked lists (MSolution.cpp:143-144).  The
+    // destination slot is the cell's first slot plus the number of its
+    // members with a higher deck number (k_bin_scatter left their numbers in
+    // tmp; with an error the storage stays as it is).
+    int j = i;
+    if (moving) {
+        const int c = S.cell_of[i];     // < 0 only with the out-of-grid error set
+        if (c >= 0) {
+            const int s0 = S.cell_start[c], e0 = S.cell_start[c + 1];
+            const int oi = S.orig[i];
+            int r = 0;
+            for (int t = s0; t < e0; t++) r += S.tmp[t] > oi;
+            j = s0 + r;
+        }
+    }
 #pragma unroll
     for (int d = 0; d < D; d++) {
         T.x[d][j] = S.x[d][i];
@@ -1620,13 +1620,6 @@ __global__ __launch_bounds__(kBlock) void kb_bin_scatter(const Params *Pt, const
     SCENE_OF(e.x);
     bin_scatter_body(Pb, Ab, e.y);
 }
-__global__ __launch_bounds__(kBlock) void k_bin_rank(Params P, Arrays A) { bin_rank_body(P, A, blockIdx.x); }
-__global__ __launch_bounds__(kBlock) void kb_bin_rank(const Params *Pt, const Arrays *At, const int2 *map)
-{
-    const int2 e = map[blockIdx.x];
-    SCENE_OF(e.x);
-    bin_rank_body(Pb, Ab, e.y);
-}
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *in, const int *n_ptr, int *aux, const Ctl *ctl)
 {
     scan_reduce_body(in, n_ptr, aux, ctl, blockIdx.x);
@@ -1889,7 +1882,6 @@ void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st)
     DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_bin_count<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
     launch_scan(P, A, A.cell_count, A.cell_start, &A.ctl->nbucket, P.ncell_cap, st);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A);
-    hipLaunchKernelGGL(k_bin_rank, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A);
 }
 
 void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st)
@@ -2122,7 +2114,6 @@ void launch_batch_step(const BatchView &B, hipStream_t st)
     hipLaunchKernelGGL(kb_scan_aux, dim3(ns), dim3(kBlock), 0, st, B.P, B.A, B.scenes);
     hipLaunchKernelGGL(kb_scan_down, dim3(B.ntile_blocks), dim3(kBlock), 0, st, B.A, B.tmap);
     hipLaunchKernelGGL(kb_bin_scatter, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap);
-    hipLaunchKernelGGL(kb_bin_rank, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.pmap);
     DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_permute<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A, B.A2, B.pmap));
     const Arrays *A2 = B.A2;     // the sorted buffer set from here on
     if (B.any_periodic) {
