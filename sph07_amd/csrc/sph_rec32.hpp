@@ -75,6 +75,9 @@ __device__ __forceinline__ float sep32(float ri, float rj, float tcs, float itcs
 // inverse of the cell-code deposit: the cell coordinate held in the bits m of v
 __device__ __forceinline__ int code_ext(int v, unsigned int m)
 {
+    if (m == 0u) return 0;
+    if (((m + (m & (0u - m))) & m) == 0u)                       // one run of bits (row-major order)
+        return (int)(((unsigned int)v & m) >> __builtin_ctz(m));
     int r = 0, b = 0;
     while (m) {
         const unsigned int low = m & (0u - m);
