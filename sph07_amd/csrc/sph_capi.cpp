@@ -954,6 +954,7 @@ int build_graph(Context *c)
 // kernels wins.  Neither the order nor a shape changes a result (the lists
 // are in the reference's traversal order whatever the storage order), and
 // the following step re-sorts, so results are those of an untuned run.
+constexpr float kTuneMargin = 0.01f;
 int autotune(Context *c, int reps)
 {
     Params &P = c->P;
@@ -961,8 +962,10 @@ int autotune(Context *c, int reps)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
+    // one untimed launch (caches as the timed ones find them), then reps
     auto time_ms = [&](auto &&launch) -> float {
         float ms = 1e30f;
+        launch();
         if (hipEventRecord(e0, st) != hipSuccess) return ms;
         for (int r = 0; r < reps; r++) launch();
         if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return ms;
@@ -1004,13 +1007,15 @@ int autotune(Context *c, int reps)
         const Arrays &A = cur(c);
         Tune t = c->tune;
         t.corder = order;
+        // each shape's default first; another replaces it only when faster
+        // by more than kTuneMargin (timing noise must not pick a slower one)
         float tn = 1e30f;
-        for (int slots : {16, 32, 64, 128}) {
+        for (int slots : {32, 16, 64, 128}) {
             const float ms = time_ms([&] {
                 hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
                 launch_neighbours(P, A, 1, st, slots);
             });
-            if (ms < tn) { tn = ms; t.cell_slots = slots; }
+            if (ms < tn * (1.0f - kTuneMargin)) { tn = ms; t.cell_slots = slots; }
         }
         hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
         launch_neighbours(P, A, 1, st, t.cell_slots);     // lists of this order for the sweeps
@@ -1018,13 +1023,13 @@ int autotune(Context *c, int reps)
             hipMemcpyAsync(save + d * n, A.x[d], n * sizeof(double), hipMemcpyDeviceToDevice, st);
         float ts = 1e30f, tm = 1e30f;
         if (c->arm == SPH_ARM_FP32C) {
-            for (int g : {4, 8}) {
+            for (int g : {8, 4}) {
                 const float ms = time_ms([&] { launch_strain32(P, A, st, g); });
-                if (ms < ts) { ts = ms; t.strain32_batch = g; }
+                if (ms < ts * (1.0f - kTuneMargin)) { ts = ms; t.strain32_batch = g; }
             }
-            for (int g : {2, 4, 8}) {
+            for (int g : {8, 4, 2}) {
                 const float ms = time_ms([&] { launch_momentum32(P, A, st, g); });
-                if (ms < tm) { tm = ms; t.momentum32_batch = g; }
+                if (ms < tm * (1.0f - kTuneMargin)) { tm = ms; t.momentum32_batch = g; }
             }
         } else {
             ts = time_ms([&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st); });
@@ -1054,7 +1059,7 @@ int sph_solution(void *ctx, int max_steps, int *steps_done)
     Context *c = ctx_of(ctx);
     if (steps_done) *steps_done = 0;
     if (!c->tuned && c->N >= kAutotuneMinN && max_steps > 0) {
-        const int rc = autotune(c, 3);
+        const int rc = autotune(c, 5);
         if (rc) return rc;
     }
     if (sync_ctl(c)) return SPH_ERR_HIP;

@@ -570,8 +570,10 @@ __device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, c
             xs[d] = xi[d] - 0.5 * vi[d] * dt;
             A.xs[d][i] = xs[d];
         }
-        if (P.arm == 1) put_srec32<D>(A, ctl, i, xs, mi / A.rho[i], vi, fi, bc);
-        else put_srec<D>(A, i, xs, mi / A.rho[i], vi, fi);
+        // m / rho as k_permute (or the walk) computed it this step
+        const double vol = A.vol[i];
+        if (P.arm == 1) put_srec32<D>(A, ctl, i, xs, vol, vi, fi, bc);
+        else put_srec<D>(A, i, xs, vol, vi, fi);
     }
 }
 

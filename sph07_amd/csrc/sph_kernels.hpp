@@ -82,7 +82,7 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st);
 struct Tune {
     int cell_slots = 32;       // slots per wave of the cell-centric neighbour kernel
     int strain32_batch = 8;    // neighbour records in flight per lane, FP32c continuity
-    int momentum32_batch = 4;  // the same, FP32c momentum
+    int momentum32_batch = 8;  // the same, FP32c momentum
     int corder = 0;            // cell storage order: 0 Morton, 1 row-major
 };
 

@@ -217,9 +217,10 @@ class Scene:
         return {"cell_slots": shape[0], "strain32_batch": shape[1], "momentum32_batch": shape[2],
                 "cell_order": "row-major" if shape[3] == 1 else "morton"}
 
-    def set_launch_shape(self, cell_slots: int = 64, strain32_batch: int = 8, momentum32_batch: int = 4,
+    def set_launch_shape(self, cell_slots: int = 32, strain32_batch: int = 8, momentum32_batch: int = 8,
                          cell_order: str = "morton"):
-        """sph_set_launch_shape: fixes the launch shapes (no automatic tuning)."""
+        """sph_set_launch_shape: fixes the launch shapes (no automatic tuning);
+        the defaults are the library's untuned shapes."""
         L = lib()
         L.sph_set_launch_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
         shape = (ctypes.c_int32 * 4)(cell_slots, strain32_batch, momentum32_batch,
