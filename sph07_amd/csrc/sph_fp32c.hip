@@ -259,7 +259,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const float nu_i = (float)mi.nu;
         const float mnu_i = m_i * nu_i;
         const float acbar_i = (float)(mi.alpha * mi.c);     // alpha_bar * c_bar for equal materials
-        for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
+        for_each_nbr_batched<G, true>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
             const uint32_t fj = KIND == 1 ? __float_as_uint(rb.w) : (KIND == 0 ? __float_as_uint(rec.c.y) : 0u);
             const int mat_j = ONEMAT ? mat_i : rec_mat(fj);
