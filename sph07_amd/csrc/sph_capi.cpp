@@ -429,7 +429,7 @@ void fused_step(Context *c)
     const Arrays &A = cur(c);
     hipStream_t st = c->st;
     if (P.periodic) timed(c, K_IMAGES, [&] { launch_ghost_setup(P, A, st); });
-    timed(c, K_NBR, [&] { launch_neighbours(P, A, 1, st, c->tune.cell_slots); });
+    timed(c, K_NBR, [&] { launch_neighbours(P, A, st, c->tune.cell_slots); });
     if (c->arm == SPH_ARM_FP32C)
         timed(c, K_STRAIN, [&] { launch_strain32(P, A, st, c->tune.strain32_batch); });
     else
@@ -739,7 +739,7 @@ int create_impl(Context *c, const sph_scene_desc *d)
     // ---- first neighbours (no images), critical step, first dt (:71-82) ----
     rc = run_phase(c, [&] {
         sort_step(c);
-        launch_neighbours(c->P, cur(c), 0, c->st);
+        launch_neighbours(c->P, cur(c), c->st);
         launch_strain(c->P, cur(c), M_CRIT, c->st);
         launch_crit_fold(c->P, cur(c), c->st);
     });
@@ -846,7 +846,7 @@ int sph_ghost_setup(void *ctx)
 int sph_linked_list_neighbours(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_neighbours(c->P, cur(c), 0, c->st); });
+    return run_phase(c, [&] { launch_neighbours(c->P, cur(c), c->st); });
 }
 
 int sph_strain(void *ctx)
@@ -1013,12 +1013,12 @@ int autotune(Context *c, int reps)
         for (int slots : {32, 16, 64, 128}) {
             const float ms = time_ms([&] {
                 hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
-                launch_neighbours(P, A, 1, st, slots);
+                launch_neighbours(P, A, st, slots);
             });
             if (ms < tn * (1.0f - kTuneMargin)) { tn = ms; t.cell_slots = slots; }
         }
         hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
-        launch_neighbours(P, A, 1, st, t.cell_slots);     // lists of this order for the sweeps
+        launch_neighbours(P, A, st, t.cell_slots);     // lists of this order for the sweeps
         for (int d = 0; d < c->ndim; d++)
             hipMemcpyAsync(save + d * n, A.x[d], n * sizeof(double), hipMemcpyDeviceToDevice, st);
         float ts = 1e30f, tm = 1e30f;

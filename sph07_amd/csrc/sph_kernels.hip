@@ -338,6 +338,29 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
     // cell code of every sorted slot (the inactive bucket is ncode)
     T.occ[j] = S.cell_of[i];
     if (!(S.info[i] & kActive)) T.nnbr[j] = 0;
+    {
+        // MSolution::Strain's half step back (:334-338) and the continuity
+        // record, written with the move (the neighbour search reads x only)
+        const Ctl *ctl = S.ctl;
+        const uint32_t fi = S.info[i];
+        const bool act = fi & kActive;
+        const double dt = ctl->dt;
+        double xs[D], vi[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            vi[d] = S.v[d][i];
+            xs[d] = act ? S.x[d][i] - 0.5 * vi[d] * dt : S.x[d][i];
+            T.xs[d][j] = xs[d];
+        }
+        const double vol = S.m[i] / S.rho[i];
+        if (P.arm == 1) {
+            const int c = S.cell_of[i];
+            const int bc[3] = {code_ext(c, ctl->cmask[0]), code_ext(c, ctl->cmask[1]), code_ext(c, ctl->cmask[2])};
+            put_srec32<D>(T, ctl, j, xs, vol, vi, fi, bc);
+        } else {
+            put_srec<D>(T, j, xs, vol, vi, fi);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -548,38 +571,22 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
 // MSolution::LinkedListNeighbours (MSolution.cpp:150-215): cells r(x) ->
 // s(y) -> t(z) ascending, in each cell images (newest first) then real
 // particles (descending number); strict d^2 < (2*AVG(h_i,h_j))^2.
-// With shift != 0 the epilogue also writes the half-step-back positions of
-// MSolution::Strain (:334-338) for the fused step.
+// The half step back of MSolution::Strain (:334-338) and the continuity
+// records are written by k_permute, with the move into cell order.
 // ---------------------------------------------------------------------------
 template <int D>
-__device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, const Ctl *ctl, int i,
-                                             const double *xi, double mi, uint32_t fi, int nn, double mind,
-                                             int shift, const int *bc)
+__device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, int i, int nn, double mind)
 {
     A.nnbr[i] = nn;
     // pad the last chunk with the particle's own index: the batched sweeps
     // load whole chunks without a per-entry bound
     for (int k = nn; k & (kNbrChunk - 1); k++) A.nbr[nbr_slot(k, i, P.nstride)] = i;
     if (P.want_mindist) A.mindist[i] = mind;
-    if (shift) {
-        const double dt = ctl->dt;
-        double xs[D], vi[D];
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            vi[d] = A.v[d][i];
-            xs[d] = xi[d] - 0.5 * vi[d] * dt;
-            A.xs[d][i] = xs[d];
-        }
-        // m / rho as k_permute (or the walk) computed it this step
-        const double vol = A.vol[i];
-        if (P.arm == 1) put_srec32<D>(A, ctl, i, xs, vol, vi, fi, bc);
-        else put_srec<D>(A, i, xs, vol, vi, fi);
-    }
 }
 
 // one particle's walk (the batched path, and the fallback of the cell kernel)
 template <int D>
-__device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A, int shift, const int i)
+__device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A, const int i)
 {
     Ctl *ctl = A.ctl;
     const uint32_t fi = A.info[i];
@@ -711,16 +718,16 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
         A.nnbr[i] = nn;
         return;
     }
-    nbr_epilogue<D>(P, A, ctl, i, xi, mi, fi, nn, mind, shift, bc);
+    nbr_epilogue<D>(P, A, i, nn, mind);
 }
 
 template <int D>
-__device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A, int shift, const int bid)
+__device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A, const int bid)
 {
     if (CTL_STOP(A.ctl)) return;
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
-    neighbours_one<D>(P, A, shift, i);
+    neighbours_one<D>(P, A, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -821,7 +828,7 @@ __device__ __forceinline__ void nbr_target_block(const Params &P, const Arrays &
 }
 
 template <int D, bool ONEMAT, bool MIND>
-__device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A, int shift, int wave, int nwave,
+__device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A, int wave, int nwave,
                                                int *stage, double *tpos)
 {
     Ctl *ctl = A.ctl;
@@ -980,7 +987,7 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
                     set_err(ctl, kErrNbrOverflow, A.orig[it]);
                     A.nnbr[it] = P.maxnbr;
                 } else {
-                    nbr_epilogue<D>(P, A, ctl, it, xi, mi, fi, nn, mind, shift, bc);
+                    nbr_epilogue<D>(P, A, it, nn, mind);
                 }
             }
         }
@@ -1699,28 +1706,32 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Para
     const int nb_ = ((P).N + kBlock - 1) / kBlock;       \
     const int bid_ = xcd_block(nb_);                     \
     if (bid_ >= nb_) return;
-template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A, int shift) { XCD_BID(P); neighbours_body<D>(P, A, shift, bid_); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P, Arrays A) { XCD_BID(P); neighbours_body<D>(P, A, bid_); }
 // The grid is a multiple of 8 blocks; each wave takes an equal share of the
-// slots.  At most 77 VGPRs (6 waves per SIMD) without spills.
+// slots.  The one-material search without distances fits 72 VGPRs (7 waves
+// per SIMD) without spills.
 #ifndef SPH_NBR_WAVES
-#define SPH_NBR_WAVES 6
+#define SPH_NBR_WAVES 7
 #endif
+// the variants with per-candidate materials or distances, and the batched
+// kernel that holds all four, need more registers: one wave fewer
+template <bool ONEMAT, bool MIND> constexpr int nbr_waves() { return ONEMAT && !MIND ? SPH_NBR_WAVES : SPH_NBR_WAVES - 1; }
 template <int D, bool ONEMAT, bool MIND>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES, 8))) void k_nbr_cells(Params P, Arrays A, int shift)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(nbr_waves<ONEMAT, MIND>(), 8))) void k_nbr_cells(Params P, Arrays A)
 {
     __shared__ int stage[kCellWaves][kCandCap];
     __shared__ double tpos[kCellWaves][kNbrTposLds ? 3 * 64 : 1];
     const int nb = (int)gridDim.x;
     const int b = xcd_block(nb);
     const int wv = (int)(threadIdx.x >> 6);
-    nbr_cells_body<D, ONEMAT, MIND>(P, A, shift, b * kCellWaves + wv, nb * kCellWaves, stage[wv], tpos[wv]);
+    nbr_cells_body<D, ONEMAT, MIND>(P, A, b * kCellWaves + wv, nb * kCellWaves, stage[wv], tpos[wv]);
 }
 
 // Batched cell kernel: block b serves scene m.x, as its block m.y of m.z
 // (the scene's waves split its slots as in k_nbr_cells); the variant is
 // chosen per scene (uniform per block).
 template <int D>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES, 8))) void kb_nbr_cells(const Params *Pt, const Arrays *At, const int4 *map, int shift)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES - 1, 8))) void kb_nbr_cells(const Params *Pt, const Arrays *At, const int4 *map)
 {
     __shared__ int stage[kCellWaves][kCandCap];
     __shared__ double tpos[kCellWaves][kNbrTposLds ? 3 * 64 : 1];
@@ -1728,10 +1739,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_
     SCENE_OF(m.x);
     const int wv = (int)(threadIdx.x >> 6);
     const int w = m.y * kCellWaves + wv, nw = m.z * kCellWaves;
-    if (Pb.nmat == 1 && !Pb.want_mindist) nbr_cells_body<D, true, false>(Pb, Ab, shift, w, nw, stage[wv], tpos[wv]);
-    else if (Pb.nmat == 1) nbr_cells_body<D, true, true>(Pb, Ab, shift, w, nw, stage[wv], tpos[wv]);
-    else if (!Pb.want_mindist) nbr_cells_body<D, false, false>(Pb, Ab, shift, w, nw, stage[wv], tpos[wv]);
-    else nbr_cells_body<D, false, true>(Pb, Ab, shift, w, nw, stage[wv], tpos[wv]);
+    if (Pb.nmat == 1 && !Pb.want_mindist) nbr_cells_body<D, true, false>(Pb, Ab, w, nw, stage[wv], tpos[wv]);
+    else if (Pb.nmat == 1) nbr_cells_body<D, true, true>(Pb, Ab, w, nw, stage[wv], tpos[wv]);
+    else if (!Pb.want_mindist) nbr_cells_body<D, false, false>(Pb, Ab, w, nw, stage[wv], tpos[wv]);
+    else nbr_cells_body<D, false, true>(Pb, Ab, w, nw, stage[wv], tpos[wv]);
 }
 
 // grid of the cell kernel: about `slots` slots per wave (a few cells), so
@@ -1746,7 +1757,7 @@ static int cell_blocks(int n, int slots)
 // cells whose stencil did not fit the staging space: one block per cell,
 // the per-particle walk for each of its particles
 constexpr int kOvfBlocks = 256;
-template <int D> __global__ __launch_bounds__(kBlock) void k_nbr_ovf(Params P, Arrays A, int shift)
+template <int D> __global__ __launch_bounds__(kBlock) void k_nbr_ovf(Params P, Arrays A)
 {
     const Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
@@ -1754,12 +1765,12 @@ template <int D> __global__ __launch_bounds__(kBlock) void k_nbr_ovf(Params P, A
     for (int w = blockIdx.x; w < n; w += gridDim.x) {
         const int q = A.ovf[w];
         const int c0 = A.cell_start[q], c1 = A.cell_start[q + 1];
-        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(P, A, shift, i);
+        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(P, A, i);
     }
 }
 // overflow cells of the batched cell kernel: kOvfBlocksPerScene blocks per scene
 constexpr int kOvfBlocksPerScene = 8;
-template <int D> __global__ __launch_bounds__(kBlock) void kb_nbr_ovf(const Params *Pt, const Arrays *At, const int *sc, int shift)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_nbr_ovf(const Params *Pt, const Arrays *At, const int *sc)
 {
     const int s = sc[blockIdx.x / kOvfBlocksPerScene], part = (int)blockIdx.x % kOvfBlocksPerScene;
     SCENE_OF(s);
@@ -1769,14 +1780,14 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_nbr_ovf(const Para
     for (int w = part; w < n; w += kOvfBlocksPerScene) {
         const int q = Ab.ovf[w];
         const int c0 = Ab.cell_start[q], c1 = Ab.cell_start[q + 1];
-        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(Pb, Ab, shift, i);
+        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(Pb, Ab, i);
     }
 }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *Pt, const Arrays *At, const int2 *map, int shift)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
-    neighbours_body<D>(Pb, Ab, shift, e.y);
+    neighbours_body<D>(Pb, Ab, e.y);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_shift(Params P, Arrays A) { shift_body<D>(P, A, blockIdx.x); }
 template <int D> __global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask) { XCD_BID(P); strain_body<D>(P, A, mask, bid_); }
@@ -1917,25 +1928,24 @@ int batch_cell_blocks(int n, int slots)
     return (int)(nb < 1 ? 1 : nb);
 }
 
-void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots)
+void launch_neighbours(const Params &P, const Arrays &A, hipStream_t st, int cell_slots)
 {
     if (!cell_lists_fit(P)) {
         // the cell kernel's list byte offsets are 32-bit
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A,
-                                              shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A));
         return;
     }
     const dim3 grid(cell_blocks(P.N, cell_slots)), blk(kBlock);
     if (P.nmat == 1 && !P.want_mindist) {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true, false>), grid, blk, 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true, false>), grid, blk, 0, st, P, A));
     } else if (P.nmat == 1) {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true, true>), grid, blk, 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true, true>), grid, blk, 0, st, P, A));
     } else if (!P.want_mindist) {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false, false>), grid, blk, 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false, false>), grid, blk, 0, st, P, A));
     } else {
-        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false, true>), grid, blk, 0, st, P, A, shift));
+        DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, false, true>), grid, blk, 0, st, P, A));
     }
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_nbr_ovf<DD>, dim3(kOvfBlocks), dim3(kBlock), 0, st, P, A, shift));
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_nbr_ovf<DD>, dim3(kOvfBlocks), dim3(kBlock), 0, st, P, A));
 }
 
 void launch_shift(const Params &P, const Arrays &A, hipStream_t st)
@@ -2129,11 +2139,11 @@ void launch_batch_step(const BatchView &B, hipStream_t st)
             }
     }
     if (B.ncell_blocks > 0) {
-        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_cells<DD>, dim3(B.ncell_blocks), dim3(kBlock), 0, st, B.P, A2, B.cmap, 1));
-        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_ovf<DD>, dim3(ns * kOvfBlocksPerScene), dim3(kBlock), 0, st, B.P, A2, B.scenes, 1));
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_cells<DD>, dim3(B.ncell_blocks), dim3(kBlock), 0, st, B.P, A2, B.cmap));
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_ovf<DD>, dim3(ns * kOvfBlocksPerScene), dim3(kBlock), 0, st, B.P, A2, B.scenes));
     }
     if (B.nwalk_blocks > 0)
-        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_neighbours<DD>, dim3(B.nwalk_blocks), dim3(kBlock), 0, st, B.P, A2, B.wmap, 1));
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_neighbours<DD>, dim3(B.nwalk_blocks), dim3(kBlock), 0, st, B.P, A2, B.wmap));
     if (B.arm == 1) launch_batch_strain32(B, st);
     else DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_strain<DD>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, A2, B.pmap, M_STRAIN | M_CONST | M_CRIT));
     if (B.any_st) {

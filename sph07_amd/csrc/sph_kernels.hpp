@@ -86,7 +86,7 @@ struct Tune {
     int corder = 0;            // cell storage order: 0 Morton, 1 row-major
 };
 
-void launch_neighbours(const Params &P, const Arrays &A, int shift, hipStream_t st, int cell_slots = 32);
+void launch_neighbours(const Params &P, const Arrays &A, hipStream_t st, int cell_slots = 32);
 void launch_shift(const Params &P, const Arrays &A, hipStream_t st);
 void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st);
 void launch_normals(const Params &P, const Arrays &A, hipStream_t st);
