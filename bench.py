@@ -37,8 +37,9 @@ def alg_bytes_3d(group, arm, k, compact=False):
     material, linear EOS) rather than the 40-byte one."""
     rec_s, rec_m = ((32, 32 if compact else 40) if arm == "fp32c" else (64, 80))
     if group == "k_neighbours":
-        # read x, info, m, rho, v; write nnbr, list, xs, continuity record
-        return 24 + 4 + 8 + 8 + 24 + 4 + 4 * k + 24 + rec_s
+        # read x, info; write nnbr, list (the half-step positions and the
+        # continuity record are written by k_permute, in the bin group)
+        return 24 + 4 + 4 + 4 * k
     if group == "k_strain":
         # read info, own record (fp32c) or xs+v, nnbr, list, rho, m, a (critical
         # step), cell code (fp32c); write trd, x, p, p/rho^2, rho_new, momentum record
@@ -52,11 +53,12 @@ def alg_bytes_3d(group, arm, k, compact=False):
         # read info, a, x, v, rho_new; write a, v, x, rho
         return 4 + 24 + 24 + 24 + 8 + 24 + 24 + 24 + 8
     if group.startswith("bin"):
-        # count: read x, info, write cell, slot; scatter: read cell, slot, write
-        # tmp; rank: read tmp, orig, write slot; permute: read x, v, a, rho, m,
-        # info, orig, slot, cell and write them sorted plus vol, cell code, nnbr
-        return (24 + 4 + 4 + 4) + (4 + 4 + 4) + (4 + 4 + 4) + (24 * 3 + 8 + 8 + 4 + 4 + 4 + 4) + \
-            (24 * 3 + 8 + 8 + 4 + 4 + 8 + 4 + 4)
+        # count: read x, info, write cell, slot; scatter: read cell, slot,
+        # orig, write the cell's keys; permute: read x, v, a, rho, m, info,
+        # orig, cell, key and write x, v, a, rho, m, vol, info, orig, cell
+        # code, nnbr, the half-step positions and the continuity record
+        return (24 + 4 + 4 + 4) + (4 + 4 + 4 + 4) + (24 * 3 + 8 + 8 + 4 + 4 + 4 + 4) + \
+            (24 * 3 + 8 + 8 + 8 + 4 + 4 + 4 + 4 + 24 + rec_s)
     return 0
 
 
