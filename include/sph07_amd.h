@@ -260,9 +260,9 @@ int  sph_write_state_dump(void *ctx, const char *path);
 /* --- autotuner (no reference counterpart) ---
    Launch shapes that change timing but never results: shape[0] = slots per
    wave of the cell-centric neighbour search, shape[1] / shape[2] = neighbour
-   records in flight per lane in the FP32c continuity / momentum sweeps,
-   shape[3] = cell storage order (0 Morton, 1 row-major).  shape holds 4
-   entries.
+   records in flight per lane in the continuity / momentum sweeps of the
+   scene's arm (FP64 or FP32c), shape[3] = cell storage order (0 Morton,
+   1 row-major).  shape holds 4 entries.
    reps > 0 times every alternative reps times on the scene's current state
    and keeps the fastest (scenes of 65536 particles or more are tuned this way
    on their first sph_solution call); reps == 0 keeps the current shapes and

@@ -68,8 +68,11 @@ struct Context {
     std::vector<void *> part_allocs, cell_allocs;
     long long *canon = nullptr;
     Timing tm;
-    // captured step sequence (hipGraph); rebuilt after any buffer change
-    hipGraphExec_t gexec = nullptr;
+    // captured step sequences (hipGraphs), one per starting buffer set: the
+    // kernels' arguments name the buffer sets, and a step moves the state to
+    // the other set, so a graph is only valid from the set it was captured
+    // from.  Both are dropped after any buffer or launch-shape change.
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int gsteps = 0;
     bool use_graph = true;
     // launch shapes (timing only, never results) and whether they were tuned
@@ -84,8 +87,10 @@ inline Arrays &cur(Context *c) { return c->AA[c->cur]; }
 
 void drop_graph(Context *c)
 {
-    if (c->gexec) hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
+    for (hipGraphExec_t &g : c->gexec) {
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
     c->gsteps = 0;
 }
 
@@ -431,17 +436,17 @@ void fused_step(Context *c)
     if (P.periodic) timed(c, K_IMAGES, [&] { launch_ghost_setup(P, A, st); });
     timed(c, K_NBR, [&] { launch_neighbours(P, A, st, c->tune.cell_slots); });
     if (c->arm == SPH_ARM_FP32C)
-        timed(c, K_STRAIN, [&] { launch_strain32(P, A, st, c->tune.strain32_batch); });
+        timed(c, K_STRAIN, [&] { launch_strain32(P, A, st, c->tune.strain_batch); });
     else
-        timed(c, K_STRAIN, [&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st); });
+        timed(c, K_STRAIN, [&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st, c->tune.strain_batch); });
     if (P.st) timed(c, K_NORMALS, [&] { launch_normals(P, A, st); });
     timed(c, K_ADVANCE, [&] { launch_advance(P, A, st); });
     timed(c, K_MOMENTUM, [&] {
         if (c->arm == SPH_ARM_FP32C) {
-            launch_momentum32(P, A, st, c->tune.momentum32_batch);
+            launch_momentum32(P, A, st, c->tune.momentum_batch);
             launch_rigid_reduce(P, A, 1, st);
         } else {
-            launch_momentum(P, A, 1, st);
+            launch_momentum(P, A, 1, st, c->tune.momentum_batch);
         }
     });
     timed(c, K_INTEGRATE, [&] { launch_integrate(P, A, M_BC | M_VEL | M_MOVE | M_RHO, st); });
@@ -793,6 +798,10 @@ int sph_create(const sph_scene_desc *desc, int arm, void **out, char *err, int e
     if (!desc || !out) return SPH_ERROR;
     Context *c = new Context();
     c->arm = arm;
+    if (arm == SPH_ARM_FP64) {
+        c->tune.strain_batch = kStrain64Batch;
+        c->tune.momentum_batch = kMomentum64Batch;
+    }
     const bool arm_ok = arm == SPH_ARM_FP64 || arm == SPH_ARM_FP32C;
     int rc = arm_ok ? create_impl(c, desc) : SPH_ERR_UNSUPPORTED;
     if (!arm_ok) c->err = "unknown arithmetic arm";
@@ -924,19 +933,22 @@ int sph_end_step(void *ctx)
     return run_phase(c, [&] { launch_finish(c->P, cur(c), 0, 1, c->st); });
 }
 
-// Captures kGraphSteps fused steps on the context's stream into one graph.
+// Captures kGraphSteps fused steps on the context's stream into the graph of
+// the current buffer set (an even step count ends on the set it starts from).
 constexpr int kGraphSteps = 16;
+static_assert(kGraphSteps % 2 == 0, "a graph ends on the buffer set it starts from");
 
 int build_graph(Context *c)
 {
     hipGraph_t g = nullptr;
+    const int from = c->cur;
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < kGraphSteps; k++) fused_step(c);
     HIPCHK(c, hipStreamEndCapture(c->st, &g));
-    hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&c->gexec[from], g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) {
-        c->gexec = nullptr;
+        c->gexec[from] = nullptr;
         c->use_graph = false;   // fall back to direct launches of the same kernels
         return 0;
     }
@@ -977,9 +989,12 @@ int autotune(Context *c, int reps)
         hipEventDestroy(e1);
         return rc;
     };
+    // positions (the continuity sweep moves them a half step) and
+    // accelerations (the momentum sweep rewrites them; the next step's
+    // critical step reads them) are saved and put back
     const size_t n = (size_t)c->N;
     double *save = nullptr;
-    if (hipMalloc((void **)&save, n * sizeof(double) * c->ndim) != hipSuccess) {
+    if (hipMalloc((void **)&save, n * sizeof(double) * 2 * c->ndim) != hipSuccess) {
         c->err = "ERROR: autotuner allocation failed.";
         return finish(SPH_ERR_HIP);
     }
@@ -1019,24 +1034,40 @@ int autotune(Context *c, int reps)
         }
         hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
         launch_neighbours(P, A, st, t.cell_slots);     // lists of this order for the sweeps
-        for (int d = 0; d < c->ndim; d++)
+        for (int d = 0; d < c->ndim; d++) {
             hipMemcpyAsync(save + d * n, A.x[d], n * sizeof(double), hipMemcpyDeviceToDevice, st);
-        float ts = 1e30f, tm = 1e30f;
-        if (c->arm == SPH_ARM_FP32C) {
-            for (int g : {8, 4}) {
-                const float ms = time_ms([&] { launch_strain32(P, A, st, g); });
-                if (ms < ts * (1.0f - kTuneMargin)) { ts = ms; t.strain32_batch = g; }
-            }
-            for (int g : {8, 4, 2}) {
-                const float ms = time_ms([&] { launch_momentum32(P, A, st, g); });
-                if (ms < tm * (1.0f - kTuneMargin)) { tm = ms; t.momentum32_batch = g; }
-            }
-        } else {
-            ts = time_ms([&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st); });
-            tm = time_ms([&] { launch_momentum(P, A, 0, st); });
+            hipMemcpyAsync(save + (c->ndim + d) * n, A.a[d], n * sizeof(double), hipMemcpyDeviceToDevice, st);
         }
-        for (int d = 0; d < c->ndim; d++)
+        // records in flight per lane, the arm's default first
+        const bool fp32c = c->arm == SPH_ARM_FP32C;
+        const int gs_def = fp32c ? 8 : kStrain64Batch, gm_def = fp32c ? 8 : kMomentum64Batch;
+        t.strain_batch = gs_def;
+        t.momentum_batch = gm_def;
+        float ts = 1e30f, tm = 1e30f;
+        for (int g : {gs_def, 2, 4, 8}) {
+            if (g == gs_def && ts < 1e30f) continue;
+            if (fp32c && g == 2) continue;
+            const float ms = time_ms([&] {
+                if (fp32c) launch_strain32(P, A, st, g);
+                else launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st, g);
+            });
+            if (ms < ts * (1.0f - kTuneMargin)) { ts = ms; t.strain_batch = g; }
+        }
+        for (int g : {gm_def, 2, 4, 8}) {
+            if (g == gm_def && tm < 1e30f) continue;
+            const float ms = time_ms([&] {
+                if (fp32c) launch_momentum32(P, A, st, g);
+                else launch_momentum(P, A, -1, st, g);
+            });
+            if (ms < tm * (1.0f - kTuneMargin)) { tm = ms; t.momentum_batch = g; }
+        }
+        for (int d = 0; d < c->ndim; d++) {
             hipMemcpyAsync(A.x[d], save + d * n, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+            hipMemcpyAsync(A.a[d], save + (c->ndim + d) * n, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+        }
+        // the timed continuity sweeps folded their critical steps into the
+        // reduction slots: the step that follows starts from empty slots
+        launch_crit_reset(A, st);
         if (hipStreamSynchronize(st) != hipSuccess) { rc = SPH_ERR_HIP; break; }
         if (tn + ts + tm < best_total) {
             best_total = tn + ts + tm;
@@ -1070,10 +1101,10 @@ int sph_solution(void *ctx, int max_steps, int *steps_done)
         const int chunk = std::min(remaining, 64);
         int k = 0;
         if (c->use_graph && !c->tm.on && chunk >= kGraphSteps) {
-            if (!c->gexec && build_graph(c)) return SPH_ERR_HIP;
-            if (c->gexec)
+            if (!c->gexec[c->cur] && build_graph(c)) return SPH_ERR_HIP;
+            if (c->gexec[c->cur])
                 for (; k + c->gsteps <= chunk; k += c->gsteps)
-                    HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+                    HIPCHK(c, hipGraphLaunch(c->gexec[c->cur], c->st));
         }
         for (; k < chunk; k++) fused_step(c);
         hipError_t e = hipGetLastError();
@@ -1108,8 +1139,8 @@ int sph_set_launch_shape(void *ctx, const int32_t *shape)
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->tune.cell_slots = slots;
-    c->tune.strain32_batch = gs;
-    c->tune.momentum32_batch = gm;
+    c->tune.strain_batch = gs;
+    c->tune.momentum_batch = gm;
     c->tune.corder = order;
     c->P.corder = order;
     c->tuned = true;
@@ -1125,8 +1156,8 @@ int sph_autotune(void *ctx, int reps, int32_t *shape)
     else if (reps == 0) c->tuned = true;
     if (shape) {
         shape[0] = c->tune.cell_slots;
-        shape[1] = c->tune.strain32_batch;
-        shape[2] = c->tune.momentum32_batch;
+        shape[1] = c->tune.strain_batch;
+        shape[2] = c->tune.momentum_batch;
         shape[3] = c->tune.corder;
     }
     return rc;

@@ -12,19 +12,10 @@
 
 #include <stdlib.h>
 
-// neighbour records in flight per lane in the FP64 sweeps
-#ifndef SPH_STRAIN_BATCH
-#define SPH_STRAIN_BATCH 4
-#endif
-#ifndef SPH_MOMENTUM_BATCH
-#define SPH_MOMENTUM_BATCH 2
-#endif
-
 #include <math.h>
 
 namespace sph {
 
-constexpr int kStrainBatch = SPH_STRAIN_BATCH, kMomentumBatch = SPH_MOMENTUM_BATCH;
 #ifndef SPH_NBR_BOX_FILTER
 #define SPH_NBR_BOX_FILTER 1
 #endif
@@ -1081,7 +1072,7 @@ __device__ __forceinline__ void block_min_to(const Arrays &A, int bid, double v)
 // ---------------------------------------------------------------------------
 enum { S_STRAIN = 1, S_CONST = 2, S_CRIT = 4, S_INPLACE = 8 };
 
-template <int D>
+template <int D, int G = kStrain64Batch>
 __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, int mask, const int bid)
 {
     Ctl *ctl = A.ctl;
@@ -1105,7 +1096,7 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
             }
             struct Rec { double4 a, b; };
             auto load = [&](int j) { return Rec{A.rsA[j], A.rsB[j]}; };
-            for_each_nbr_batched<kStrainBatch>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
+            for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
                 const double4 ra = rec.a, rb = rec.b;
                 const double volj = ra.w;
                 const PairConst &pc = P.pc[mat_i][P.nmat == 1 ? mat_i : info_mat((uint32_t)rb.w)];
@@ -1272,7 +1263,7 @@ __device__ __forceinline__ void advance_body(const Params &P, const Arrays &A)
 // Monaghan (1992) artificial viscosity, Pi_ij = -alpha*cbar*mu_ij/rhobar for
 // approaching pairs (extension; the reference's visc_type field is unused).
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, int G = kMomentum64Batch>
 __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
@@ -1296,7 +1287,7 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
         }
         struct Rec { double4 a, b; double2 c; };
         auto load = [&](int j) { return Rec{A.rmA[j], A.rmB[j], A.rmC[j]}; };
-        for_each_nbr_batched<kMomentumBatch>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
+        for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
             const double4 ra = rec.a, rb = rec.b;
             const double2 rc = rec.c;
             const uint32_t fj = (uint32_t)rc.y;
@@ -1790,7 +1781,7 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const P
     neighbours_body<D>(Pb, Ab, e.y);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_shift(Params P, Arrays A) { shift_body<D>(P, A, blockIdx.x); }
-template <int D> __global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask) { XCD_BID(P); strain_body<D>(P, A, mask, bid_); }
+template <int D, int G> __global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask) { XCD_BID(P); strain_body<D, G>(P, A, mask, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_strain(const Params *Pt, const Arrays *At, const int2 *map, int mask)
 {
     const int2 e = map[blockIdx.x];
@@ -1819,7 +1810,7 @@ __global__ __launch_bounds__(64) void kb_advance(const Params *Pt, const Arrays 
     SCENE_OF(sc[blockIdx.x]);
     advance_body(Pb, Ab);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { XCD_BID(P); momentum_body<D>(P, A, bid_); }
+template <int D, int G> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { XCD_BID(P); momentum_body<D, G>(P, A, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
@@ -1953,9 +1944,20 @@ void launch_shift(const Params &P, const Arrays &A, hipStream_t st)
     DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_shift<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
 }
 
-void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st)
+// records in flight per lane: 2, 4 or 8 (0: the default); the use order is
+// the list order whatever the batch, so the sums are the same bits
+#define DISPATCH_G(G_, DEF, CALL)                              \
+    switch ((G_) ? (G_) : (DEF)) {                             \
+    case 2: { constexpr int GG = 2; CALL; } break;             \
+    case 8: { constexpr int GG = 8; CALL; } break;             \
+    default: { constexpr int GG = 4; CALL; } break;            \
+    }
+
+void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st, int batch)
 {
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_strain<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A, mask));
+    DISPATCH_G(batch, kStrain64Batch,
+               DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_strain<DD, GG>), dim3(xcd_grid(nblk(P.N))), dim3(kBlock),
+                                                     0, st, P, A, mask)));
 }
 
 void launch_normals(const Params &P, const Arrays &A, hipStream_t st)
@@ -1975,11 +1977,24 @@ void launch_crit_fold(const Params &P, const Arrays &A, hipStream_t st)
     hipLaunchKernelGGL(k_crit_fold, dim3(1), dim3(64), 0, st, A);
 }
 
-void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st)
+void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st, int batch)
 {
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_momentum<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A));
-    if (P.rigid)
+    DISPATCH_G(batch, kMomentum64Batch,
+               DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_momentum<DD, GG>), dim3(xcd_grid(nblk(P.N))), dim3(kBlock),
+                                                     0, st, P, A)));
+    if (P.rigid && rigid_kin >= 0)
         hipLaunchKernelGGL(k_rigid_reduce, dim3(1), dim3(kRigidBlock), 0, st, P, A, rigid_kin);
+}
+
+__global__ __launch_bounds__(64) void k_crit_reset(Arrays A)
+{
+    A.bbox[(size_t)threadIdx.x * kBboxStride + kCritSlot] = dkey(1.0e20);
+}
+
+void launch_crit_reset(const Arrays &A, hipStream_t st)
+{
+    static_assert(kBboxSlots == 64, "one slot per lane");
+    hipLaunchKernelGGL(k_crit_reset, dim3(1), dim3(64), 0, st, A);
 }
 
 void launch_rigid_reduce(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st)

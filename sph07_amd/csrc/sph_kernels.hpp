@@ -81,18 +81,23 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st);
 // autotuner of sph_capi.cpp; defaults measured on the 3D dam break).
 struct Tune {
     int cell_slots = 32;       // slots per wave of the cell-centric neighbour kernel
-    int strain32_batch = 8;    // neighbour records in flight per lane, FP32c continuity
-    int momentum32_batch = 8;  // the same, FP32c momentum
+    int strain_batch = 8;      // neighbour records in flight per lane, continuity sweep
+    int momentum_batch = 8;    // the same, momentum sweep
     int corder = 0;            // cell storage order: 0 Morton, 1 row-major
 };
 
 void launch_neighbours(const Params &P, const Arrays &A, hipStream_t st, int cell_slots = 32);
 void launch_shift(const Params &P, const Arrays &A, hipStream_t st);
-void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st);
+void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st, int batch = 0);
 void launch_normals(const Params &P, const Arrays &A, hipStream_t st);
 void launch_advance(const Params &P, const Arrays &A, hipStream_t st);
 void launch_crit_fold(const Params &P, const Arrays &A, hipStream_t st);
-void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st);
+// rigid_kin < 0: the sweep alone, without the rigid-body reduction
+void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st, int batch = 0);
+// records in flight per lane of the FP64 sweeps when none is chosen
+constexpr int kStrain64Batch = 4, kMomentum64Batch = 2;
+// resets the critical-step reduction slots (results of sweeps run for timing only)
+void launch_crit_reset(const Arrays &A, hipStream_t st);
 void launch_rigid_kin(const Params &P, const Arrays &A, hipStream_t st);
 void launch_rigid_reduce(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st);
 void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st);

@@ -214,16 +214,23 @@ class Scene:
         L.sph_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
         shape = (ctypes.c_int32 * 4)()
         self._chk(L.sph_autotune(self._h, reps, shape))
-        return {"cell_slots": shape[0], "strain32_batch": shape[1], "momentum32_batch": shape[2],
+        return {"cell_slots": shape[0], "strain_batch": shape[1], "momentum_batch": shape[2],
                 "cell_order": "row-major" if shape[3] == 1 else "morton"}
 
-    def set_launch_shape(self, cell_slots: int = 32, strain32_batch: int = 8, momentum32_batch: int = 8,
+    def set_launch_shape(self, cell_slots: int = 32, strain_batch: int = None, momentum_batch: int = None,
                          cell_order: str = "morton"):
-        """sph_set_launch_shape: fixes the launch shapes (no automatic tuning);
-        the defaults are the library's untuned shapes."""
+        """sph_set_launch_shape: fixes the launch shapes (no automatic tuning).
+        The batches are the neighbour records in flight per lane of the
+        scene's arm's continuity and momentum sweeps; None keeps the arm's
+        untuned default (FP64 4 / 2, FP32c 8 / 8)."""
         L = lib()
         L.sph_set_launch_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
-        shape = (ctypes.c_int32 * 4)(cell_slots, strain32_batch, momentum32_batch,
+        fp32c = self.arm == ARM_FP32C
+        if strain_batch is None:
+            strain_batch = 8 if fp32c else 4
+        if momentum_batch is None:
+            momentum_batch = 8 if fp32c else 2
+        shape = (ctypes.c_int32 * 4)(cell_slots, strain_batch, momentum_batch,
                                      1 if cell_order == "row-major" else 0)
         self._chk(L.sph_set_launch_shape(self._h, shape))
 

@@ -196,7 +196,11 @@ def serve_batched(requests: List[Request], arm: int, hbm_budget: float = 200e9,
 
 def serve_one_at_a_time(requests: List[Request], arm: int, keep_results: bool = False,
                         clock: Callable[[], float] = time.perf_counter) -> ServeStats:
-    """Baseline: requests in arrival order, each alone to completion."""
+    """Baseline: requests in arrival order, each alone to completion.  Each
+    scene runs at the library's default launch shapes: sph_solution would
+    otherwise time every shape on the first call of a scene of 65,536
+    particles or more, inside the request's time to solution (the batched
+    loop tunes on its own steps, at no extra cost)."""
     from .native import Scene, F_V
     pending = sorted(requests, key=lambda r: r.arrival)
     t0 = clock()
@@ -206,6 +210,7 @@ def serve_one_at_a_time(requests: List[Request], arm: int, keep_results: bool = 
             time.sleep(r.arrival - now)
         r.t_admit = clock() - t0
         sc = Scene(r.deck, arm)
+        sc.autotune(0)
         sc.Solution(r.steps)
         sc.synchronize()
         r.t_done = clock() - t0

@@ -385,8 +385,8 @@ def test_fp32c_launch_shapes_same_bits(strain_g, momentum_g):
     from sph07_amd.native import Scene, ARM_FP32C, F_X, F_V, F_RHO
     deck = scenes.dam_break(nx_fluid=24, ny=40, nz_fluid=20, nx_tank=60, nz_tank=30, maxnbr=100)
     ref, other = Scene(deck, ARM_FP32C), Scene(deck, ARM_FP32C)
-    ref.set_launch_shape(strain32_batch=8, momentum32_batch=4)
-    other.set_launch_shape(strain32_batch=strain_g, momentum32_batch=momentum_g)
+    ref.set_launch_shape(strain_batch=8, momentum_batch=4)
+    other.set_launch_shape(strain_batch=strain_g, momentum_batch=momentum_g)
     assert ref.Solution(20) == other.Solution(20) == 20
     for f in (F_X, F_V, F_RHO):
         assert np.array_equal(ref.field(f), other.field(f)), f
