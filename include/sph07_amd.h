@@ -29,6 +29,7 @@ extern "C" {
 #define SPH_ERR_OUT_OF_GRID 3   /* particle outside the grid (unchecked UB in the reference) */
 #define SPH_ERR_UNSUPPORTED 4   /* deck option outside the hot path */
 #define SPH_ERR_HIP         5   /* HIP runtime failure */
+#define SPH_ERR_CAPACITY    6   /* input larger than an entry point's fixed capacity */
 
 /* One material (material cards, MSimulationData.cpp:454-536, after
    MMaterial::CheckDefaults MMaterial.cpp:71-81). */
@@ -185,6 +186,20 @@ int  sph_get_loop_limits(void *ctx, int ndim, const int32_t *boxcrd, int32_t *lo
    (0 beyond ndim), dwdr[n]; any output may be NULL.  ndim 1-3, havg > 0. --- */
 int  sph_kernel_eval(int ndim, double havg, int n, const double *xi, const double *xj,
                      double *w, double *dwdx, double *dwdr);
+
+/* sph_kernel_eval without any device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The staging of coordinates and results lives in one static
+   device array per GPU, sized for SPH_KERNEL_EVAL_NOALLOC_MAX pairs
+   (11 doubles per pair, 5.8 MB) and held until the process ends; n above
+   that returns SPH_ERR_CAPACITY.  Runs on the calling thread's default
+   stream and returns when the results are in host memory.  Every call on one
+   GPU shares the array: calls that overlap on one GPU (several host threads)
+   race and must be serialised by the caller. */
+#define SPH_KERNEL_EVAL_NOALLOC_MAX 65536
+int  sph_kernel_eval_noalloc(int ndim, double havg, int n, const double *xi, const double *xj,
+                             double *w, double *dwdx, double *dwdr);
 
 /* --- batched scenes (no reference counterpart: the reference runs one scene
    per process, MSolution.h:16-22 singletons).  Many independent scenes of one

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1427,6 +1428,69 @@ int sph_kernel_eval(int ndim, double havg, int n, const double *xi, const double
 }
 
 }  // extern "C"
+
+// ---- sph_kernel_eval without allocation -------------------------------------
+// The staging of sph_kernel_eval (coordinates in, W / dW/dx / dW/dr out: 11
+// doubles per pair) lives in one static device array per GPU instead of a
+// per-call hipMalloc.  Host code never uses the array's name as an address:
+// its device address is looked up once per device with hipGetSymbolAddress
+// (with that device current) and kept in scratch_dev[].
+namespace {
+
+constexpr int kNoAllocMaxPairs = SPH_KERNEL_EVAL_NOALLOC_MAX;
+constexpr size_t kScratchDoubles = 11 * (size_t)kNoAllocMaxPairs;
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex scratch_mu;
+double *scratch_dev[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device; nullptr on failure.
+double *scratch_ptr()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        scratch_dev[dev] = (double *)p;
+    }
+    return scratch_dev[dev];
+}
+
+}  // namespace
+
+extern "C" int sph_kernel_eval_noalloc(int ndim, double havg, int n, const double *xi, const double *xj,
+                                       double *w, double *dwdx, double *dwdr)
+{
+    if (ndim < 1 || ndim > 3 || !(havg > 0.0) || n < 0 || (n > 0 && (!xi || !xj))) return SPH_ERROR;
+    if (n > kNoAllocMaxPairs) return SPH_ERR_CAPACITY;
+    if (n == 0) return SPH_OK;
+    double *d = scratch_ptr();
+    if (!d) return SPH_ERR_HIP;
+    const size_t v3 = 3 * (size_t)n * sizeof(double), v1 = (size_t)n * sizeof(double);
+    // the same layout as sph_kernel_eval's staging, packed at the front
+    double *dxi = d, *dxj = d + 3 * (size_t)n, *ddw = d + 6 * (size_t)n, *dw = d + 9 * (size_t)n,
+           *ddr = d + 10 * (size_t)n;
+    const PairConst pc = kernel_const(ndim, havg);
+    // the calling thread's default stream: nothing is created
+    hipStream_t st = hipStreamPerThread;
+    hipError_t e = hipMemcpyAsync(dxi, xi, v3, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dxj, xj, v3, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        launch_kernel_eval(ndim, pc, n, dxi, dxj, w ? dw : nullptr, dwdx ? ddw : nullptr, dwdr ? ddr : nullptr, st);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && w) e = hipMemcpyAsync(w, dw, v1, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && dwdx) e = hipMemcpyAsync(dwdx, ddw, v3, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && dwdr) e = hipMemcpyAsync(dwdr, ddr, v1, hipMemcpyDeviceToHost, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es;
+    return e == hipSuccess ? SPH_OK : SPH_ERR_HIP;
+}
 
 // ---- kernel timing (HIP events on the context's stream) --------------------
 extern "C" {
