@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libsph_oracle.so")
+# ORACLE_LIB overrides the library path (another build of the same sources,
+# e.g. one with a planted mistake that the pins must catch)
+_LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "_build", "libsph_oracle.so")
 _lib = None
 
 PH_NEIGHBOURS, PH_STRAIN, PH_NORMALS, PH_CONSTITUTIVE, PH_TIMESTEP, PH_MOMENTUM, \

@@ -11,20 +11,31 @@
  * floating-point evaluation order (every expression below keeps the operand
  * order of the cited line; compile with -ffp-contract=off so no FMA is formed).
  *
- * Pinning (what checks this file, beyond reading it against the reference):
+ * Pinning (what checks this file, beyond reading it against the reference;
+ * each check fails for a planted mistake of the kind it names):
  *   - kernel W/dW: closed form, partition of unity, continuity at z=1,2,
  *     derivative check (tests/test_oracle_kernel.py);
- *   - neighbour sets incl. periodic images: brute-force O(N^2) search
- *     (tests/test_oracle_neighbours.py);
- *   - continuity/EOS/momentum: lattice at rest, linear velocity field,
- *     pressure antisymmetry, hydrostatics (tests/test_oracle_physics.py);
- *   - Poiseuille start-up series (Morris 1997) for the full loop;
- *   - integration and dt: one-step closed forms;
- *   - CSF: Laplace law; rigid body: symmetry + momentum balance.
+ *   - grid, linked lists, neighbour sets incl. periodic images: brute-force
+ *     O(N^2) search, lattice counts, list order (tests/test_oracle_neighbours.py);
+ *   - strain: tr D of a linear field (tests/test_oracle_physics.py); the
+ *     half-step shift from the closed-form kernel gradient of two approaching
+ *     particles (tests/test_oracle_pins.py);
+ *   - RhoUpdate: one-step uniform dilation, rho0 (1 - D eps dt);
+ *   - EOS 29 / 28 with the pressure term: hydrostatic balance of a column
+ *     (a = g rho0/rho), p(rho0) = 0, c^2 = dp/drho;
+ *   - momentum: pair antisymmetry (sum m a = 0), Poiseuille start-up series
+ *     (Morris 1997) for the whole loop; Monaghan term: zero for separating
+ *     pairs, dissipative otherwise;
+ *   - CSF: Laplace law of a relaxed droplet, |n|-weighted curvature -1/R;
+ *   - rigid body: force and torque balance with every other particle,
+ *     rigid-body kinematics v = V + Omega x r;
+ *   - critical step: closed forms of options 1, 2 (incl. the step-1 1e-9)
+ *     and 3 (sound, viscous, acceleration, capillary terms); integration and
+ *     the dt rule: one-step closed forms of a lone particle.
  * The reference itself cannot be built here (it needs Boost uBLAS, which is
  * not installed), so no compiled-reference vectors exist; parity for the
  * uBLAS "v = 0" paths (CSF normals, rigid A/ALPHA) follows the evident intent
- * (zero fill) and is pinned only by the invariants above.
+ * (zero fill) and is pinned by the invariants above.
  */
 #include "sph_oracle.h"
 

@@ -11,7 +11,7 @@
   * BoundCond / UpdateVelocity / MoveParticles and the dt rule (:275-277,
     :602-809): exact one-step closed forms for a lone particle;
   * moving-lid walls (dispbc 8): position fixed, velocity kept;
-  * CSF (:1429-1493): curvature of a circular colour patch is 1/R;
+  * CSF (:1429-1493): curvature of a circular colour patch is 1/R (5 %);
   * rigid body (:1496-1591): a body moves rigidly; a symmetric body at rest
     in quiescent fluid gets no angular acceleration.
 """
@@ -165,7 +165,7 @@ def test_csf_curvature_of_circle():
     radial = np.stack([d.x[:, 0] - 0.5, d.x[:, 1] - 0.5], 1) / np.maximum(r, 1e-12)[:, None]
     assert np.all((nrm[band, :2] * radial[band]).sum(1) < 0)
     kmean = kap[band].mean()
-    assert abs(kmean + 1.0 / R) < 0.25 / R, kmean
+    assert abs(kmean + 1.0 / R) < 0.05 / R, kmean
 
 
 def test_rigid_body_moves_rigidly():
