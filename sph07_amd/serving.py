@@ -78,7 +78,7 @@ class Request:
     t_admit: float = -1.0
     t_done: float = -1.0
     slot: int = -1
-    result: Optional[np.ndarray] = None
+    result: Optional[dict] = None     # field id -> array, with keep_results
 
     @property
     def updates(self) -> int:
@@ -152,7 +152,7 @@ def serve_batched(requests: List[Request], arm: int, hbm_budget: float = 200e9,
                   clock: Callable[[], float] = time.perf_counter) -> ServeStats:
     """Continuous batching: admit on arrival (memory and slots permitting),
     advance all live scenes together, retire as they finish."""
-    from .native import Batch, F_V, footprint
+    from .native import Batch, F_RHO, F_V, F_X, footprint
     batch = Batch(2, arm)
     pending = sorted(requests, key=lambda r: r.arrival)
     qi = 0
@@ -184,7 +184,7 @@ def serve_batched(requests: List[Request], arm: int, hbm_budget: float = 200e9,
             if st["error"]:
                 raise RuntimeError(f"request {r.rid} failed on the device (code {st['error']})")
             if keep_results:
-                r.result = batch.field(s, F_V)
+                r.result = {f: batch.field(s, f) for f in (F_X, F_V, F_RHO)}
             batch.remove(s)
             used -= r.fp
     end = clock() - t0
@@ -201,7 +201,7 @@ def serve_one_at_a_time(requests: List[Request], arm: int, keep_results: bool = 
     otherwise time every shape on the first call of a scene of 65,536
     particles or more, inside the request's time to solution (the batched
     loop tunes on its own steps, at no extra cost)."""
-    from .native import Scene, F_V
+    from .native import Scene, F_RHO, F_V, F_X
     pending = sorted(requests, key=lambda r: r.arrival)
     t0 = clock()
     for r in pending:
@@ -215,7 +215,7 @@ def serve_one_at_a_time(requests: List[Request], arm: int, keep_results: bool = 
         sc.synchronize()
         r.t_done = clock() - t0
         if keep_results:
-            r.result = sc.field(F_V)
+            r.result = {f: sc.field(f) for f in (F_X, F_V, F_RHO)}
         sc.close()
     end = clock() - t0
     first = min(r.arrival for r in requests)

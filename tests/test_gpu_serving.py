@@ -27,7 +27,7 @@ def test_served_results_equal_alone():
     for r in reqs:
         alone = Scene(r.deck, ARM_FP32C)
         assert alone.Solution(r.steps) == r.steps
-        assert np.array_equal(r.result, alone.field(F_V)), (r.rid, r.kind)
+        assert np.array_equal(r.result[F_V], alone.field(F_V)), (r.rid, r.kind)
         alone.close()
 
 
