@@ -22,8 +22,9 @@ PH_NEIGHBOURS, PH_STRAIN, PH_NORMALS, PH_CONSTITUTIVE, PH_TIMESTEP, PH_MOMENTUM,
 NPHASES = 9
 
 F_X, F_V, F_A, F_RHO, F_P, F_C, F_MASS, F_H, F_TRD, F_MINDIST, F_NORMAL, F_NNORMAL, \
-    F_N, F_KAPPA, F_CRITTS, F_RHOOLD, F_ACTIVE, F_VABS = range(18)
-_VEC = {F_X, F_V, F_A, F_NORMAL, F_NNORMAL}
+    F_N, F_KAPPA, F_CRITTS, F_RHOOLD, F_ACTIVE, F_VABS, F_Q, F_SIGMA_DIAG, F_SIGMA_OFF, \
+    F_ROD_DIAG, F_ROD_OFF = range(23)
+_VEC = {F_X, F_V, F_A, F_NORMAL, F_NNORMAL, F_SIGMA_DIAG, F_SIGMA_OFF, F_ROD_DIAG, F_ROD_OFF}
 
 GLOBAL_NAMES = ["dt", "dtold", "ptime", "critts", "timestep", "hmax",
                 "cmm_min0", "cmm_min1", "cmm_min2", "cmm_max0", "cmm_max1", "cmm_max2", "ngp",
@@ -119,6 +120,13 @@ class Oracle:
         out = np.zeros((n, 3) if f in _VEC else n)
         self._check(lib().orc_get_field(self._h, f, out.ctypes.data))
         return out
+
+    def set_field(self, f: int, data):
+        """Overwrites a field between phases (velocities only)."""
+        L = lib()
+        L.orc_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        arr = np.ascontiguousarray(data, np.float64).reshape(self.np, 3)
+        self._check(L.orc_set_field(self._h, f, arr.ctypes.data))
 
     def linked_list(self) -> dict:
         """Grid and linked lists of the last SetupLinkedList + GhostSetup."""

@@ -91,6 +91,7 @@ struct orc_scene {
     double *x, *v, *a;                  /* 3 per particle */
     double *rho, *rhoold, *rho0, *p, *c, *m, *h, *vabs, *critts_p, *trd, *mindist;
     double *normal, *nnormal, *N, *kappa;
+    double *rod, *sig, *q;              /* disctype 0: 3x3 row-major rod and sigma, q (diagonal) */
     int *mat_id, *color, *dispbc, *active, *llp, *nnbr, *nbr;
     /* ghosts, 1-based */
     int ngp, gcapn;
@@ -191,12 +192,13 @@ double orc_kernel_dwdr(const double *xi, const double *xj, int ndim, double havg
 /* ---- neighbour record access: id>0 real, id<0 ghost (MSolution.cpp:187-188) ---- */
 typedef struct {
     const double *x, *v;
-    double m, h, rho, p, c, N;
-    const double *nn;
+    double m, h, rho, p, c, N, q;
+    const double *nn, *sig;   /* sig: 3x3 row-major (disctype 0) */
     int mat, color;
 } orc_view;
 
 static const double orc_zero3[3] = {0.0, 0.0, 0.0};
+static const double orc_zero9[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 
 static orc_view view_of(const orc_scene *s, int id)
 {
@@ -206,6 +208,8 @@ static orc_view view_of(const orc_scene *s, int id)
         r.m = s->m[id]; r.h = s->h[id]; r.rho = s->rho[id]; r.p = s->p[id];
         r.c = s->c[id]; r.N = s->N[id]; r.nn = &s->nnormal[3 * id];
         r.mat = s->mat_id[id]; r.color = s->color[id];
+        r.sig = s->sig ? &s->sig[9 * (size_t)id] : orc_zero9;
+        r.q = s->q ? s->q[id] : 0.0;
     } else {
         const orc_ghost *g = &s->g[-id];
         r.x = g->x; r.v = g->v; r.m = g->m; r.h = g->h; r.rho = g->rho;
@@ -213,6 +217,7 @@ static orc_view view_of(const orc_scene *s, int id)
            (MGhostParticleData.cpp:16-28, MCommonParticle.cpp:12,17-19) */
         r.p = 0.0; r.N = 0.0; r.nn = orc_zero3; r.c = g->c;
         r.mat = g->mat; r.color = g->color;
+        r.sig = orc_zero9; r.q = 0.0;       /* sigma and q: not copied either */
     }
     return r;
 }
@@ -439,18 +444,29 @@ static int phase_strain(orc_scene *s)
         double dwdx[3];
         if (!s->active[i]) continue;
         const double *xi = &s->x[3 * i], *vi = &s->v[3 * i];
-        double g[3] = {0.0, 0.0, 0.0};   /* diagonal of grad v */
+        /* grad v (MSolution.cpp:397-398); Morris needs its diagonal only */
+        double g[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        const int full = s->disctype == 0;
         for (int k = 0; k < s->nnbr[i]; k++) {
             int id = s->nbr[(size_t)i * s->maxnbr + k];
             orc_view pj = view_of(s, id);
             double volj = pj.m / pj.rho;
             double havg = AVG(s->h[i], pj.h);
             orc_gradw(dwdx, xi, pj.x, s->ndim, havg);
-            for (int r = 0; r < s->ndim; r++) g[r] += volj * (pj.v[r] - vi[r]) * dwdx[r];
+            if (full) {
+                for (int r = 0; r < s->ndim; r++)
+                    for (int c = 0; c < s->ndim; c++) g[r][c] += volj * (pj.v[r] - vi[r]) * dwdx[c];
+            } else {
+                for (int r = 0; r < s->ndim; r++) g[r][r] += volj * (pj.v[r] - vi[r]) * dwdx[r];
+            }
         }
-        /* rod = 0.5*(g + g^T): the diagonal is 0.5*(g_rr + g_rr) */
+        /* rod = 0.5*(g + g^T), spin unused (:406-407); trace over ndim
+           (MParticle::CalcTraceROD, MParticle.cpp:43-47) */
+        if (full)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) s->rod[9 * (size_t)i + 3 * r + c] = 0.5 * (g[r][c] + g[c][r]);
         double tr = 0.0;
-        for (int r = 0; r < s->ndim; r++) tr += 0.5 * (g[r] + g[r]);
+        for (int r = 0; r < s->ndim; r++) tr += 0.5 * (g[r][r] + g[r][r]);
         s->trd[i] = tr;
     }
     for (int i = 1; i <= s->np; i++) {
@@ -526,6 +542,53 @@ static int phase_constitutive(orc_scene *s)
         } else {
             return fail(s, "constitutive: EOS not supported");
         }
+    }
+    return 0;
+}
+
+/* Model 9 for the Classic discretisation, MConstitutive.cpp:10-42 per
+   particle: F3DM9 (:44-68) deviatoric stress from rod, AViscosity (:123-133)
+   q only when tr D < 0 (else q keeps its value), the EOS, StressUp
+   (:210-222) sigma = -p I + s.  HieUpdate (:225-262) only moves the trial
+   energy, which EOS 28/29 never read.  After RhoUpdate (MSolution.cpp:444). */
+static int phase_constitutive_classic(orc_scene *s)
+{
+    for (int i = 1; i <= s->np; i++) {
+        if (!s->active[i]) continue;
+        s->rhoold[i] = s->rho[i];
+        s->rho[i] *= (1.0 - s->trd[i] * s->dt);
+    }
+    for (int i = 1; i <= s->np; i++) {
+        if (!s->active[i]) continue;
+        const orc_material *mt = &s->mat[s->mat_id[i]];
+        const double *rod = &s->rod[9 * (size_t)i];
+        double *sg = &s->sig[9 * (size_t)i];
+        double sdev[9];
+        /* F3DM9: xmu = av_l * rho of the material (:49-52) */
+        const double xmu = mt->av_l * mt->rho;
+        if (fabs(xmu) < 1.e-10) {
+            for (int k = 0; k < 9; k++) sdev[k] = 0.0;
+        } else {
+            const double davg = 1.0 / 3.0 * s->trd[i];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) sdev[3 * r + c] = 2 * xmu * (rod[3 * r + c] - (r == c ? davg : 0.0));
+        }
+        /* AViscosity: c is the particle's sound speed before this EOS call */
+        if (s->trd[i] < 0.0)
+            s->q[i] = mt->av_l * s->rho[i] * s->c[i] * s->h[i] * fabs(s->trd[i]) +
+                      mt->av_q * s->rho[i] * SQR(s->h[i] * s->trd[i]);
+        if (mt->eos == 29) {
+            s->c[i] = mt->eosinput[0];
+            s->p[i] = SQR(s->c[i]) * (s->rho[i] - s->rho0[i]);
+        } else if (mt->eos == 28) {
+            double b = mt->eosinput[0], gamma = mt->eosinput[1];
+            s->p[i] = b * (pow(s->rho[i] / s->rho0[i], gamma) - 1.0);
+            s->c[i] = sqrt(b * gamma / s->rho0[i]);
+        } else {
+            return fail(s, "constitutive: EOS not supported");
+        }
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) sg[3 * r + c] = -(r == c ? s->p[i] : 0.0) + sdev[3 * r + c];
     }
     return 0;
 }
@@ -667,6 +730,33 @@ static int phase_momentum(orc_scene *s)
     return 0;
 }
 
+/* MSolution::ClassicAcceleration, MSolution.cpp:506-548: a is cleared for
+   computable particles only; images carry sigma = q = 0. */
+static int phase_momentum_classic(orc_scene *s)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 1; i <= s->np; i++) {
+        double dwdx[3];
+        if (!s->active[i]) continue;
+        double *ai = &s->a[3 * i];
+        ai[0] = ai[1] = ai[2] = 0.0;
+        const double *xi = &s->x[3 * i], *sgi = &s->sig[9 * (size_t)i];
+        const double rho_i = s->rho[i], q_i = s->q[i];
+        for (int k = 0; k < s->nnbr[i]; k++) {
+            orc_view pj = view_of(s, s->nbr[(size_t)i * s->maxnbr + k]);
+            double havg = AVG(s->h[i], pj.h);
+            orc_gradw(dwdx, xi, pj.x, s->ndim, havg);
+            double ds[3] = {0.0, 0.0, 0.0};
+            for (int n = 0; n < s->ndim; n++)
+                for (int m = 0; m < s->ndim; m++)
+                    ds[n] += ((pj.sig[3 * m + n] - (m == n ? pj.q : 0.0)) / SQR(pj.rho) +
+                              (sgi[3 * m + n] - (m == n ? q_i : 0.0)) / SQR(rho_i)) * dwdx[m];
+            for (int n = 0; n < 3; n++) ai[n] = ai[n] + pj.m * ds[n];
+        }
+    }
+    return 0;
+}
+
 /* MSolution::ConstrainQuantity, MSolution.cpp:729-767; code 8 is the
    prescribed-velocity wall (extension): acceleration fully constrained. */
 static void constrain(double *q, int code)
@@ -765,9 +855,9 @@ int orc_phase(orc_scene *s, int phase)
     case ORC_PH_NEIGHBOURS: return phase_neighbours(s, 1);
     case ORC_PH_STRAIN: return phase_strain(s);
     case ORC_PH_NORMALS: return phase_normals(s);
-    case ORC_PH_CONSTITUTIVE: return phase_constitutive(s);
+    case ORC_PH_CONSTITUTIVE: return s->disctype == 0 ? phase_constitutive_classic(s) : phase_constitutive(s);
     case ORC_PH_TIMESTEP: return phase_timestep(s);
-    case ORC_PH_MOMENTUM: return phase_momentum(s);
+    case ORC_PH_MOMENTUM: return s->disctype == 0 ? phase_momentum_classic(s) : phase_momentum(s);
     case ORC_PH_BOUNDCOND: return phase_boundcond(s);
     case ORC_PH_VELOCITY: return phase_velocity(s);
     case ORC_PH_MOVE: return phase_move(s);
@@ -801,7 +891,8 @@ orc_scene *orc_create(const orc_deck *d, char *err, int errlen)
     if (!s) return NULL;
 #define BAD(m) do { msg = (m); goto bad; } while (0)
     if (d->ndim < 1 || d->ndim > 3) BAD("ndim must be 1, 2 or 3");
-    if (d->disctype != 2) BAD("only the Morris discretisation (disctype 2) is on the path");
+    if (d->disctype != 0 && d->disctype != 2) BAD("disctype must be 0 (Classic) or 2 (Morris)");
+    if (d->disctype == 0 && d->rigid_body) BAD("the rigid body needs disctype 2 (MSolution.cpp:492-500)");
     if (d->veloc_opt == 1) BAD("XSPH (veloc_opt 1) is out of scope");
     if (d->h_opt != 0) BAD("variable h (h_opt > 0) is out of scope");
     if (d->init_h_opt != 0) BAD("init_h_opt 1 is not supported");
@@ -848,6 +939,12 @@ orc_scene *orc_create(const orc_deck *d, char *err, int errlen)
     int **iscal[] = {&s->mat_id, &s->color, &s->dispbc, &s->active, &s->llp, &s->nnbr};
     for (size_t k = 0; k < sizeof iscal / sizeof iscal[0]; k++) *iscal[k] = calloc(n1, sizeof(int));
     s->nbr = calloc(n1 * (size_t)s->maxnbr, sizeof(int));
+    if (s->disctype == 0) {
+        s->rod = calloc(9 * n1, sizeof(double));
+        s->sig = calloc(9 * n1, sizeof(double));
+        s->q = calloc(n1, sizeof(double));
+        if (!s->rod || !s->sig || !s->q) BAD("out of memory");
+    }
     if (!s->x || !s->v || !s->a || !s->nnormal || !s->normal || !s->kappa || !s->nbr || !s->nnbr)
         BAD("out of memory");
 
@@ -933,7 +1030,7 @@ void orc_destroy(orc_scene *s)
     void *ptrs[] = {s->x, s->v, s->a, s->normal, s->nnormal, s->rho, s->rhoold, s->rho0, s->p,
                     s->c, s->m, s->h, s->vabs, s->critts_p, s->trd, s->mindist, s->N, s->kappa,
                     s->mat_id, s->color, s->dispbc, s->active, s->llp, s->nnbr, s->nbr, s->g,
-                    s->grid};
+                    s->grid, s->rod, s->sig, s->q};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++) free(ptrs[k]);
     free(s);
 }
@@ -967,6 +1064,19 @@ int orc_globals(orc_scene *s, double *out, int n)
     return k;
 }
 
+/* Overwrites the velocities (field ORC_F_V, np*3) between phases. */
+int orc_set_field(orc_scene *s, int f, const double *in)
+{
+    if (f != ORC_F_V) return 1;
+    memcpy(s->v + 3, in, (size_t)s->np * 3 * sizeof(double));
+    for (int i = 1; i <= s->np; i++) {
+        double va = 0.0;
+        for (int j = 0; j < s->ndim; j++) va += SQR(s->v[3 * i + j]);
+        s->vabs[i] = sqrt(va);
+    }
+    return 0;
+}
+
 int orc_get_field(orc_scene *s, int f, double *out)
 {
     const double *src = NULL;
@@ -992,6 +1102,24 @@ int orc_get_field(orc_scene *s, int f, double *out)
     case ORC_F_ACTIVE:
         for (int i = 1; i <= s->np; i++) out[i - 1] = s->active[i];
         return 0;
+    case ORC_F_Q:
+        for (int i = 1; i <= s->np; i++) out[i - 1] = s->q ? s->q[i] : 0.0;
+        return 0;
+    case ORC_F_SIGMA_DIAG: case ORC_F_SIGMA_OFF: case ORC_F_ROD_DIAG: case ORC_F_ROD_OFF: {
+        /* diagonal (xx, yy, zz) or off-diagonal (xy, xz, yz); Morris: sigma = -p I */
+        static const int dg[3] = {0, 4, 8}, od[3] = {1, 2, 5};
+        const int *ix = (f == ORC_F_SIGMA_DIAG || f == ORC_F_ROD_DIAG) ? dg : od;
+        const double *t = (f == ORC_F_SIGMA_DIAG || f == ORC_F_SIGMA_OFF) ? s->sig : s->rod;
+        for (int i = 1; i <= s->np; i++)
+            for (int k = 0; k < 3; k++) {
+                double v;
+                if (t) v = t[9 * (size_t)i + ix[k]];
+                else if (f == ORC_F_SIGMA_DIAG) v = -s->p[i];
+                else v = 0.0;
+                out[3 * (size_t)(i - 1) + k] = v;
+            }
+        return 0;
+    }
     default: return 1;
     }
     if (vec) memcpy(out, src + 3, (size_t)s->np * 3 * sizeof(double));

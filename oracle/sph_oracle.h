@@ -91,8 +91,14 @@ int         orc_globals(orc_scene *s, double *out, int n);
 /* Per-particle field download (0-based, np entries, vectors np*3). */
 enum { ORC_F_X = 0, ORC_F_V, ORC_F_A, ORC_F_RHO, ORC_F_P, ORC_F_C, ORC_F_MASS,
        ORC_F_H, ORC_F_TRD, ORC_F_MINDIST, ORC_F_NORMAL, ORC_F_NNORMAL,
-       ORC_F_N, ORC_F_KAPPA, ORC_F_CRITTS, ORC_F_RHOOLD, ORC_F_ACTIVE, ORC_F_VABS };
+       ORC_F_N, ORC_F_KAPPA, ORC_F_CRITTS, ORC_F_RHOOLD, ORC_F_ACTIVE, ORC_F_VABS,
+       ORC_F_Q, ORC_F_SIGMA_DIAG, ORC_F_SIGMA_OFF, ORC_F_ROD_DIAG, ORC_F_ROD_OFF };
+/* ORC_F_SIGMA_* / ORC_F_ROD_* are vector fields (np*3): the diagonal xx, yy,
+   zz or the off-diagonal xy, xz, yz of the symmetric tensors (disctype 0;
+   with disctype 2 sigma is -p I and rod is not kept). */
 int         orc_get_field(orc_scene *s, int field, double *out);
+/* Test hook: overwrites the velocities (ORC_F_V only) between phases. */
+int         orc_set_field(orc_scene *s, int field, const double *in);
 /* Neighbour lists in the reference's order.  canon ids: a real neighbour j is
    j (0-based); a periodic image of real j shifted by (sx,sy,sz) in {-1,0,1}^3
    is encoded as -(1 + j*27 + (sx+1)*9 + (sy+1)*3 + (sz+1)). */
