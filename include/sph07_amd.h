@@ -92,7 +92,12 @@ typedef struct sph_globals {
 enum {
     SPH_F_X = 0, SPH_F_V, SPH_F_A, SPH_F_RHO, SPH_F_P, SPH_F_C, SPH_F_MASS, SPH_F_H,
     SPH_F_TRD, SPH_F_MINDIST, SPH_F_NORMAL, SPH_F_NNORMAL, SPH_F_N, SPH_F_KAPPA,
-    SPH_F_CRITTS, SPH_F_RHOOLD, SPH_F_ACTIVE, SPH_F_VABS, SPH_F_NNBR
+    SPH_F_CRITTS, SPH_F_RHOOLD, SPH_F_ACTIVE, SPH_F_VABS, SPH_F_NNBR,
+    /* Classic discretisation (disctype 0): the artificial viscosity q (np),
+       and the symmetric stress and rate of deformation as vector fields
+       (np*3): diagonal xx, yy, zz or off-diagonal xy, xz, yz.  With disctype
+       2 the stress is -p I, q and the rate of deformation are zero. */
+    SPH_F_Q, SPH_F_SIGMA_DIAG, SPH_F_SIGMA_OFF, SPH_F_ROD_DIAG, SPH_F_ROD_OFF
 };
 
 /* --- lifetime --- */
@@ -117,17 +122,22 @@ int  sph_neighbours(void *ctx);
 int  sph_ghost_setup(void *ctx);
 /* MSolution::LinkedListNeighbours (MSolution.cpp:150-215). */
 int  sph_linked_list_neighbours(void *ctx);
-/* MSolution::Strain + ClassicStrain (MSolution.cpp:327-413). */
+/* MSolution::Strain + ClassicStrain (MSolution.cpp:327-413); with disctype 0
+   the full rate of deformation. */
 int  sph_strain(void *ctx);
 /* MSolution::CalculateSurfaceNormals (MSolution.cpp:1429-1493). */
 int  sph_calculate_surface_normals(void *ctx);
-/* MSolution::Constitutive + RhoUpdate (MSolution.cpp:436-477). */
+/* MSolution::Constitutive + RhoUpdate (MSolution.cpp:436-477): the EOS;
+   with disctype 0 also F3DM9, AViscosity and StressUp of model 9
+   (MConstitutive.cpp:10-68,123-133,210-222). */
 int  sph_constitutive(void *ctx);
 /* MSolution::CalcCriticalTimestep (MSolution.cpp:34-94); critts may be NULL. */
 int  sph_calc_critical_timestep(void *ctx, double *critts);
 /* The dt rule inside MSolution::Solution (MSolution.cpp:275-277). */
 int  sph_time_advance(void *ctx);
-/* MSolution::Momentum (MSolution.cpp:479-504) incl. RigidBodyAcceleration. */
+/* MSolution::Momentum (MSolution.cpp:479-504): MorrisAcceleration incl.
+   RigidBodyAcceleration (disctype 2) or ClassicAcceleration (disctype 0,
+   :506-548; the FP64 arm, no rigid body). */
 int  sph_momentum(void *ctx);
 /* MSolution::BoundCond (MSolution.cpp:700-726). */
 int  sph_boundcond(void *ctx);

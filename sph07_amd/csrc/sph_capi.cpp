@@ -210,6 +210,19 @@ int alloc_particles(Context *c, int gcap)
         if (P.rigid) rc |= dalloc(c, L, &S->rrank, c->N);
         else S->rrank = nullptr;
     }
+    if (P.disctype == 0) {
+        rc |= dalloc(c, L, &A.q, c->N);
+        rc |= dalloc(c, L, &B.q, c->N);
+        for (int k = 0; k < 6; k++) {
+            rc |= dalloc(c, L, &A.rod[k], c->N);
+            rc |= dalloc(c, L, &A.sig[k], c->N);
+        }
+        rc |= dalloc(c, L, &A.rcC, ncap);
+    } else {
+        A.q = B.q = nullptr;
+        for (int k = 0; k < 6; k++) A.rod[k] = A.sig[k] = nullptr;
+        A.rcC = nullptr;
+    }
     rc |= dalloc(c, L, &A.rsA, ncap);
     rc |= dalloc(c, L, &A.rsB, ncap);
     rc |= dalloc(c, L, &A.rmA, ncap);
@@ -259,6 +272,11 @@ int alloc_particles(Context *c, int gcap)
     B.Nf = A.Nf; B.kappa = A.kappa; B.nnbr = A.nnbr; B.nbr = A.nbr;
     B.cell_of = A.cell_of; B.slot = A.slot; B.tmp = A.tmp; B.occ = A.occ; B.ovf = A.ovf;
     B.gsrc = A.gsrc; B.gshift = A.gshift;
+    for (int k = 0; k < 6; k++) {
+        B.rod[k] = A.rod[k];
+        B.sig[k] = A.sig[k];
+    }
+    B.rcC = A.rcC;
     return rc;
 }
 
@@ -329,6 +347,13 @@ int grow_particles(Context *c, int new_gcap)
     rc |= cp(A.info, old.info, n * 4);
     rc |= cp(A.orig, old.orig, n * 4);
     if (c->P.rigid) rc |= cp(A.rrank, old.rrank, n * 4);
+    if (c->P.disctype == 0) {
+        rc |= cp(A.q, old.q, n * 8);
+        for (int k = 0; k < 6; k++) {
+            rc |= cp(A.rod[k], old.rod[k], n * 8);
+            rc |= cp(A.sig[k], old.sig[k], n * 8);
+        }
+    }
     rc |= cp(A.trd, old.trd, n * 8);
     rc |= cp(A.mindist, old.mindist, n * 8);
     rc |= cp(A.critts_p, old.critts_p, n * 8);
@@ -459,7 +484,9 @@ int build_context(Context *c, const sph_scene_desc *d)
 {
     Params &P = c->P;
     if (d->ndim < 1 || d->ndim > 3) { c->err = "ndim must be 1, 2 or 3"; return SPH_ERROR; }
-    if (d->disctype != 2) { c->err = "only disctype 2 (Morris) is on the GPU path"; return SPH_ERR_UNSUPPORTED; }
+    if (d->disctype != 0 && d->disctype != 2) { c->err = "disctype must be 0 (Classic) or 2 (Morris)"; return SPH_ERR_UNSUPPORTED; }
+    if (d->disctype == 0 && c->arm != SPH_ARM_FP64) { c->err = "the Classic discretisation (disctype 0) runs in the FP64 arm"; return SPH_ERR_UNSUPPORTED; }
+    if (d->disctype == 0 && d->rigid_body) { c->err = "the rigid body needs disctype 2 (MSolution.cpp:492-500)"; return SPH_ERR_UNSUPPORTED; }
     if (d->veloc_opt == 1) { c->err = "XSPH (veloc_opt 1) is outside the GPU path"; return SPH_ERR_UNSUPPORTED; }
     if (d->h_opt != 0) { c->err = "variable h (h_opt > 0) is outside the GPU path"; return SPH_ERR_UNSUPPORTED; }
     if (d->init_h_opt != 0) { c->err = "init_h_opt 1 is not supported"; return SPH_ERR_UNSUPPORTED; }
@@ -475,6 +502,7 @@ int build_context(Context *c, const sph_scene_desc *d)
     P.tcrit_opt = d->tcrit_opt;
     P.maxnbr = c->maxnbr;
     P.arm = c->arm;
+    P.disctype = d->disctype;
     P.st = d->surface_tension != 0;
     P.rigid = d->rigid_body != 0;
     P.baseaccel = d->baseaccel != 0;
@@ -525,6 +553,7 @@ int build_context(Context *c, const sph_scene_desc *d)
         mt.eos = md.eos;
         mt.visc_type = md.visc_type;
         mt.alpha = md.av_alpha;
+        mt.av_q = md.av_q;
         mt.eos_b = md.eosinput[0];
         mt.eos_gamma = md.eosinput[1];
         mt.c = md.eos == 29 ? md.eosinput[0] : sqrt(md.eosinput[0] * md.eosinput[1] / md.rho);
@@ -727,6 +756,15 @@ int create_impl(Context *c, const sph_scene_desc *d)
     HIPCHK(c, hipMemcpy(A.info, hinfo.data(), N * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(A.trd, 0, N * 8));
     HIPCHK(c, hipMemset(A.nnbr, 0, N * 4));
+    if (P.disctype == 0) {
+        // q starts at zero (AViscosity at initialisation sees tr D = 0,
+        // MSimulationInit.cpp:62-68); sigma and rod are written every step
+        HIPCHK(c, hipMemset(A.q, 0, N * 8));
+        for (int k = 0; k < 6; k++) {
+            HIPCHK(c, hipMemset(A.rod[k], 0, N * 8));
+            HIPCHK(c, hipMemset(A.sig[k], 0, N * 8));
+        }
+    }
     if (P.st) {
         HIPCHK(c, hipMemset(A.Nf, 0, N * 8));
         HIPCHK(c, hipMemset(A.kappa, 0, N * 8));
@@ -768,14 +806,17 @@ int slot_orig(Context *c, std::vector<int> &orig)
     return 0;
 }
 
-int download_vec(Context *c, double *const *src, double *out)
+// ncomp: components kept on the device (the rest are written as zero);
+// -1 for ndim
+int download_vec(Context *c, double *const *src, double *out, int ncomp = -1)
 {
     const int N = c->N;
+    const int nc = ncomp < 0 ? c->ndim : ncomp;
     std::vector<double> tmp(N);
     std::vector<int> orig;
     if (slot_orig(c, orig)) return SPH_ERR_HIP;
     for (int d = 0; d < 3; d++) {
-        if (d < c->ndim && src[d]) {
+        if (d < nc && src[d]) {
             HIPCHK(c, hipMemcpy(tmp.data(), src[d], N * 8, hipMemcpyDeviceToHost));
             for (int i = 0; i < N; i++) out[3 * (size_t)orig[i] + d] = tmp[i];
         } else {
@@ -1234,6 +1275,35 @@ int sph_download(void *ctx, int field, double *out)
     case SPH_F_A: return download_vec(c, A.a, out);
     case SPH_F_NORMAL: return download_vec(c, A.nrm, out);
     case SPH_F_NNORMAL: return download_vec(c, A.nn, out);
+    case SPH_F_Q:
+        if (c->P.disctype != 0) {
+            for (int i = 0; i < N; i++) out[i] = 0.0;
+            return SPH_OK;
+        }
+        return scalar(A.q);
+    case SPH_F_SIGMA_DIAG:
+    case SPH_F_SIGMA_OFF:
+    case SPH_F_ROD_DIAG:
+    case SPH_F_ROD_OFF: {
+        const bool sig = field == SPH_F_SIGMA_DIAG || field == SPH_F_SIGMA_OFF;
+        const int k0 = (field == SPH_F_SIGMA_DIAG || field == SPH_F_ROD_DIAG) ? 0 : 3;
+        if (c->P.disctype == 0) {
+            double *const *t = sig ? A.sig : A.rod;
+            double *src[3] = {t[k0], t[k0 + 1], t[k0 + 2]};
+            return download_vec(c, src, out, 3);   // every component (a 2D zz too)
+        }
+        // disctype 2: sigma = -p I, no rate of deformation kept
+        std::vector<double> p(N, 0.0);
+        if (field == SPH_F_SIGMA_DIAG) {
+            if (slot_orig(c, orig)) return SPH_ERR_HIP;
+            tmp.resize(N);
+            HIPCHK(c, hipMemcpy(tmp.data(), A.p, N * 8, hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; i++) p[orig[i]] = -tmp[i];
+        }
+        for (int i = 0; i < N; i++)
+            for (int d = 0; d < 3; d++) out[3 * (size_t)i + d] = p[i];
+        return SPH_OK;
+    }
     case SPH_F_RHO: return scalar(A.rho);
     case SPH_F_P: return scalar(A.p);
     case SPH_F_MASS: return scalar(A.m);
@@ -1793,6 +1863,10 @@ int sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int *
     if (desc->ndim != b->ndim) {
         if (err && errlen > 0) snprintf(err, (size_t)errlen, "scene ndim differs from the batch");
         return SPH_ERROR;
+    }
+    if (desc->disctype != 2) {
+        if (err && errlen > 0) snprintf(err, (size_t)errlen, "batched scenes use disctype 2 (Morris)");
+        return SPH_ERR_UNSUPPORTED;
     }
     BCHK(b, hipStreamSynchronize(b->st));
     void *ctx = nullptr;

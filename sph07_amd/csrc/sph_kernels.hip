@@ -135,6 +135,19 @@ __device__ __forceinline__ void put_mrec(const Arrays &A, int j, const double *x
     A.rmC[j] = make_double2(prho2, (double)info);
 }
 
+// symmetric tensors of the Classic discretisation: xx, yy, zz, xy, xz, yz
+__device__ __forceinline__ int sym6(int r, int c)
+{
+    return r == c ? r : (r + c == 1 ? 3 : (r + c == 2 ? 4 : 5));
+}
+
+template <int D>
+__device__ __forceinline__ void put_crec(const Arrays &A, int j, const double *T, uint32_t info)
+{
+    A.rmB[j] = make_double4(T[0], T[1], T[2], (double)info);
+    A.rcC[j] = make_double4(T[3], T[4], T[5], 0.0);
+}
+
 template <int D>
 __device__ __forceinline__ void vec_of(const double4 &r, double *out)
 {
@@ -319,6 +332,7 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
     T.m[j] = S.m[i];
     T.info[j] = S.info[i];
     T.orig[j] = S.orig[i];
+    if (P.disctype == 0) T.q[j] = S.q[i];
     if (P.rigid) {
         const int rr = S.rrank[i];
         T.rrank[j] = rr;
@@ -543,6 +557,11 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
             case 1: put_srec<1>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<1>(A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
             case 2: put_srec<2>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<2>(A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
             default: put_srec<3>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<3>(A, gi, gx, mass, gv, rho, 0.0, ginfo);
+            }
+            if (P.disctype == 0) {
+                // images carry sigma = q = 0 (not copied, MGhostParticleData.cpp:16-28)
+                const double T0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                put_crec<3>(A, gi, T0, ginfo);
             }
             if (P.arm == 1) {
                 const int tb[3] = {i + g.off[0], j + g.off[1], k + g.off[2]};   // the image's cell
@@ -1338,6 +1357,167 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
 }
 
 // ---------------------------------------------------------------------------
+// Classic discretisation (disctype 0).
+// Sweep 1: ClassicStrain (MSolution.cpp:362-413) with the full velocity
+// gradient, rod = (grad v + grad v^T) / 2; with S_CONST the model-9
+// constitutive update of MConstitutive::Constitutive (MConstitutive.cpp:10-42)
+// after RhoUpdate: F3DM9 (:44-68) s = 2 mu (rod - tr/3 I) with mu = av_l rho0,
+// AViscosity (:123-133) q only when tr D < 0 (q persists otherwise), the EOS,
+// StressUp (:210-222) sigma = -p I + s; the momentum record holds
+// T = (sigma - q I) / rho^2.  Symmetric tensors are kept as xx, yy, zz, xy,
+// xz, yz (index sym6).
+// ---------------------------------------------------------------------------
+template <int D, int G = kStrain64Batch>
+__device__ __forceinline__ void strain_classic_body(const Params &P, const Arrays &A, int mask, const int bid)
+{
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = bid * kBlock + threadIdx.x;
+    double crit = 1.0e20;
+    if (i < P.N) {
+        const uint32_t fi = A.info[i];
+        const bool act = fi & kActive;
+        const int mat_i = info_mat(fi);
+        const Mat &mt = P.mat[mat_i];
+        const double dt = ctl->dt;
+        double tr = 0.0, rod[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if ((mask & S_STRAIN) && act) {
+            double xi[D], vi[D], g[D][D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                xi[d] = A.xs[d][i];
+                vi[d] = A.v[d][i];
+#pragma unroll
+                for (int e = 0; e < D; e++) g[d][e] = 0.0;
+            }
+            struct Rec { double4 a, b; };
+            auto load = [&](int j) { return Rec{A.rsA[j], A.rsB[j]}; };
+            for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
+                const double4 ra = rec.a, rb = rec.b;
+                const double volj = ra.w;
+                const PairConst &pc = P.pc[mat_i][P.nmat == 1 ? mat_i : info_mat((uint32_t)rb.w)];
+                double xj[D], vj[D], dw[D];
+                vec_of<D>(ra, xj);
+                vec_of<D>(rb, vj);
+                gradw64<D>(dw, xi, xj, pc);
+#pragma unroll
+                for (int r = 0; r < D; r++)
+#pragma unroll
+                    for (int c = 0; c < D; c++) g[r][c] += volj * (vj[r] - vi[r]) * dw[c];
+            });
+#pragma unroll
+            for (int r = 0; r < D; r++)
+#pragma unroll
+                for (int c = r; c < D; c++) rod[sym6(r, c)] = 0.5 * (g[r][c] + g[c][r]);
+#pragma unroll
+            for (int d = 0; d < D; d++) tr += 0.5 * (g[d][d] + g[d][d]);
+            A.trd[i] = tr;
+#pragma unroll
+            for (int k = 0; k < 6; k++) A.rod[k][i] = rod[k];
+#pragma unroll
+            for (int d = 0; d < D; d++) A.x[d][i] = xi[d] + 0.5 * vi[d] * dt;
+        } else if (mask & S_CONST) {
+            tr = A.trd[i];
+#pragma unroll
+            for (int k = 0; k < 6; k++) rod[k] = A.rod[k][i];
+        }
+        double rho_i = A.rho[i];
+        if (mask & S_CONST) {
+            if (act) {
+                rho_i = rho_i * (1.0 - tr * dt);
+                // F3DM9
+                double sdev[6];
+                const double xmu = mt.nu * mt.rho0;
+                if (fabs(xmu) < 1.e-10) {
+#pragma unroll
+                    for (int k = 0; k < 6; k++) sdev[k] = 0.0;
+                } else {
+                    const double davg = 1.0 / 3.0 * tr;
+#pragma unroll
+                    for (int k = 0; k < 6; k++) sdev[k] = 2 * xmu * (rod[k] - (k < 3 ? davg : 0.0));
+                }
+                // AViscosity: the sound speed before this step's EOS (constant for EOS 28/29)
+                double q = A.q[i];
+                if (tr < 0.0)
+                    q = mt.nu * rho_i * mt.c * mt.h * fabs(tr) + mt.av_q * rho_i * ((mt.h * tr) * (mt.h * tr));
+                A.q[i] = q;
+                const double p = eos_p(mt, rho_i);
+                A.p[i] = p;
+                A.prho2[i] = p / (rho_i * rho_i);
+                double sig[6], T[6];
+                const double r2 = rho_i * rho_i;
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    sig[k] = -(k < 3 ? p : 0.0) + sdev[k];
+                    A.sig[k][i] = sig[k];
+                    T[k] = (sig[k] - (k < 3 ? q : 0.0)) / r2;
+                }
+                double xr[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) xr[d] = A.x[d][i];
+                double4 ra;
+                ra.x = xr[0]; ra.y = D > 1 ? xr[1] : 0.0; ra.z = D > 2 ? xr[2] : 0.0; ra.w = A.m[i];
+                A.rmA[i] = ra;
+                put_crec<D>(A, i, T, fi);
+            }
+            A.rho_new[i] = rho_i;
+            if (mask & S_INPLACE) A.rho[i] = rho_i;
+        }
+        if (mask & S_CRIT) {
+            const double ct = crit_one<D>(P, A, ctl, i, mt, rho_i);
+            if (!(mask & S_CONST)) A.critts_p[i] = ct;
+            if (act && !(fi & kRigid)) crit = ct;
+        }
+    }
+    if (mask & S_CRIT) block_min_to(A, bid, crit);
+}
+
+// Sweep 2: MSolution::ClassicAcceleration (:506-548); a is cleared for
+// computable particles only (:519-521).
+template <int D, int G = kMomentum64Batch>
+__device__ __forceinline__ void momentum_classic_body(const Params &P, const Arrays &A, const int bid)
+{
+    const Ctl *ctl = A.ctl;
+    if (CTL_STOP(ctl)) return;
+    const int i = bid * kBlock + threadIdx.x;
+    if (i >= P.N) return;
+    const uint32_t fi = A.info[i];
+    if (!(fi & kActive)) return;
+    const int mat_i = info_mat(fi);
+    double ai[D], xi[D], Ti[6];
+    {
+        const double4 a = A.rmA[i], b = A.rmB[i], c = A.rcC[i];
+        vec_of<D>(a, xi);
+        Ti[0] = b.x; Ti[1] = b.y; Ti[2] = b.z; Ti[3] = c.x; Ti[4] = c.y; Ti[5] = c.z;
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) ai[d] = 0.0;
+    struct Rec { double4 a, b, c; };
+    auto load = [&](int j) { return Rec{A.rmA[j], A.rmB[j], A.rcC[j]}; };
+    for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
+        const double4 ra = rec.a, rb = rec.b, rc = rec.c;
+        const int mat_j = P.nmat == 1 ? mat_i : info_mat((uint32_t)rb.w);
+        const PairConst &pc = P.pc[mat_i][mat_j];
+        const double Tj[6] = {rb.x, rb.y, rb.z, rc.x, rc.y, rc.z};
+        double xj[D], dw[D];
+        vec_of<D>(ra, xj);
+        gradw64<D>(dw, xi, xj, pc);
+        double ds[D];
+#pragma unroll
+        for (int n = 0; n < D; n++) {
+            ds[n] = 0.0;
+#pragma unroll
+            for (int m = 0; m < D; m++) ds[n] += (Tj[sym6(m, n)] + Ti[sym6(m, n)]) * dw[m];
+        }
+        const double mj = ra.w;
+#pragma unroll
+        for (int d = 0; d < D; d++) ai[d] = ai[d] + mj * ds[d];
+    });
+#pragma unroll
+    for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
+}
+
+// ---------------------------------------------------------------------------
 // Rigid body: MSolution::RigidBodyAcceleration (:1496-1564) with
 // MRigidBody::CalculateCenterOfMass (MRigidBody.cpp:37-49); one block,
 // wave-shuffle FP64 reductions in a fixed order (deterministic).
@@ -1811,6 +1991,8 @@ __global__ __launch_bounds__(64) void kb_advance(const Params *Pt, const Arrays 
     advance_body(Pb, Ab);
 }
 template <int D, int G> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { XCD_BID(P); momentum_body<D, G>(P, A, bid_); }
+template <int D, int G> __global__ __launch_bounds__(kBlock) void k_strain_classic(Params P, Arrays A, int mask) { XCD_BID(P); strain_classic_body<D, G>(P, A, mask, bid_); }
+template <int D, int G> __global__ __launch_bounds__(kBlock) void k_momentum_classic(Params P, Arrays A) { XCD_BID(P); momentum_classic_body<D, G>(P, A, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
@@ -1955,6 +2137,12 @@ void launch_shift(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_strain(const Params &P, const Arrays &A, int mask, hipStream_t st, int batch)
 {
+    if (P.disctype == 0) {
+        DISPATCH_G(batch, kStrain64Batch,
+                   DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_strain_classic<DD, GG>), dim3(xcd_grid(nblk(P.N))),
+                                                         dim3(kBlock), 0, st, P, A, mask)));
+        return;
+    }
     DISPATCH_G(batch, kStrain64Batch,
                DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_strain<DD, GG>), dim3(xcd_grid(nblk(P.N))), dim3(kBlock),
                                                      0, st, P, A, mask)));
@@ -1979,6 +2167,12 @@ void launch_crit_fold(const Params &P, const Arrays &A, hipStream_t st)
 
 void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st, int batch)
 {
+    if (P.disctype == 0) {   // no rigid body on this path (rejected at creation)
+        DISPATCH_G(batch, kMomentum64Batch,
+                   DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_momentum_classic<DD, GG>), dim3(xcd_grid(nblk(P.N))),
+                                                         dim3(kBlock), 0, st, P, A)));
+        return;
+    }
     DISPATCH_G(batch, kMomentum64Batch,
                DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_momentum<DD, GG>), dim3(xcd_grid(nblk(P.N))), dim3(kBlock),
                                                      0, st, P, A)));

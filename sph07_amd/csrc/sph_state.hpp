@@ -79,6 +79,7 @@ struct Ctl {
 // Per-material constants (MMaterial.h:17-40 after CheckDefaults).
 struct Mat {
     double rho0, nu, sigma, c, eos_b, eos_gamma, alpha, h;
+    double av_q;          // quadratic artificial-viscosity coefficient (Classic discretisation)
     int eos, visc_type;
 };
 
@@ -101,6 +102,7 @@ struct Params {
     int ncell_cap;
     int want_mindist;
     int arm;              // 0 FP64, 1 FP32-compensated (selects the records the fused step writes)
+    int disctype;         // 2 Morris (MorrisAcceleration), 0 Classic (full stress, ClassicAcceleration)
     // storage order of the cells: 0 Morton (bit-interleaved code), 1 row-major
     // (z fastest); results do not depend on it, only the memory locality
     int corder;
@@ -140,6 +142,14 @@ struct Arrays {
     //   momentum:   fm[2j] = {r[0..2], m}, fm[2j+1] = {v[0..2], rho}, fmC[j] = {p/rho^2, packed}
     float4 *fs, *fm;
     float2 *fmC;
+    // Classic discretisation (disctype 0): q, the diagonal artificial viscosity
+    // that persists between steps (moved with the particle); the rate of
+    // deformation and stress (symmetric: xx, yy, zz, xy, xz, yz); the momentum
+    // record is rmA = {x, m}, rmB = {T_xx, T_yy, T_zz, info}, rcC = {T_xy,
+    // T_xz, T_yz, 0} with T = (sigma - q I) / rho^2 (zero for images)
+    double *q;
+    double *rod[6], *sig[6];
+    double4 *rcC;
 
     uint32_t *info;
     int *nnbr, *nbr;

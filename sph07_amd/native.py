@@ -24,8 +24,9 @@ ARM_FP64 = 0
 ARM_FP32C = 1
 
 F_X, F_V, F_A, F_RHO, F_P, F_C, F_MASS, F_H, F_TRD, F_MINDIST, F_NORMAL, F_NNORMAL, \
-    F_N, F_KAPPA, F_CRITTS, F_RHOOLD, F_ACTIVE, F_VABS, F_NNBR = range(19)
-_VEC = {F_X, F_V, F_A, F_NORMAL, F_NNORMAL}
+    F_N, F_KAPPA, F_CRITTS, F_RHOOLD, F_ACTIVE, F_VABS, F_NNBR, F_Q, F_SIGMA_DIAG, F_SIGMA_OFF, \
+    F_ROD_DIAG, F_ROD_OFF = range(24)
+_VEC = {F_X, F_V, F_A, F_NORMAL, F_NNORMAL, F_SIGMA_DIAG, F_SIGMA_OFF, F_ROD_DIAG, F_ROD_OFF}
 
 
 class SphError(RuntimeError):
