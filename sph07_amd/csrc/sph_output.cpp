@@ -8,9 +8,9 @@
 //   * a full-precision dump of the state (hexadecimal floats).
 // Fields come from sph_download in deck order.  Quantities the device path
 // does not carry are written as the reference's model-9 fluid would hold
-// them or as zero: the deviatoric stress (the Morris discretisation needs
-// only tr D, so sigma = -p I here), efps, energy, temperature and the
-// boundary flag (never assigned in the reference).
+// them or as zero: efps, energy, temperature and the boundary flag (never
+// assigned in the reference).  The stress is -p I for the Morris
+// discretisation (it needs only tr D) and the full stress for the Classic one.
 #include "../../include/sph07_amd.h"
 
 #include <stdio.h>
@@ -41,7 +41,7 @@ FILE *open_file(Output *o, const char *ext, bool append)
 }
 
 struct Fields {
-    std::vector<double> x, v, a, rho, p, c, m, nnbr;
+    std::vector<double> x, v, a, rho, p, c, m, nnbr, sd, so;   // sd, so: stress diagonal, off-diagonal
 };
 
 int fetch(Output *o, Fields &F)
@@ -55,6 +55,8 @@ int fetch(Output *o, Fields &F)
     F.c.resize(n);
     F.m.resize(n);
     F.nnbr.resize(n);
+    F.sd.resize(3 * n);
+    F.so.resize(3 * n);
     int rc = sph_download(o->ctx, SPH_F_X, F.x.data());
     if (!rc) rc = sph_download(o->ctx, SPH_F_V, F.v.data());
     if (!rc) rc = sph_download(o->ctx, SPH_F_A, F.a.data());
@@ -63,6 +65,8 @@ int fetch(Output *o, Fields &F)
     if (!rc) rc = sph_download(o->ctx, SPH_F_C, F.c.data());
     if (!rc) rc = sph_download(o->ctx, SPH_F_MASS, F.m.data());
     if (!rc) rc = sph_download(o->ctx, SPH_F_NNBR, F.nnbr.data());
+    if (!rc) rc = sph_download(o->ctx, SPH_F_SIGMA_DIAG, F.sd.data());
+    if (!rc) rc = sph_download(o->ctx, SPH_F_SIGMA_OFF, F.so.data());
     if (rc) o->err = sph_last_error(o->ctx);
     return rc;
 }
@@ -215,11 +219,13 @@ int write_vars(Output *o, const Fields &F, double ptime)
         for (; i <= n; i++) {
             if (o->mat[i - 1] != m) continue;
             count++;
-            const double s = -F.p[i - 1];    // sigma = -p I (no deviatoric part on the path)
+            // sigma: -p I for the Morris discretisation, with the deviatoric
+            // part for the Classic one
+            const double *sd = &F.sd[3 * (size_t)(i - 1)], *so = &F.so[3 * (size_t)(i - 1)];
             if (D == 1)
-                fprintf(fsig, "%12.4e%12.4e%12.4e%12.4e%12.4e%12.4e\n", s, 0.0, 0.0, 0.0, 0.0, 0.0);
+                fprintf(fsig, "%12.4e%12.4e%12.4e%12.4e%12.4e%12.4e\n", sd[0], 0.0, 0.0, 0.0, 0.0, 0.0);
             else    // the 2D and 3D writers both write (00, 11, 0, 01, 0, 0)
-                fprintf(fsig, "%12.4e%12.4e%12.4e%12.4e%12.4e%12.4e\n", s, s, 0.0, 0.0, 0.0, 0.0);
+                fprintf(fsig, "%12.4e%12.4e%12.4e%12.4e%12.4e%12.4e\n", sd[0], sd[1], 0.0, so[0], 0.0, 0.0);
             fprintf(frho, "%12.4e", F.rho[i - 1]);
             fprintf(feps, "%12.4e", 0.0);
             fprintf(fegy, "%12.4e", 0.0);
