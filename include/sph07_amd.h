@@ -226,8 +226,17 @@ int  sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int 
                    char *err, int errlen);
 int  sph_batch_remove(void *batch, int slot);
 /* Advances every live scene by up to nsteps steps; *live receives the number
-   of scenes still running. */
+   of scenes still running (scenes admitted since the last step included). */
 int  sph_batch_step(void *batch, int nsteps, int *live);
+/* sph_batch_step in two halves, so that the host admits and prepares scenes
+   while the device steps the batch: sph_batch_launch queues nsteps steps of
+   every live scene (admitted scenes join first) and returns without waiting;
+   sph_batch_sync waits for them and updates the live set.  While steps are
+   queued, sph_batch_add initialises the new scene on its own stream (it joins
+   at the next launch) and sph_batch_remove defers the free to the next sync.
+   A launch while steps are queued syncs first. */
+int  sph_batch_launch(void *batch, int nsteps);
+int  sph_batch_sync(void *batch, int *live);
 int  sph_batch_status(void *batch, int slot, sph_globals *out);
 /* The scene context of a slot, for sph_download / sph_get_neighbours. */
 void *sph_batch_scene(void *batch, int slot);

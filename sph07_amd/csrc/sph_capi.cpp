@@ -1604,7 +1604,8 @@ struct Batch {
     hipStream_t st = nullptr;
     int ndim = 2, arm = SPH_ARM_FP64;
     std::vector<Context *> slots;          // nullptr = free slot
-    std::vector<int> live;                 // resident, not finished, no error
+    std::vector<int> live;                 // resident, not finished, no error (in the device tables)
+    std::vector<int> pending;              // admitted, joining the live set at the next launch
     Params *dP = nullptr;
     Arrays *dA = nullptr, *dA2 = nullptr;   // per slot: current / other buffer set
     int cap_slots = 0;
@@ -1627,6 +1628,13 @@ struct Batch {
     int trial = 0;                       // next candidate to time
     double trial_cost[8] = {0};          // ms per million particle-steps of each candidate
     hipEvent_t ev[2] = {nullptr, nullptr};
+    bool trial_pending = false;          // a timed trial is in flight (read at the next sync)
+    long long trial_steps = 0;           // particle-steps of that trial
+    // steps queued on the stream and not yet synchronised (sph_batch_launch):
+    // scenes removed meanwhile are freed at the next synchronisation, because
+    // the queued kernels still read their tables and control blocks
+    bool inflight = false;
+    std::vector<char> zombie;            // per slot: removed, freed at the next sync
 };
 
 // candidates of the batch tuner: (cell storage order, slots per wave)
@@ -1868,7 +1876,9 @@ int sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int *
         if (err && errlen > 0) snprintf(err, (size_t)errlen, "batched scenes use disctype 2 (Morris)");
         return SPH_ERR_UNSUPPORTED;
     }
-    BCHK(b, hipStreamSynchronize(b->st));
+    // no drain: the scene is initialised on its own stream while queued
+    // batch steps run, and joins the live set at the next launch (the tables
+    // are rebuilt there)
     void *ctx = nullptr;
     int rc = sph_create(desc, b->arm, &ctx, err, errlen);
     if (rc) return rc;
@@ -1886,10 +1896,10 @@ int sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int *
     if (s < 0) {
         s = (int)b->slots.size();
         b->slots.push_back(nullptr);
+        b->zombie.push_back(0);
     }
     b->slots[s] = c;
-    b->live.push_back(s);
-    b->dirty = true;
+    b->pending.push_back(s);
     if (slot) *slot = s;
     return SPH_OK;
 }
@@ -1897,12 +1907,19 @@ int sph_batch_add(void *batch, const sph_scene_desc *desc, int step_limit, int *
 int sph_batch_remove(void *batch, int slot)
 {
     Batch *b = (Batch *)batch;
-    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return SPH_ERROR;
-    BCHK(b, hipStreamSynchronize(b->st));
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot] || b->zombie[slot]) return SPH_ERROR;
+    const auto pend = std::find(b->pending.begin(), b->pending.end(), slot);
+    if (pend != b->pending.end()) {
+        b->pending.erase(pend);    // never in the device tables
+    } else if (b->inflight) {
+        b->zombie[slot] = 1;       // queued steps still read it: freed at the next sync
+        return SPH_OK;
+    } else {
+        b->live.erase(std::remove(b->live.begin(), b->live.end(), slot), b->live.end());
+        b->dirty = true;
+    }
     sph_destroy(b->slots[slot]);
     b->slots[slot] = nullptr;
-    b->live.erase(std::remove(b->live.begin(), b->live.end(), slot), b->live.end());
-    b->dirty = true;
     return SPH_OK;
 }
 
@@ -1913,84 +1930,152 @@ int sph_batch_remove(void *batch, int slot)
 // candidate shape and is timed with events; its cost per particle-step is
 // kept, and after the last candidate the cheapest is fixed.  Every candidate
 // gives the same results, so the tuning is done on the run's own steps.
-long long batch_particles(const Batch *b)
+static long long batch_particles(const Batch *b)
 {
     long long n = 0;
     for (int s : b->live) n += b->slots[s]->N;
     return n;
 }
 
+constexpr int kBatchGraphSteps = 8;    // even: a graph starts and ends on the same parity
+
+// Queues nsteps steps of every live scene on the batch stream (a rebuild of
+// the tables, after admissions or retirements, drains the stream first).
+// With online tuning pending, the first captured graph runs the next
+// candidate shape between two events, read at the next sync.
+// Admitted scenes join the live set (and the tables at the next rebuild);
+// only while no steps are queued, so the tables match the live set at sync.
+static void merge_pending(Batch *b)
+{
+    if (b->pending.empty()) return;
+    b->live.insert(b->live.end(), b->pending.begin(), b->pending.end());
+    b->pending.clear();
+    b->dirty = true;
+}
+
+static int batch_launch(Batch *b, int nsteps)
+{
+    merge_pending(b);
+    if (b->live.empty() || nsteps <= 0) return 0;
+    const long long npart = batch_particles(b);
+    const bool tuning = !b->tuned && !b->trial_pending && npart >= kBatchTuneMinN && nsteps >= kBatchGraphSteps;
+    if (tuning && (b->corder != kBatchCand[b->trial][0] || b->cell_slots != kBatchCand[b->trial][1])) {
+        b->corder = kBatchCand[b->trial][0];
+        b->cell_slots = kBatchCand[b->trial][1];
+        b->dirty = true;
+    }
+    if (b->dirty) {
+        if (batch_rebuild(b)) return SPH_ERR_HIP;
+        b->phase = 0;
+    }
+    if (tuning && !b->ev[0]) {
+        BCHK(b, hipEventCreate(&b->ev[0]));
+        BCHK(b, hipEventCreate(&b->ev[1]));
+    }
+    int k = 0;
+    BatchView views[2] = {b->view, b->view};
+    std::swap(views[1].A, views[1].A2);
+    if (nsteps >= kBatchGraphSteps) {
+        hipGraphExec_t &gx = b->gexec2[b->phase];
+        if (!gx) {
+            hipGraph_t g = nullptr;
+            BCHK(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+            for (int q = 0; q < kBatchGraphSteps; q++) launch_batch_step(views[(b->phase + q) & 1], b->st);
+            BCHK(b, hipStreamEndCapture(b->st, &g));
+            if (hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess) gx = nullptr;
+            hipGraphDestroy(g);
+        }
+        if (gx) {
+            // (timed after the capture, so a trial's cost is its steps only)
+            if (tuning) BCHK(b, hipEventRecord(b->ev[0], b->st));
+            for (; k + kBatchGraphSteps <= nsteps; k += kBatchGraphSteps) {
+                BCHK(b, hipGraphLaunch(gx, b->st));
+                if (tuning && k == 0) {
+                    BCHK(b, hipEventRecord(b->ev[1], b->st));
+                    b->trial_pending = true;
+                    b->trial_steps = npart * kBatchGraphSteps;
+                }
+            }
+        }
+    }
+    // each remaining step sorts every live scene into its other buffer set
+    for (; k < nsteps; k++) {
+        launch_batch_step(views[b->phase], b->st);
+        b->phase ^= 1;
+        for (int s : b->live) b->slots[s]->cur ^= 1;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        b->err = std::string("HIP launch error: ") + hipGetErrorString(e);
+        return SPH_ERR_HIP;
+    }
+    b->inflight = true;
+    return 0;
+}
+
+// Waits for the queued steps, folds a pending tuning trial, updates the live
+// set (finished and failed scenes leave it) and frees the scenes removed
+// while steps were queued.
+static int batch_sync(Batch *b)
+{
+    BCHK(b, hipStreamSynchronize(b->st));
+    b->inflight = false;
+    if (b->trial_pending) {
+        float ms = 0.f;
+        BCHK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+        b->trial_cost[b->trial] = (double)ms * 1e6 / (double)std::max(b->trial_steps, 1LL);
+        b->trial_pending = false;
+        if (++b->trial == kBatchNCand) {
+            int best = 0;
+            for (int k = 1; k < kBatchNCand; k++)
+                if (b->trial_cost[k] < b->trial_cost[best]) best = k;
+            b->corder = kBatchCand[best][0];
+            b->cell_slots = kBatchCand[best][1];
+            b->tuned = true;
+            b->dirty = true;
+        }
+    }
+    if (batch_check(b)) return SPH_ERR_HIP;
+    for (int s = 0; s < (int)b->slots.size(); s++)
+        if (b->zombie[s]) {
+            b->live.erase(std::remove(b->live.begin(), b->live.end(), s), b->live.end());
+            b->dirty = true;
+            sph_destroy(b->slots[s]);
+            b->slots[s] = nullptr;
+            b->zombie[s] = 0;
+        }
+    return 0;
+}
+
 int sph_batch_step(void *batch, int nsteps, int *live)
 {
     Batch *b = (Batch *)batch;
-    constexpr int kBatchGraphSteps = 8;    // even: a graph starts and ends on the same parity
     int done_steps = 0;
+    if (b->inflight && batch_sync(b)) return SPH_ERR_HIP;
+    merge_pending(b);
     while (done_steps < nsteps && !b->live.empty()) {
-        const long long npart = batch_particles(b);
-        const bool tuning = !b->tuned && npart >= kBatchTuneMinN && nsteps - done_steps >= kBatchGraphSteps;
-        if (tuning && (b->corder != kBatchCand[b->trial][0] || b->cell_slots != kBatchCand[b->trial][1])) {
-            b->corder = kBatchCand[b->trial][0];
-            b->cell_slots = kBatchCand[b->trial][1];
-            b->dirty = true;
-        }
-        if (b->dirty) {
-            if (batch_rebuild(b)) return SPH_ERR_HIP;
-            b->phase = 0;
-        }
-        if (tuning && !b->ev[0]) {
-            BCHK(b, hipEventCreate(&b->ev[0]));
-            BCHK(b, hipEventCreate(&b->ev[1]));
-        }
+        const bool tuning = !b->tuned && batch_particles(b) >= kBatchTuneMinN && nsteps - done_steps >= kBatchGraphSteps;
         const int chunk = tuning ? kBatchGraphSteps : std::min(nsteps - done_steps, 32);
-        int k = 0;
-        BatchView views[2] = {b->view, b->view};
-        std::swap(views[1].A, views[1].A2);
-        if (chunk >= kBatchGraphSteps) {
-            hipGraphExec_t &gx = b->gexec2[b->phase];
-            if (!gx) {
-                hipGraph_t g = nullptr;
-                BCHK(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-                for (int q = 0; q < kBatchGraphSteps; q++) launch_batch_step(views[(b->phase + q) & 1], b->st);
-                BCHK(b, hipStreamEndCapture(b->st, &g));
-                if (hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess) gx = nullptr;
-                hipGraphDestroy(g);
-            }
-            // (timed after the capture, so a trial's cost is its steps only)
-            if (tuning) BCHK(b, hipEventRecord(b->ev[0], b->st));
-            if (gx)
-                for (; k + kBatchGraphSteps <= chunk; k += kBatchGraphSteps) BCHK(b, hipGraphLaunch(gx, b->st));
-        }
-        // each remaining step sorts every live scene into its other buffer set
-        for (; k < chunk; k++) {
-            launch_batch_step(views[b->phase], b->st);
-            b->phase ^= 1;
-            for (int s : b->live) b->slots[s]->cur ^= 1;
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) {
-            b->err = std::string("HIP launch error: ") + hipGetErrorString(e);
-            return SPH_ERR_HIP;
-        }
+        if (batch_launch(b, chunk)) return SPH_ERR_HIP;
         done_steps += chunk;
-        if (tuning) {
-            BCHK(b, hipEventRecord(b->ev[1], b->st));
-            BCHK(b, hipEventSynchronize(b->ev[1]));
-            float ms = 0.f;
-            BCHK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
-            b->trial_cost[b->trial] = (double)ms * 1e6 / ((double)npart * chunk);
-            if (++b->trial == kBatchNCand) {
-                int best = 0;
-                for (int k = 1; k < kBatchNCand; k++)
-                    if (b->trial_cost[k] < b->trial_cost[best]) best = k;
-                b->corder = kBatchCand[best][0];
-                b->cell_slots = kBatchCand[best][1];
-                b->tuned = true;
-                b->dirty = true;
-            }
-        }
-        if (batch_check(b)) return SPH_ERR_HIP;
+        if (batch_sync(b)) return SPH_ERR_HIP;
     }
-    if (live) *live = (int)b->live.size();
+    if (live) *live = (int)(b->live.size() + b->pending.size());
+    return SPH_OK;
+}
+
+int sph_batch_launch(void *batch, int nsteps)
+{
+    Batch *b = (Batch *)batch;
+    if (b->inflight && batch_sync(b)) return SPH_ERR_HIP;
+    return batch_launch(b, nsteps) ? SPH_ERR_HIP : SPH_OK;
+}
+
+int sph_batch_sync(void *batch, int *live)
+{
+    Batch *b = (Batch *)batch;
+    if (b->inflight && batch_sync(b)) return SPH_ERR_HIP;
+    if (live) *live = (int)(b->live.size() + b->pending.size());
     return SPH_OK;
 }
 
@@ -2034,13 +2119,16 @@ int sph_batch_finished(void *batch, int *slots, int max)
     Batch *b = (Batch *)batch;
     int n = 0;
     for (int s = 0; s < (int)b->slots.size() && n < max; s++)
-        if (b->slots[s] && std::find(b->live.begin(), b->live.end(), s) == b->live.end()) slots[n++] = s;
+        if (b->slots[s] && !b->zombie[s] && std::find(b->live.begin(), b->live.end(), s) == b->live.end() &&
+            std::find(b->pending.begin(), b->pending.end(), s) == b->pending.end())
+            slots[n++] = s;
     return n;
 }
 
 int sph_batch_live(void *batch)
 {
-    return (int)((Batch *)batch)->live.size();
+    const Batch *b = (const Batch *)batch;
+    return (int)(b->live.size() + b->pending.size());
 }
 
 void sph_batch_destroy(void *batch)

@@ -523,6 +523,20 @@ class Batch:
         self._chk(lib().sph_batch_step(self._h, nsteps, ctypes.byref(live)))
         return live.value
 
+    def launch(self, nsteps: int):
+        """sph_batch_launch: queues nsteps steps and returns at once."""
+        L = lib()
+        L.sph_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self._chk(L.sph_batch_launch(self._h, nsteps))
+
+    def sync(self) -> int:
+        """sph_batch_sync: waits for the queued steps; returns the live count."""
+        L = lib()
+        L.sph_batch_sync.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        live = ctypes.c_int(0)
+        self._chk(L.sph_batch_sync(self._h, ctypes.byref(live)))
+        return live.value
+
     def status(self, slot: int) -> dict:
         g = CGlobals()
         self._chk(lib().sph_batch_status(self._h, slot, ctypes.byref(g)))

@@ -4,7 +4,8 @@ Requests (independent 2D scenes with a step count) arrive on a seeded Poisson
 process.  The loop:
   * admits arrived requests in arrival order while the device-memory budget
     (sph_scene_footprint) and the live-slot limit allow -- a request that
-    arrives while others run joins the batch at the next chunk boundary;
+    arrives while others run is initialised on its own stream while the batch
+    steps, and joins the batch at the next chunk boundary;
   * advances every live scene together, `chunk` steps at a time, in one
     batched launch sequence per step (state stays device-resident);
   * retires finished scenes, frees their memory and slot, and records the
@@ -151,7 +152,10 @@ def serve_batched(requests: List[Request], arm: int, hbm_budget: float = 200e9,
                   max_live: int = 512, chunk: int = 25, keep_results: bool = False,
                   clock: Callable[[], float] = time.perf_counter) -> ServeStats:
     """Continuous batching: admit on arrival (memory and slots permitting),
-    advance all live scenes together, retire as they finish."""
+    advance all live scenes together, retire as they finish.  The host works
+    while the device steps: each chunk is queued (sph_batch_launch) before the
+    finished scenes are freed and the arrived requests are initialised on
+    their own streams; those join the live set at the next chunk boundary."""
     from .native import Batch, F_RHO, F_V, F_X, footprint
     batch = Batch(2, arm)
     pending = sorted(requests, key=lambda r: r.arrival)
@@ -159,8 +163,11 @@ def serve_batched(requests: List[Request], arm: int, hbm_budget: float = 200e9,
     resident = {}
     used = 0
     t0 = clock()
-    while qi < len(pending) or resident:
+
+    def admit() -> int:
+        nonlocal qi, used
         now = clock() - t0
+        n = 0
         while qi < len(pending) and pending[qi].arrival <= now and len(resident) < max_live:
             r = pending[qi]
             fp = footprint(r.deck)
@@ -172,21 +179,35 @@ def serve_batched(requests: List[Request], arm: int, hbm_budget: float = 200e9,
             used += fp
             resident[r.slot] = r
             qi += 1
-        if not resident:
-            if qi < len(pending):
-                time.sleep(max(0.0, pending[qi].arrival - (clock() - t0)))
-            continue
-        batch.step(chunk)
-        for s in batch.finished():
+            n += 1
+        return n
+
+    inflight = False
+    while qi < len(pending) or resident:
+        finished = []
+        if inflight:
+            batch.sync()
+            inflight = False
+            now = clock() - t0
+            for s in batch.finished():
+                r = resident[s]
+                r.t_done = now
+                st = batch.status(s)
+                if st["error"]:
+                    raise RuntimeError(f"request {r.rid} failed on the device (code {st['error']})")
+                if keep_results:
+                    r.result = {f: batch.field(s, f) for f in (F_X, F_V, F_RHO)}
+                finished.append(s)
+        if batch.live:
+            batch.launch(chunk)
+            inflight = True
+        for s in finished:                 # freed at the next sync while steps are queued
             r = resident.pop(s)
-            r.t_done = clock() - t0
-            st = batch.status(s)
-            if st["error"]:
-                raise RuntimeError(f"request {r.rid} failed on the device (code {st['error']})")
-            if keep_results:
-                r.result = {f: batch.field(s, f) for f in (F_X, F_V, F_RHO)}
             batch.remove(s)
             used -= r.fp
+        admitted = admit()
+        if not inflight and not admitted and not batch.live and qi < len(pending):
+            time.sleep(max(0.0, pending[qi].arrival - (clock() - t0)))
     end = clock() - t0
     batch.close()
     first = min(r.arrival for r in requests)

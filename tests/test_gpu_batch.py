@@ -91,3 +91,32 @@ def test_batch_online_tuning_keeps_results(arm):
     assert np.array_equal(alone.field(F_V), tuned.field(st[0], F_V))
     tuned.close()
     fixed.close()
+
+
+@pytest.mark.parametrize("arm", [ARM_FP64, ARM_FP32C])
+def test_admission_and_removal_while_steps_are_queued(arm):
+    """sph_batch_launch returns with steps queued; a scene added meanwhile is
+    initialised on its own stream and joins at the next launch, a scene
+    removed meanwhile is freed at the next sync.  Results equal the scenes
+    run alone."""
+    d1, d2, d3 = scenes.cavity(n=20), scenes.poiseuille(nx=24, ny=24, dx=1e-3 / 24), scenes.droplet(n_box=20)
+    b = Batch(2, arm)
+    s1 = b.add(d1, 40)
+    s3 = b.add(d3, 100)
+    b.launch(16)
+    s2 = b.add(d2, 30)                   # while 16 steps are queued
+    b.remove(s3)                         # queued steps still read it: deferred
+    assert b.sync() == 2                 # s1 live, s2 admitted
+    assert b.status(s2)["timestep"] == 1 # joins at the next launch
+    assert b.status(s1)["timestep"] == 17
+    s4 = b.add(d3, 10)
+    assert s4 == s3                      # the deferred slot was freed at the sync
+    while b.step(8):
+        pass
+    for d, s, n in ((d1, s1, 40), (d2, s2, 30), (d3, s4, 10)):
+        alone = Scene(d, arm)
+        assert alone.Solution(n) == n
+        for f in (F_X, F_V, F_RHO):
+            assert np.array_equal(b.field(s, f), alone.field(f)), (d.name, f)
+        alone.close()
+    b.close()
