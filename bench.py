@@ -140,7 +140,7 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="extra steps run with per-kernel HIP-event timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--small", action="store_true", help="reduced dam break (for quick checks)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for HBM counters)")
