@@ -306,6 +306,8 @@ int alloc_cells(Context *c, int cap)
     rc |= dalloc(c, L, &A.slab_cnt, (size_t)cap);
     rc |= dalloc(c, L, &A.slab_off, (size_t)cap);
     rc |= dalloc(c, L, &A.scan_aux, (size_t)cap / 2048 + 4);
+    // the binning leaves the counts zero after each scan; they start zero
+    if (!rc) HIPCHK(c, hipMemset(A.cell_count, 0, ((size_t)cap + 1) * sizeof(int)));
     share_cells(c);
     return rc;
 }
@@ -399,6 +401,8 @@ int check_device(Context *c)
     if (sync_ctl(c)) return SPH_ERR_HIP;
     Ctl &h = *c->hctl;
     if (h.err == 0) return 0;
+    // a binning stopped by the error may have left counts that no scan cleared
+    HIPCHK(c, hipMemsetAsync(cur(c).cell_count, 0, ((size_t)c->P.ncell_cap + 1) * sizeof(int), c->st));
     if (h.err == kErrGridCap) {
         long long want = (long long)h.ncell * 3 / 2 + 1024;   // h.ncell: cells or codes needed
         if (want > 0x7fffffffLL - 1) {

@@ -424,6 +424,41 @@ __device__ __forceinline__ void scan_down_body(const int *in, int *out, const in
     }
 }
 
+// The down pass of a scene's scan without the single-block pass over the tile
+// sums: each block adds the totals of the tiles before it itself (a few
+// hundred integers, cached) and the last block writes the total to out[n].
+// It also clears the counts it has read, so that the next binning starts
+// from zero without a clearing launch (the counts are zeroed at allocation
+// and by the host after any device error, which can leave a binning unscanned).
+__device__ __forceinline__ void scan_down_clear_body(int *in, int *out, const int *n_ptr, const int *aux, int nblocks,
+                                                     const Ctl *ctl, const int bid)
+{
+    if (CTL_STOP(ctl)) return;
+    const int n = *n_ptr;
+    int pre = 0;
+    for (int b = threadIdx.x; b < bid; b += kBlock) pre += aux[b];
+    int prefix;
+    block_excl_scan(pre, &prefix);
+    const int base = bid * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems], s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+        v[q] = (base + q < n) ? in[base + q] : 0;
+        s += v[q];
+    }
+    int total;
+    int ex = block_excl_scan(s, &total) + prefix;
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+        if (base + q < n) {
+            out[base + q] = ex;
+            in[base + q] = 0;
+        }
+        ex += v[q];
+    }
+    if (bid == nblocks - 1 && threadIdx.x == 0) out[n] = prefix + total;
+}
+
 // ---------------------------------------------------------------------------
 // periodic images: MSolution::GhostSetup / GenerateGhostParticles
 // (MSolution.cpp:811-918), MGhostParticleData::CreateGhostFromReal
@@ -1810,25 +1845,22 @@ __global__ __launch_bounds__(kBlock) void kb_scan_reduce(const Arrays *At, const
     const Arrays &Ab = At[e.x];
     scan_reduce_body(Ab.cell_count, &Ab.ctl->nbucket, Ab.scan_aux, Ab.ctl, e.y);
 }
-__global__ __launch_bounds__(kBlock) void k_scan_aux(int *aux, int nblocks, const int *n_ptr, int *out, const Ctl *ctl)
-{
-    scan_aux_body(aux, nblocks, n_ptr, out, ctl);
-}
 __global__ __launch_bounds__(kBlock) void kb_scan_aux(const Params *Pt, const Arrays *At, const int *sc)
 {
     SCENE_OF(sc[blockIdx.x]);
     const int nb = (Pb.ncell_cap + kScanTile - 1) / kScanTile;
     scan_aux_body(Ab.scan_aux, nb, &Ab.ctl->nbucket, Ab.cell_start, Ab.ctl);
 }
-__global__ __launch_bounds__(kBlock) void k_scan_down(const int *in, int *out, const int *n_ptr, const int *aux, const Ctl *ctl)
-{
-    scan_down_body(in, out, n_ptr, aux, ctl, blockIdx.x);
-}
 __global__ __launch_bounds__(kBlock) void kb_scan_down(const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     const Arrays &Ab = At[e.x];
     scan_down_body(Ab.cell_count, Ab.cell_start, &Ab.ctl->nbucket, Ab.scan_aux, Ab.ctl, e.y);
+}
+__global__ __launch_bounds__(kBlock) void k_scan_down_clear(int *in, int *out, const int *n_ptr, const int *aux, int nblocks,
+                                                            const Ctl *ctl)
+{
+    scan_down_clear_body(in, out, n_ptr, aux, nblocks, ctl, blockIdx.x);
 }
 __global__ void k_reset_ngp(Ctl *ctl) { reset_ngp_body(ctl); }
 __global__ void kb_reset_ngp(const Arrays *At, const int *sc, int ns)
@@ -2051,20 +2083,23 @@ static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
     default: { constexpr int DD = 3; CALL; } break;            \
     }
 
-void launch_scan(const Params &P, const Arrays &A, const int *in, int *out, const int *n_ptr,
-                 int cap, hipStream_t st)
+// exclusive scan of the counts into the starts (total at out[n]); the counts
+// are left zero
+void launch_scan(const Params &P, const Arrays &A, int *in, int *out, const int *n_ptr, int cap, hipStream_t st)
 {
     const int nb = (cap + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, st, in, n_ptr, A.scan_aux, A.ctl);
-    hipLaunchKernelGGL(k_scan_aux, dim3(1), dim3(kBlock), 0, st, A.scan_aux, nb, n_ptr, out, A.ctl);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, st, in, out, n_ptr, A.scan_aux, A.ctl);
+    hipLaunchKernelGGL(k_scan_down_clear, dim3(nb), dim3(kBlock), 0, st, in, out, n_ptr, A.scan_aux, nb, A.ctl);
 }
 
 void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st)
 {
     hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1), 0, st, P, A.ctl);
-    hipLaunchKernelGGL(k_zero_cells, dim3(nblk(P.ncell_cap + 1) < 1024 ? nblk(P.ncell_cap + 1) : 1024),
-                       dim3(kBlock), 0, st, P, A);
+    // the counts start at zero (left so by the previous scan); the image
+    // counts of periodic scenes are cleared here
+    if (P.periodic)
+        hipLaunchKernelGGL(k_zero_cells, dim3(nblk(P.ncell_cap + 1) < 1024 ? nblk(P.ncell_cap + 1) : 1024),
+                           dim3(kBlock), 0, st, P, A);
     DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_bin_count<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
     launch_scan(P, A, A.cell_count, A.cell_start, &A.ctl->nbucket, P.ncell_cap, st);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A);
