@@ -450,17 +450,21 @@ int run_phase(Context *c, F &&f)
 
 // MSolution::SetupLinkedList: binning, then the move of the persistent
 // fields into cell order in the other buffer set, which becomes current.
-void sort_step(Context *c)
+void sort_step(Context *c, bool grid = true)
 {
-    launch_setup_linked_list(c->P, cur(c), c->st);
+    launch_setup_linked_list(c->P, cur(c), c->st, grid);
     launch_permute(c->P, cur(c), c->AA[c->cur ^ 1], c->st);
     c->cur ^= 1;
 }
 
-void fused_step(Context *c)
+// A chain of fused steps (first .. last) sets up each binning's grid in the
+// previous step's k_finish, one launch fewer per step; the chain's first step
+// sets up its own, and its last leaves the grid of its own binning in place
+// for the downloads that follow.
+void fused_step(Context *c, bool first = true, bool last = true)
 {
     const Params &P = c->P;
-    timed(c, K_BIN, [&] { sort_step(c); });
+    timed(c, K_BIN, [&] { sort_step(c, first); });
     const Arrays &A = cur(c);
     hipStream_t st = c->st;
     if (P.periodic) timed(c, K_IMAGES, [&] { launch_ghost_setup(P, A, st); });
@@ -480,7 +484,7 @@ void fused_step(Context *c)
         }
     });
     timed(c, K_INTEGRATE, [&] { launch_integrate(P, A, M_BC | M_VEL | M_MOVE | M_RHO, st); });
-    timed(c, K_FINISH, [&] { launch_finish(P, A, 1, 1, st); });
+    timed(c, K_FINISH, [&] { launch_finish(P, A, 1, 1, st, last ? 0 : 1); });
 }
 
 // ---- host part of MSimulationInit::Initialize -------------------------------
@@ -989,7 +993,7 @@ int build_graph(Context *c)
     hipGraph_t g = nullptr;
     const int from = c->cur;
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < kGraphSteps; k++) fused_step(c);
+    for (int k = 0; k < kGraphSteps; k++) fused_step(c, k == 0, k == kGraphSteps - 1);
     HIPCHK(c, hipStreamEndCapture(c->st, &g));
     hipError_t e = hipGraphInstantiate(&c->gexec[from], g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -1152,7 +1156,7 @@ int sph_solution(void *ctx, int max_steps, int *steps_done)
                 for (; k + c->gsteps <= chunk; k += c->gsteps)
                     HIPCHK(c, hipGraphLaunch(c->gexec[c->cur], c->st));
         }
-        for (; k < chunk; k++) fused_step(c);
+        for (const int k1 = k; k < chunk; k++) fused_step(c, k == k1, k == chunk - 1);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             c->err = std::string("HIP launch error: ") + hipGetErrorString(e);

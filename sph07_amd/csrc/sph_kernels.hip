@@ -1764,7 +1764,7 @@ __device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A,
 // bounding box of MoveParticles into coord_minmax; with end != 0 also the
 // step counter and stop test of MSolution::Solution (:317-319)
 // one wave: lane l reduces and resets bounding-box slot l
-__device__ __forceinline__ void finish_body(const Params &P, const Arrays &A, int bbox, int end)
+__device__ __forceinline__ void finish_body(const Params &P, const Arrays &A, int bbox, int end, int next_grid = 0)
 {
     Ctl *ctl = A.ctl;
     if (CTL_STOP(ctl)) return;
@@ -1799,6 +1799,9 @@ __device__ __forceinline__ void finish_body(const Params &P, const Arrays &A, in
         if (ctl->ptime - ctl->dt >= P.endtime) ctl->done = 1;
         if (ctl->step_limit > 0 && ctl->timestep - 1 >= ctl->step_limit) ctl->done = 1;
     }
+    // the next binning's grid, from the bounding box just written (lane 0
+    // wrote it; grid_setup_body returns if the loop has stopped)
+    if (next_grid && lane == 0) grid_setup_body(P, ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -2045,7 +2048,10 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_integrate(const Pa
     SCENE_OF(e.x);
     integrate_body<D>(Pb, Ab, mask, e.y);
 }
-__global__ __launch_bounds__(64) void k_finish(Params P, Arrays A, int bbox, int end) { finish_body(P, A, bbox, end); }
+__global__ __launch_bounds__(64) void k_finish(Params P, Arrays A, int bbox, int end, int next_grid)
+{
+    finish_body(P, A, bbox, end, next_grid);
+}
 // one wave per scene
 __global__ __launch_bounds__(64) void kb_finish(const Params *Pt, const Arrays *At, const int *sc, int bbox, int end)
 {
@@ -2092,9 +2098,9 @@ void launch_scan(const Params &P, const Arrays &A, int *in, int *out, const int 
     hipLaunchKernelGGL(k_scan_down_clear, dim3(nb), dim3(kBlock), 0, st, in, out, n_ptr, A.scan_aux, nb, A.ctl);
 }
 
-void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st)
+void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st, bool grid)
 {
-    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1), 0, st, P, A.ctl);
+    if (grid) hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1), 0, st, P, A.ctl);
     // the counts start at zero (left so by the previous scan); the image
     // counts of periodic scenes are cleared here
     if (P.periodic)
@@ -2242,9 +2248,9 @@ void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st
     DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_integrate<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A, mask));
 }
 
-void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st)
+void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st, int next_grid)
 {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, P, A, bbox, end);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, P, A, bbox, end, next_grid);
 }
 
 void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st)

@@ -74,7 +74,9 @@ enum { M_STRAIN = 1, M_CONST = 2, M_CRIT = 4, M_INPLACE = 8 };
 // masks of launch_integrate
 enum { M_BC = 1, M_VEL = 2, M_MOVE = 4, M_RHO = 8 };
 
-void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st);
+// grid = false: the grid of this binning was set up by the previous step's
+// launch_finish (next_grid)
+void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st, bool grid = true);
 void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st);
 void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st);
 // Launch shapes that change timing, never results (chosen per scene by the
@@ -101,7 +103,9 @@ void launch_crit_reset(const Arrays &A, hipStream_t st);
 void launch_rigid_kin(const Params &P, const Arrays &A, hipStream_t st);
 void launch_rigid_reduce(const Params &P, const Arrays &A, int rigid_kin, hipStream_t st);
 void launch_integrate(const Params &P, const Arrays &A, int mask, hipStream_t st);
-void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st);
+// next_grid: also sets up the grid of the next step's binning from the new
+// bounding box (a chained step then skips k_grid_setup)
+void launch_finish(const Params &P, const Arrays &A, int bbox, int end, hipStream_t st, int next_grid = 0);
 void launch_canon(const Params &P, const Arrays &A, long long *out, int maxnbr, hipStream_t st);
 // the reference's linked-list arrays (sph_llgrid, llpointer of reals and
 // images, image sources) from the last binning; any output may be null
