@@ -239,7 +239,6 @@ int alloc_particles(Context *c, int gcap)
     rc |= dalloc(c, L, &A.rho_new, ncap);
     rc |= dalloc(c, L, &A.p, ncap);
     rc |= dalloc(c, L, &A.prho2, ncap);
-    rc |= dalloc(c, L, &A.vol, ncap);
     rc |= dalloc(c, L, &A.trd, c->N);
     rc |= dalloc(c, L, &A.mindist, c->N);
     rc |= dalloc(c, L, &A.critts_p, c->N);
@@ -267,7 +266,7 @@ int alloc_particles(Context *c, int gcap)
     }
     B.rsA = A.rsA; B.rsB = A.rsB; B.rmA = A.rmA; B.rmB = A.rmB; B.rmC = A.rmC;
     B.fs = A.fs; B.fm = A.fm; B.fmC = A.fmC;
-    B.rho_new = A.rho_new; B.p = A.p; B.prho2 = A.prho2; B.vol = A.vol;
+    B.rho_new = A.rho_new; B.p = A.p; B.prho2 = A.prho2;
     B.trd = A.trd; B.mindist = A.mindist; B.critts_p = A.critts_p; B.rhoold = A.rhoold;
     B.Nf = A.Nf; B.kappa = A.kappa; B.nnbr = A.nnbr; B.nbr = A.nbr;
     B.cell_of = A.cell_of; B.slot = A.slot; B.tmp = A.tmp; B.occ = A.occ; B.ovf = A.ovf;
@@ -344,7 +343,6 @@ int grow_particles(Context *c, int new_gcap)
     rc |= cp(A.rho_new, old.rho_new, n * 8);
     rc |= cp(A.p, old.p, n * 8);
     rc |= cp(A.prho2, old.prho2, n * 8);
-    rc |= cp(A.vol, old.vol, n * 8);
     rc |= cp(A.m, old.m, n * 8);
     rc |= cp(A.info, old.info, n * 4);
     rc |= cp(A.orig, old.orig, n * 4);
@@ -1861,10 +1859,10 @@ long long sph_scene_footprint(const sph_scene_desc *d)
     const long long D = d->ndim;
     const long long maxnbr = d->maxnbr > 0 ? d->maxnbr : 40;
     // per slot (reals and images): x, v in both buffer sets, xs, rho and m in
-    // both sets, info in both sets, the two gather records, rho_new/p/prho2/vol,
+    // both sets, info in both sets, the two gather records, rho_new/p/prho2,
     // the neighbour list
     // (the list is counted twice: its chunk stride is a power of two >= slots)
-    const long long per_slot = 32 * D + 8 * D + 32 + 8 + 144 + 32 + 8 * ((long long)maxnbr + 16);
+    const long long per_slot = 32 * D + 8 * D + 32 + 8 + 144 + 24 + 8 * ((long long)maxnbr + 16);
     // per real particle: a in both sets, orig, trd/mindist/critts, nnbr, sort
     // scratch, CSF fields
     const long long per_real = 16 * D + 8 + 24 + 4 + 12 + (d->surface_tension ? 16 * D + 16 : 0);

@@ -187,8 +187,12 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
             double xi[D], vi[D], xr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                xi[d] = A.xs[d][i];
+                // MSolution::Strain's half step back and forward again
+                // (:334-338, :353-357), each rounded as written (no FMA): the
+                // back step is the one k_permute used for the records
+#pragma clang fp contract(off)
                 vi[d] = A.v[d][i];
+                xi[d] = A.x[d][i] - 0.5 * vi[d] * dt;
                 xr[d] = xi[d] + 0.5 * vi[d] * dt;
                 A.x[d][i] = xr[d];
             }

@@ -338,7 +338,6 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
         T.rrank[j] = rr;
         if (rr >= 0) T.rigid_idx[rr] = j;
     }
-    T.vol[j] = S.m[i] / S.rho[i];
     if (!moving) return;
     // cell code of every sorted slot (the inactive bucket is ncode)
     T.occ[j] = S.cell_of[i];
@@ -355,7 +354,8 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
         for (int d = 0; d < D; d++) {
             vi[d] = S.v[d][i];
             xs[d] = act ? S.x[d][i] - 0.5 * vi[d] * dt : S.x[d][i];
-            T.xs[d][j] = xs[d];
+            // the FP32c continuity sweep derives xs from x itself
+            if (P.arm != 1) T.xs[d][j] = xs[d];
         }
         const double vol = S.m[i] / S.rho[i];
         if (P.arm == 1) {
@@ -578,7 +578,6 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
             A.m[gi] = mass;
             A.rho[gi] = rho;
             A.rho_new[gi] = rho;
-            A.vol[gi] = mass / rho;
             A.p[gi] = 0.0;        // images keep p = 0 (not copied, :28)
             A.prho2[gi] = 0.0;
             const uint32_t ginfo = (A.info[src] & (kMatMask | kRigid | (0xFFFFu << kColorShift))) | kGhost;
@@ -636,7 +635,6 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
     Ctl *ctl = A.ctl;
     const uint32_t fi = A.info[i];
     const double mi = A.m[i];
-    A.vol[i] = mi / A.rho[i];
     if (!(fi & kActive)) { A.nnbr[i] = 0; return; }
     double xi[D];
 #pragma unroll

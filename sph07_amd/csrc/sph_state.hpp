@@ -127,7 +127,7 @@ struct Params {
 
 struct Arrays {
     double *x[3], *xs[3], *v[3], *a[3];
-    double *rho, *rho_new, *p, *prho2, *vol, *m;
+    double *rho, *rho_new, *p, *prho2, *m;
     double *trd, *mindist, *critts_p, *rhoold;
     double *nrm[3], *nn[3], *Nf, *kappa;
     // gather records of the pair sweeps (one or two 32-byte loads per
