@@ -196,6 +196,8 @@ int alloc_particles(Context *c, int gcap)
             rc |= dalloc(c, L, &S->v[d], ncap);
             rc |= dalloc(c, L, &S->a[d], c->N);
         }
+        if (d == 0)
+            for (Arrays *S : {&A, &B}) rc |= dalloc(c, L, &S->aa, c->N);
         rc |= dalloc(c, L, &A.xs[d], ncap);
         if (P.st) {
             rc |= dalloc(c, L, &A.nrm[d], c->N);
@@ -334,6 +336,7 @@ int grow_particles(Context *c, int new_gcap)
         rc |= cp(A.xs[d], old.xs[d], n * 8);
         rc |= cp(A.v[d], old.v[d], n * 8);
         rc |= cp(A.a[d], old.a[d], n * 8);
+        if (d == 0) rc |= cp(A.aa, old.aa, n * 8);
         if (c->P.st) {
             rc |= cp(A.nrm[d], old.nrm[d], n * 8);
             rc |= cp(A.nn[d], old.nn[d], n * 8);
@@ -448,10 +451,10 @@ int run_phase(Context *c, F &&f)
 
 // MSolution::SetupLinkedList: binning, then the move of the persistent
 // fields into cell order in the other buffer set, which becomes current.
-void sort_step(Context *c, bool grid = true)
+void sort_step(Context *c, bool grid = true, bool move_a = true)
 {
     launch_setup_linked_list(c->P, cur(c), c->st, grid);
-    launch_permute(c->P, cur(c), c->AA[c->cur ^ 1], c->st);
+    launch_permute(c->P, cur(c), c->AA[c->cur ^ 1], c->st, move_a);
     c->cur ^= 1;
 }
 
@@ -462,7 +465,9 @@ void sort_step(Context *c, bool grid = true)
 void fused_step(Context *c, bool first = true, bool last = true)
 {
     const Params &P = c->P;
-    timed(c, K_BIN, [&] { sort_step(c, first); });
+    // the Morris momentum sweeps rewrite every a; the Classic one only the
+    // computable particles' (the others keep theirs, so a moves)
+    timed(c, K_BIN, [&] { sort_step(c, first, P.disctype == 0); });
     const Arrays &A = cur(c);
     hipStream_t st = c->st;
     if (P.periodic) timed(c, K_IMAGES, [&] { launch_ghost_setup(P, A, st); });
@@ -749,6 +754,7 @@ int create_impl(Context *c, const sph_scene_desc *d)
         HIPCHK(c, hipMemcpy(A.xs[dd], hx[dd].data(), N * 8, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(A.v[dd], hv[dd].data(), N * 8, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemset(A.a[dd], 0, N * 8));
+        if (dd == 0) HIPCHK(c, hipMemset(A.aa, 0, N * 8));
         if (P.st) {
             HIPCHK(c, hipMemset(A.nrm[dd], 0, N * 8));
             HIPCHK(c, hipMemset(A.nn[dd], 0, N * 8));
@@ -790,6 +796,7 @@ int create_impl(Context *c, const sph_scene_desc *d)
     rc = run_phase(c, [&] {
         sort_step(c);
         launch_neighbours(c->P, cur(c), c->st);
+        launch_accel_sq(c->P, cur(c), c->st);
         launch_strain(c->P, cur(c), M_CRIT, c->st);
         launch_crit_fold(c->P, cur(c), c->st);
     });
@@ -932,6 +939,7 @@ int sph_calc_critical_timestep(void *ctx, double *critts)
 {
     Context *c = ctx_of(ctx);
     int rc = run_phase(c, [&] {
+        launch_accel_sq(c->P, cur(c), c->st);
         launch_strain(c->P, cur(c), M_CRIT, c->st);
         launch_crit_fold(c->P, cur(c), c->st);
     });
@@ -1863,9 +1871,9 @@ long long sph_scene_footprint(const sph_scene_desc *d)
     // the neighbour list
     // (the list is counted twice: its chunk stride is a power of two >= slots)
     const long long per_slot = 32 * D + 8 * D + 32 + 8 + 144 + 24 + 8 * ((long long)maxnbr + 16);
-    // per real particle: a in both sets, orig, trd/mindist/critts, nnbr, sort
+    // per real particle: a and aa in both sets, orig, trd/mindist/critts, nnbr, sort
     // scratch, CSF fields
-    const long long per_real = 16 * D + 8 + 24 + 4 + 12 + (d->surface_tension ? 16 * D + 16 : 0);
+    const long long per_real = 16 * D + 16 + 8 + 24 + 4 + 12 + (d->surface_tension ? 16 * D + 16 : 0);
     const long long img = d->boundary ? n / 4 + 4096 : 0;
     return (n + img) * per_slot + n * per_real + n * 16 /* cell arrays, estimate */ + 65536;
 }

@@ -88,10 +88,8 @@ __device__ __forceinline__ double crit_one32(const Params &P, const Arrays &A, c
         ct = 0.25 * h / mt.c;
         const double b = 0.125 * (h * h) / mt.nu;
         ct = ct < b ? ct : b;
-        double aabs = 0.0;
-#pragma unroll
-        for (int d = 0; d < D; d++) aabs += A.a[d][i] * A.a[d][i];
-        aabs = sqrt(aabs);
+        // |a|^2 summed over the axes in order, after the last step (aa)
+        const double aabs = sqrt(A.aa[i]);
         const double c3 = 0.25 * sqrt(h / aabs);
         ct = ct < c3 ? ct : c3;
         if (P.st) {

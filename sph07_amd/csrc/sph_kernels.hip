@@ -301,7 +301,8 @@ __device__ __forceinline__ void bin_scatter_body(const Params &P, const Arrays &
 // buffer set).  After an error or the stop test the move is the identity, so
 // the two buffer sets still alternate once per queued step.
 template <int D>
-__device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, const Arrays &T, const int bid)
+__device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, const Arrays &T, const int bid,
+                                             bool move_a = true)
 {
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
@@ -326,8 +327,14 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
     for (int d = 0; d < D; d++) {
         T.x[d][j] = S.x[d][i];
         T.v[d][j] = S.v[d][i];
-        T.a[d][j] = S.a[d][i];
     }
+    // a fused step rewrites every a in its momentum sweep before reading it
+    // (the critical step reads aa): a moves only for the other paths, and
+    // when the loop has stopped (nothing rewrites it then)
+    if (move_a || !moving)
+#pragma unroll
+        for (int d = 0; d < D; d++) T.a[d][j] = S.a[d][i];
+    T.aa[j] = S.aa[i];
     T.rho[j] = S.rho[i];
     T.m[j] = S.m[i];
     T.info[j] = S.info[i];
@@ -1079,10 +1086,8 @@ __device__ __forceinline__ double crit_one(const Params &P, const Arrays &A, con
         ct = 0.25 * h / mt.c;
         const double b = 0.125 * (h * h) / mt.nu;
         ct = ct < b ? ct : b;
-        double aabs = 0.0;
-#pragma unroll
-        for (int d = 0; d < D; d++) aabs += A.a[d][i] * A.a[d][i];
-        aabs = sqrt(aabs);
+        // |a|^2 summed over the axes in order, after the last step (aa)
+        const double aabs = sqrt(A.aa[i]);
         const double c3 = 0.25 * sqrt(h / aabs);
         ct = ct < c3 ? ct : c3;
         if (P.st) {
@@ -1671,16 +1676,23 @@ __device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A,
         const uint32_t fi = A.info[i];
         const bool act = fi & kActive, rig = fi & kRigid;
         const int disp = info_disp(fi);
-        if ((mask & I_BC) && act) {
-            double a[D];
+        double a[D];
 #pragma unroll
-            for (int d = 0; d < D; d++) a[d] = A.a[d][i];
+        for (int d = 0; d < D; d++) a[d] = (mask & (I_BC | I_VEL)) ? A.a[d][i] : 0.0;
+        if ((mask & I_BC) && act) {
             if (P.baseaccel && !rig)
 #pragma unroll
                 for (int d = 0; d < D; d++) a[d] += P.base_a[d];
             constrain(a, disp, D);
 #pragma unroll
             for (int d = 0; d < D; d++) A.a[d][i] = a[d];
+        }
+        if (mask & I_VEL) {
+            // the next critical step's |a|^2 (CalcCriticalTimestep's sum, :41-44)
+            double s = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; d++) s += a[d] * a[d];
+            A.aa[i] = s;
         }
         double x[D], v[D];
 #pragma unroll
@@ -1692,7 +1704,7 @@ __device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A,
             if (!rig) {
                 const double dtn = 0.5 * (ctl->dt + ctl->dtold);
 #pragma unroll
-                for (int d = 0; d < D; d++) v[d] = v[d] + dtn * A.a[d][i];
+                for (int d = 0; d < D; d++) v[d] = v[d] + dtn * a[d];
             } else {
                 double rl[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -1900,11 +1912,24 @@ __global__ __launch_bounds__(kBlock) void kb_ghost_emit(const Params *Pt, const 
     SCENE_OF(sc[blockIdx.x / nb]);
     if (has_pass(Pb, m, side)) ghost_emit_body(Pb, Ab, m, side, blockIdx.x % nb, nb);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void k_permute(Params P, Arrays S, Arrays T) { permute_body<D>(P, S, T, blockIdx.x); }
+template <int D> __global__ __launch_bounds__(kBlock) void k_permute(Params P, Arrays S, Arrays T, int move_a)
+{
+    permute_body<D>(P, S, T, blockIdx.x, move_a != 0);
+}
+// aa from the current accelerations (before a critical step outside the fused loop)
+template <int D> __global__ __launch_bounds__(kBlock) void k_accel_sq(Params P, Arrays A)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.N) return;
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) s += A.a[d][i] * A.a[d][i];
+    A.aa[i] = s;
+}
 template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Params *Pt, const Arrays *At, const Arrays *At2, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    permute_body<D>(Pt[e.x], At[e.x], At2[e.x], e.y);
+    permute_body<D>(Pt[e.x], At[e.x], At2[e.x], e.y, false);   // batched scenes are Morris (a rewritten)
 }
 #define XCD_BID(P)                                       \
     const int nb_ = ((P).N + kBlock - 1) / kBlock;       \
@@ -2109,9 +2134,14 @@ void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st, 
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A);
 }
 
-void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st)
+void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st, bool move_a)
 {
-    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_permute<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, S, T));
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_permute<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, S, T, move_a ? 1 : 0));
+}
+
+void launch_accel_sq(const Params &P, const Arrays &A, hipStream_t st)
+{
+    DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_accel_sq<DD>, dim3(nblk(P.N)), dim3(kBlock), 0, st, P, A));
 }
 
 void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)

@@ -77,7 +77,11 @@ enum { M_BC = 1, M_VEL = 2, M_MOVE = 4, M_RHO = 8 };
 // grid = false: the grid of this binning was set up by the previous step's
 // launch_finish (next_grid)
 void launch_setup_linked_list(const Params &P, const Arrays &A, hipStream_t st, bool grid = true);
-void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st);
+// move_a = false: the accelerations stay behind (a fused Morris step rewrites
+// all of them before any read)
+void launch_permute(const Params &P, const Arrays &S, const Arrays &T, hipStream_t st, bool move_a = true);
+// aa (|a|^2 per particle) from the current accelerations
+void launch_accel_sq(const Params &P, const Arrays &A, hipStream_t st);
 void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st);
 // Launch shapes that change timing, never results (chosen per scene by the
 // autotuner of sph_capi.cpp; defaults measured on the 3D dam break).

@@ -128,6 +128,9 @@ struct Params {
 struct Arrays {
     double *x[3], *xs[3], *v[3], *a[3];
     double *rho, *rho_new, *p, *prho2, *m;
+    // |a|^2 after the last step's BoundCond (what the critical step of option 3
+    // reads; written with the velocity update, moved with the particle)
+    double *aa;
     double *trd, *mindist, *critts_p, *rhoold;
     double *nrm[3], *nn[3], *Nf, *kappa;
     // gather records of the pair sweeps (one or two 32-byte loads per
