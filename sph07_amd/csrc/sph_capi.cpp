@@ -680,6 +680,15 @@ int create_impl(Context *c, const sph_scene_desc *d)
         P.mcompact = uniform ? ((d->nummat == 1 && !P.rigid) ? 2 : 1) : 0;
         for (int m = 0; m <= kMaxMat; m++) P.mass32[m] = mref[m] > 0.0 ? (float)mref[m] : 0.0f;
     }
+    // compact FP64 momentum records: one material with one particle mass, no
+    // rigid body, the Morris discretisation (the mass is the particles' own)
+    {
+        bool one = c->arm == SPH_ARM_FP64 && d->nummat == 1 && !P.rigid && d->disctype == 2;
+        for (int i = 1; i < N && one; i++)
+            if (hm[i] != hm[0]) one = false;
+        P.mcompact64 = one ? 1 : 0;
+        P.mass64 = one ? hm[0] : 0.0;
+    }
     P.n_rigid = (int)rigid_idx.size();
 
     // ---- capacities ----

@@ -124,15 +124,15 @@ __device__ __forceinline__ void put_srec(const Arrays &A, int j, const double *x
 }
 
 template <int D>
-__device__ __forceinline__ void put_mrec(const Arrays &A, int j, const double *x, double m,
+__device__ __forceinline__ void put_mrec(const Params &P, const Arrays &A, int j, const double *x, double m,
                                          const double *v, double rho, double prho2, uint32_t info)
 {
     double4 a, b;
-    a.x = x[0]; a.y = D > 1 ? x[1] : 0.0; a.z = D > 2 ? x[2] : 0.0; a.w = m;
+    a.x = x[0]; a.y = D > 1 ? x[1] : 0.0; a.z = D > 2 ? x[2] : 0.0; a.w = P.mcompact64 ? prho2 : m;
     b.x = v[0]; b.y = D > 1 ? v[1] : 0.0; b.z = D > 2 ? v[2] : 0.0; b.w = rho;
     A.rmA[j] = a;
     A.rmB[j] = b;
-    A.rmC[j] = make_double2(prho2, (double)info);
+    if (!P.mcompact64) A.rmC[j] = make_double2(prho2, (double)info);
 }
 
 // symmetric tensors of the Classic discretisation: xx, yy, zz, xy, xz, yz
@@ -595,9 +595,9 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
                 gv[d] = A.v[d][gi];
             }
             switch (P.ndim) {
-            case 1: put_srec<1>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<1>(A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
-            case 2: put_srec<2>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<2>(A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
-            default: put_srec<3>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<3>(A, gi, gx, mass, gv, rho, 0.0, ginfo);
+            case 1: put_srec<1>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<1>(P, A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
+            case 2: put_srec<2>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<2>(P, A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
+            default: put_srec<3>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<3>(P, A, gi, gx, mass, gv, rho, 0.0, ginfo);
             }
             if (P.disctype == 0) {
                 // images carry sigma = q = 0 (not copied, MGhostParticleData.cpp:16-28)
@@ -1187,7 +1187,7 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
                     xr[d] = A.x[d][i];
                     vr[d] = A.v[d][i];
                 }
-                put_mrec<D>(A, i, xr, A.m[i], vr, rho_i, pr, fi);
+                put_mrec<D>(P, A, i, xr, A.m[i], vr, rho_i, pr, fi);
             }
             A.rho_new[i] = rho_i;
             if (mask & S_INPLACE) A.rho[i] = rho_i;
@@ -1320,7 +1320,7 @@ __device__ __forceinline__ void advance_body(const Params &P, const Arrays &A)
 // Monaghan (1992) artificial viscosity, Pi_ij = -alpha*cbar*mu_ij/rhobar for
 // approaching pairs (extension; the reference's visc_type field is unused).
 // ---------------------------------------------------------------------------
-template <int D, int G = kMomentum64Batch>
+template <int D, int G = kMomentum64Batch, bool CMP = false>
 __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, const int bid)
 {
     const Ctl *ctl = A.ctl;
@@ -1343,14 +1343,17 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
             vi[d] = A.v[d][i];
         }
         struct Rec { double4 a, b; double2 c; };
-        auto load = [&](int j) { return Rec{A.rmA[j], A.rmB[j], A.rmC[j]}; };
+        auto load = [&](int j) {
+            return CMP ? Rec{A.rmA[j], A.rmB[j], make_double2(0.0, 0.0)} : Rec{A.rmA[j], A.rmB[j], A.rmC[j]};
+        };
         for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
             const double4 ra = rec.a, rb = rec.b;
-            const double2 rc = rec.c;
-            const uint32_t fj = (uint32_t)rc.y;
-            const int mat_j = P.nmat == 1 ? mat_i : info_mat(fj);
+            // the compact record: one material and mass, no rigid body; p/rho^2 in a.w
+            const double2 rc = CMP ? make_double2(ra.w, 0.0) : rec.c;
+            const uint32_t fj = CMP ? 0u : (uint32_t)rc.y;
+            const int mat_j = (CMP || P.nmat == 1) ? mat_i : info_mat(fj);
             const PairConst &pc = P.pc[mat_i][mat_j];
-            const double mj = ra.w, rho_j = rb.w;
+            const double mj = CMP ? P.mass64 : ra.w, rho_j = rb.w;
             double xj[D], vj[D], dw[D];
             vec_of<D>(ra, xj);
             vec_of<D>(rb, vj);
@@ -1364,7 +1367,7 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
             const double sp = -mj * (rc.x + pr_i);
 #pragma unroll
             for (int d = 0; d < D; d++) ai[d] = ai[d] + sp * dw[d];
-            if (rig_i && (fj & kRigid)) return;
+            if (!CMP && rig_i && (fj & kRigid)) return;
             const double nu_j = P.mat[mat_j].nu;
             const double tv = mj * (nu_i * rho_i + nu_j * rho_j) * prod / (rho_i * rho_j) / (rab2 + pc.eps2);
 #pragma unroll
@@ -2048,14 +2051,15 @@ __global__ __launch_bounds__(64) void kb_advance(const Params *Pt, const Arrays 
     SCENE_OF(sc[blockIdx.x]);
     advance_body(Pb, Ab);
 }
-template <int D, int G> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { XCD_BID(P); momentum_body<D, G>(P, A, bid_); }
+template <int D, int G, bool CMP> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { XCD_BID(P); momentum_body<D, G, CMP>(P, A, bid_); }
 template <int D, int G> __global__ __launch_bounds__(kBlock) void k_strain_classic(Params P, Arrays A, int mask) { XCD_BID(P); strain_classic_body<D, G>(P, A, mask, bid_); }
 template <int D, int G> __global__ __launch_bounds__(kBlock) void k_momentum_classic(Params P, Arrays A) { XCD_BID(P); momentum_classic_body<D, G>(P, A, bid_); }
 template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Params *Pt, const Arrays *At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
-    momentum_body<D>(Pb, Ab, e.y);
+    if (Pb.mcompact64) momentum_body<D, kMomentum64Batch, true>(Pb, Ab, e.y);
+    else momentum_body<D, kMomentum64Batch, false>(Pb, Ab, e.y);
 }
 __global__ __launch_bounds__(kRigidBlock) void k_rigid_reduce(Params P, Arrays A, int kin) { rigid_reduce_body(P, A, kin); }
 __global__ __launch_bounds__(kRigidBlock) void kb_rigid_reduce(const Params *Pt, const Arrays *At, const int *sc, int kin)
@@ -2242,9 +2246,15 @@ void launch_momentum(const Params &P, const Arrays &A, int rigid_kin, hipStream_
                                                          dim3(kBlock), 0, st, P, A)));
         return;
     }
-    DISPATCH_G(batch, kMomentum64Batch,
-               DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_momentum<DD, GG>), dim3(xcd_grid(nblk(P.N))), dim3(kBlock),
-                                                     0, st, P, A)));
+    if (P.mcompact64) {
+        DISPATCH_G(batch, kMomentum64Batch,
+                   DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_momentum<DD, GG, true>), dim3(xcd_grid(nblk(P.N))),
+                                                         dim3(kBlock), 0, st, P, A)));
+    } else {
+        DISPATCH_G(batch, kMomentum64Batch,
+                   DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_momentum<DD, GG, false>), dim3(xcd_grid(nblk(P.N))),
+                                                         dim3(kBlock), 0, st, P, A)));
+    }
     if (P.rigid && rigid_kin >= 0)
         hipLaunchKernelGGL(k_rigid_reduce, dim3(1), dim3(kRigidBlock), 0, st, P, A, rigid_kin);
 }

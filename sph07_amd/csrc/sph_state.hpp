@@ -113,6 +113,11 @@ struct Params {
     // are stored (sph_rec32.hpp)
     int mcompact;
     float mass32[kMaxMat + 1];
+    // FP64 momentum record: 0 {x, m}, {v, rho}, {p/rho^2, info} (80 bytes);
+    // 1 {x, p/rho^2}, {v, rho} (64 bytes) for one material with one particle
+    // mass (mass64), no rigid body and the Morris discretisation
+    int mcompact64;
+    double mass64;
     int btype[3];
     int bcode[2][3];
     double bx[2][3];
