@@ -114,3 +114,26 @@ def test_fp64_launch_shapes_same_bits():
         assert other.Solution(20) == 20
         _same(ref, other, f"fp64 strain {gs} momentum {gm}")
         other.close()
+
+
+def test_acceleration_bound_dt_matches_oracle():
+    """The critical step's |a| term from the |a|^2 kept per particle (written
+    with the velocity update and moved by the sort, while a itself stays
+    behind in a fused step) against the oracle, which sums a of every
+    particle each step: 40 fused steps (graph and direct launches) of a
+    channel whose dt the |a| term sets, bit-identical, dt included."""
+    from oracle import oracle as orc
+    from sph07_amd.native import Scene
+    deck = _acceleration_bound_channel()
+    print(f"acceleration-bound dt vs oracle: {deck.np} particles", flush=True)
+    o, g = orc.Oracle(deck), Scene(deck)
+    assert o.run(40) == g.Solution(40) == 40
+    h = deck.materials[0].h
+    go, gg = o.globals(), g.globals()
+    assert go["critts"] < 0.25 * h / 1.0e-3 and go["critts"] < 0.125 * h * h / deck.materials[0].av_l
+    for k in ("dt", "dtold", "ptime", "critts"):
+        assert gg[k] == go[k], (k, gg[k], go[k])
+    from sph07_amd import native as nv
+    for name, fg, fo in (("x", nv.F_X, orc.F_X), ("v", nv.F_V, orc.F_V), ("a", nv.F_A, orc.F_A),
+                         ("rho", nv.F_RHO, orc.F_RHO)):
+        assert np.array_equal(g.field(fg), o.field(fo)), name
