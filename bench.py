@@ -156,7 +156,10 @@ def parse():
     ap.add_argument("--ensemble", type=int, default=512, help="scenes of the ensemble leg (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=2000,
                     help="steps per ensemble scene (the workload's 2000)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.profile_steps < 1:
+        ap.error("--profile-steps must be at least 1 (the roofline needs per-kernel times)")
+    return args
 
 
 def serving_leg(args):

@@ -247,7 +247,8 @@ int  sph_batch_finished(void *batch, int *slots, int max);
 /* Device memory a scene of this deck will hold, in bytes (admission control). */
 long long sph_scene_footprint(const sph_scene_desc *desc);
 /* Launch shapes of a batch (timing only, never results): shape[0] = slots
-   per wave of the batched cell kernel, shape[1] = cell storage order (0
+   per wave of the batched cell kernel (0: the per-particle neighbour walk
+   for every scene), shape[1] = cell storage order (0
    Morton, 1 row-major), shape[2] = 1 once fixed.  A batch of 65536 live
    particles or more times each candidate on one 8-step graph of its own run
    and keeps the cheapest per particle-step.  reps == 0 fixes the current
@@ -293,7 +294,8 @@ int  sph_write_state_dump(void *ctx, const char *path);
 
 /* --- autotuner (no reference counterpart) ---
    Launch shapes that change timing but never results: shape[0] = slots per
-   wave of the cell-centric neighbour search, shape[1] / shape[2] = neighbour
+   wave of the cell-centric neighbour search (0: the per-particle walk, the
+   same lists in the same order), shape[1] / shape[2] = neighbour
    records in flight per lane in the continuity / momentum sweeps of the
    scene's arm (FP64 or FP32c), shape[3] = cell storage order (0 Morton,
    1 row-major).  shape holds 4 entries.
@@ -303,7 +305,8 @@ int  sph_write_state_dump(void *ctx, const char *path);
    turns the automatic tuning off; reps < 0 only reports.  shape may be NULL. */
 int  sph_autotune(void *ctx, int reps, int32_t *shape);
 /* Sets the four launch shapes of sph_autotune explicitly (slots per wave
-   4-4096, batches 2/4/8, order 0/1) and turns the automatic tuning off. */
+   4-4096 or 0 for the walk, batches 2/4/8, order 0/1) and turns the automatic
+   tuning off. */
 int  sph_set_launch_shape(void *ctx, const int32_t *shape);
 
 /* --- measurement (no reference counterpart: the reference only prints total

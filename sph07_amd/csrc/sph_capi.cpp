@@ -1090,7 +1090,7 @@ int autotune(Context *c, int reps)
         // each shape's default first; another replaces it only when faster
         // by more than kTuneMargin (timing noise must not pick a slower one)
         float tn = 1e30f;
-        for (int slots : {32, 16, 64, 128}) {
+        for (int slots : {32, 16, 64, 128, 0}) {     // 0: the per-particle walk
             const float ms = time_ms([&] {
                 hipMemsetAsync(&A.ctl->novf, 0, sizeof(int), st);
                 launch_neighbours(P, A, st, slots);
@@ -1197,7 +1197,7 @@ int sph_set_launch_shape(void *ctx, const int32_t *shape)
     Context *c = ctx_of(ctx);
     if (!c || !shape) return SPH_ERROR;
     const int slots = shape[0], gs = shape[1], gm = shape[2], order = shape[3];
-    if (slots < 4 || slots > 4096 || (gs != 2 && gs != 4 && gs != 8) || (gm != 2 && gm != 4 && gm != 8) ||
+    if ((slots != 0 && (slots < 4 || slots > 4096)) || (gs != 2 && gs != 4 && gs != 8) || (gm != 2 && gm != 4 && gm != 8) ||
         (order != 0 && order != 1)) {
         c->err = "sph_set_launch_shape: shape out of range";
         return SPH_ERROR;
@@ -1661,7 +1661,8 @@ struct Batch {
 };
 
 // candidates of the batch tuner: (cell storage order, slots per wave)
-constexpr int kBatchCand[][2] = {{0, 64}, {1, 64}, {0, 32}, {1, 32}, {0, 128}, {1, 128}};
+// (cell order, slots per wave of the cell kernel; 0: the per-particle walk)
+constexpr int kBatchCand[][2] = {{0, 64}, {1, 64}, {0, 32}, {1, 32}, {0, 128}, {1, 128}, {0, 0}, {1, 0}};
 constexpr int kBatchNCand = (int)(sizeof(kBatchCand) / sizeof(kBatchCand[0]));
 // batches of at least this many live particles tune their shapes online
 constexpr long long kBatchTuneMinN = 65536;
@@ -1716,7 +1717,8 @@ int batch_rebuild(Batch *b)
         c->P.corder = b->corder;           // the next step sorts in this order
         scenes.push_back(s);
         for (int lb = 0; lb < (c->N + kBlock - 1) / kBlock; lb++) pmap.push_back(make_int2(s, lb));
-        if (cell_lists_fit(c->P)) {
+        // cell_slots 0: the per-particle walk for every scene (a tuning candidate)
+        if (cell_lists_fit(c->P) && b->cell_slots > 0) {
             const int nb = batch_cell_blocks(c->N, b->cell_slots);
             for (int lb = 0; lb < nb; lb++) cmap.push_back(make_int4(s, lb, nb, 0));
         } else {

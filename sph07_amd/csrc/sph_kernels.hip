@@ -2176,8 +2176,9 @@ int batch_cell_blocks(int n, int slots)
 
 void launch_neighbours(const Params &P, const Arrays &A, hipStream_t st, int cell_slots)
 {
-    if (!cell_lists_fit(P)) {
-        // the cell kernel's list byte offsets are 32-bit
+    if (!cell_lists_fit(P) || cell_slots == 0) {
+        // the cell kernel's list byte offsets are 32-bit; cell_slots 0 asks
+        // for the per-particle walk (the same lists in the same order)
         DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A));
         return;
     }

@@ -221,6 +221,7 @@ class Scene:
     def set_launch_shape(self, cell_slots: int = 32, strain_batch: int = None, momentum_batch: int = None,
                          cell_order: str = "morton"):
         """sph_set_launch_shape: fixes the launch shapes (no automatic tuning).
+        cell_slots 0 selects the per-particle neighbour walk (the same lists).
         The batches are the neighbour records in flight per lane of the
         scene's arm's continuity and momentum sweeps; None keeps the arm's
         untuned default (FP64 4 / 2, FP32c 8 / 8)."""

@@ -390,3 +390,23 @@ def test_fp32c_launch_shapes_same_bits(strain_g, momentum_g):
     assert ref.Solution(20) == other.Solution(20) == 20
     for f in (F_X, F_V, F_RHO):
         assert np.array_equal(ref.field(f), other.field(f)), f
+
+
+@pytest.mark.parametrize("name", ["dam_break", "poiseuille"])
+def test_walk_shape_same_bits(name):
+    """Slots per wave 0 (the per-particle neighbour walk, an autotuner
+    candidate) builds the same lists in the same order as the cell kernel:
+    20 fused steps give the same bits."""
+    from sph07_amd.native import Scene
+    deck = (scenes.dam_break(nx_fluid=10, ny=8, nz_fluid=10, nx_tank=20, nz_tank=14, maxnbr=100)
+            if name == "dam_break" else scenes.poiseuille(nx=30, ny=30))
+    print(f"walk shape: {name}, {deck.np} particles", flush=True)
+    cell, walk = Scene(deck), Scene(deck)
+    cell.set_launch_shape(cell_slots=32)
+    walk.set_launch_shape(cell_slots=0)
+    assert cell.Solution(20) == walk.Solution(20) == 20
+    nn_c, ids_c = cell.neighbours()
+    nn_w, ids_w = walk.neighbours()
+    assert np.array_equal(nn_c, nn_w) and np.array_equal(ids_c, ids_w)
+    for f in (0, 1, 2, 3):
+        assert np.array_equal(cell.field(f), walk.field(f)), f
