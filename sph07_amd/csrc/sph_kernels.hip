@@ -556,7 +556,10 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
     int shift[3];
     for (int d = 0; d < 3; d++) shift[d] = addv[d] != 0.0 ? (addv[d] >= 0 ? 1 : -1) : 0;
     const int shift_code = (shift[0] + 1) * 9 + (shift[1] + 1) * 3 + (shift[2] + 1);
-    for (int s = bid * kBlock + threadIdx.x; s < g.n; s += nb * kBlock) {
+    // one wave per source cell, one lane per image (the cell's images are
+    // consecutive sources and consecutive image slots)
+    const int lane = threadIdx.x & 63;
+    for (int s = bid * (kBlock / 64) + (int)(threadIdx.x >> 6); s < g.n; s += nb * (kBlock / 64)) {
         const int cnt = A.slab_cnt[s];
         if (cnt == 0) continue;
         int i, j, k;
@@ -565,14 +568,12 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
         const int tc = ctl->glim[2] * (ctl->glim[1] * (i + g.off[0]) + (j + g.off[1])) + (k + g.off[2]);
         if (A.gcount[tc] != 0) {
             // a cell fed by two passes would need a merge of image runs
-            set_err(ctl, 1, -2);
+            if (lane == 0) set_err(ctl, 1, -2);
             continue;
         }
         const int g0 = A.slab_off[s];
-        A.gfirst[tc] = g0;
-        A.gcount[tc] = cnt;
         const int s0 = A.cell_start[cell_code(ctl, i, j, k)];
-        for (int r = 0; r < cnt; r++) {
+        for (int r = lane; r < cnt; r += 64) {
             const int src = s0 + r;              // descending deck number
             const int gi = P.N + g0 + r;
             for (int d = 0; d < P.ndim; d++) {
@@ -614,6 +615,11 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
             }
             A.gsrc[g0 + r] = src;
             A.gshift[g0 + r] = shift_code;
+        }
+        // after every lane has read the old count
+        if (lane == 0) {
+            A.gfirst[tc] = g0;
+            A.gcount[tc] = cnt;
         }
     }
 }
