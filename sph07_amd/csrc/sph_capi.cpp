@@ -1623,6 +1623,11 @@ int sph_get_timing(void *ctx, int slot, char *name, int namelen, double *total_m
 // array blocks and the block maps of the batched kernels (launch_batch_step).
 namespace {
 
+// candidates of the batch tuner: (cell storage order, slots per wave of the
+// cell kernel; 0: the per-particle walk)
+constexpr int kBatchCand[][2] = {{0, 64}, {1, 64}, {0, 32}, {1, 32}, {0, 128}, {1, 128}, {0, 0}, {1, 0}};
+constexpr int kBatchNCand = (int)(sizeof(kBatchCand) / sizeof(kBatchCand[0]));
+
 struct Batch {
     hipStream_t st = nullptr;
     int ndim = 2, arm = SPH_ARM_FP64;
@@ -1649,7 +1654,7 @@ struct Batch {
     int corder = 0, cell_slots = kBatchCellSlots;
     bool tuned = false;                  // shapes fixed (tuned, or tuning turned off)
     int trial = 0;                       // next candidate to time
-    double trial_cost[8] = {0};          // ms per million particle-steps of each candidate
+    double trial_cost[kBatchNCand] = {0};   // ms per million particle-steps of each candidate
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool trial_pending = false;          // a timed trial is in flight (read at the next sync)
     long long trial_steps = 0;           // particle-steps of that trial
@@ -1660,10 +1665,6 @@ struct Batch {
     std::vector<char> zombie;            // per slot: removed, freed at the next sync
 };
 
-// candidates of the batch tuner: (cell storage order, slots per wave)
-// (cell order, slots per wave of the cell kernel; 0: the per-particle walk)
-constexpr int kBatchCand[][2] = {{0, 64}, {1, 64}, {0, 32}, {1, 32}, {0, 128}, {1, 128}, {0, 0}, {1, 0}};
-constexpr int kBatchNCand = (int)(sizeof(kBatchCand) / sizeof(kBatchCand[0]));
 // batches of at least this many live particles tune their shapes online
 constexpr long long kBatchTuneMinN = 65536;
 
