@@ -1003,12 +1003,16 @@ int sph_end_step(void *ctx)
 constexpr int kGraphSteps = 16;
 static_assert(kGraphSteps % 2 == 0, "a graph ends on the buffer set it starts from");
 
-int build_graph(Context *c)
+int build_graph(Context *c, int from)
 {
     hipGraph_t g = nullptr;
-    const int from = c->cur;
+    // capture records launches without running them: the steps start from
+    // buffer set `from` whatever the current one is (an even count ends there)
+    const int cur = c->cur;
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    c->cur = from;
     for (int k = 0; k < kGraphSteps; k++) fused_step(c, k == 0, k == kGraphSteps - 1);
+    c->cur = cur;
     HIPCHK(c, hipStreamEndCapture(c->st, &g));
     hipError_t e = hipGraphInstantiate(&c->gexec[from], g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -1158,6 +1162,11 @@ int sph_solution(void *ctx, int max_steps, int *steps_done)
         const int rc = autotune(c, 5);
         if (rc) return rc;
     }
+    // both buffer sets' graphs are captured on the first call (capture runs
+    // nothing), so that later calls of any length launch ready graphs
+    if (c->use_graph && !c->tm.on && max_steps > 0)
+        for (const int from : {c->cur, c->cur ^ 1})
+            if (!c->gexec[from] && build_graph(c, from)) return SPH_ERR_HIP;
     if (sync_ctl(c)) return SPH_ERR_HIP;
     const int t0 = c->hctl->timestep;
     int remaining = max_steps;
@@ -1166,7 +1175,7 @@ int sph_solution(void *ctx, int max_steps, int *steps_done)
         const int chunk = std::min(remaining, 64);
         int k = 0;
         if (c->use_graph && !c->tm.on && chunk >= kGraphSteps) {
-            if (!c->gexec[c->cur] && build_graph(c)) return SPH_ERR_HIP;
+            if (!c->gexec[c->cur] && build_graph(c, c->cur)) return SPH_ERR_HIP;
             if (c->gexec[c->cur])
                 for (; k + c->gsteps <= chunk; k += c->gsteps)
                     HIPCHK(c, hipGraphLaunch(c->gexec[c->cur], c->st));
