@@ -587,6 +587,14 @@ int build_context(Context *c, const sph_scene_desc *d)
             P.cut2max = std::max(P.cut2max, P.pc[a][b].cut2);
             P.cut2min = std::min(P.cut2min, P.pc[a][b].cut2);
         }
+    {
+        // quantum of the FP32c record positions: cut = f 2^e, f in [0.5, 1),
+        // q = 2^(e - 23), so cut / q = f 2^23
+        int e = 0;
+        frexp(sqrt(P.cut2max), &e);
+        P.qexp = e - 23;
+        P.qinv = ldexp(1.0, -P.qexp);
+    }
     for (int m = 0; m < D; m++)
         if (P.btype[m] == 4) {
             const double gs = 1.0 / (2.0 * P.hmax);

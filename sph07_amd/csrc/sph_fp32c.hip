@@ -4,10 +4,8 @@
 // per-particle update stay FP64 (the same k_integrate / k_advance / rigid
 // kernels as the FP64 arm).  Only the bulk pair sums run in FP32: each
 // neighbour is one compact FP32 record (sph_rec32.hpp) holding its position
-// relative to a cell corner, so the pair separation is rebuilt exactly from
-// small numbers (off * cs32 + (r_i - r_j) in the continuity sweep, R_i - R_j
-// less the nearest multiple of 2 cs32 in the momentum sweep) without
-// cancellation of large coordinates; the
+// as a fixed-point integer, so a pair separation is one integer subtraction
+// and one conversion, without cancellation of large coordinates; the
 // kernel gradient, pressure, viscous and Monaghan terms are FP32 with hardware
 // reciprocal and reciprocal square root; each particle's sums are accumulated
 // with the TwoSum error-free transformation (Knuth 1969) into (sum,
@@ -44,21 +42,26 @@ __device__ __forceinline__ void two_sum_acc(float &s, float &c, float x)
     c += err;
 }
 
+// Pair constants for separations in units of q (qinv = 1 / q, a power of
+// two): ih = q / havg, so z = r' ih with r' = r / q; eps2 = 0.01 havg^2 / q^2;
+// havg / q.  c1ih = const1 / havg is unchanged: grad W = f dx = f' dx' with
+// f' = dW/dz c1ih / r' (gradw_factor of r'^2).
 struct PairConst32 {
     float ih, c1ih, eps2, havg;
 };
 
-__device__ __forceinline__ PairConst32 pc32(const PairConst &pc)
+__device__ __forceinline__ PairConst32 pc32(const PairConst &pc, double qinv)
 {
     PairConst32 q;
-    q.havg = (float)pc.havg;
-    q.ih = (float)(1.0 / pc.havg);
+    q.havg = (float)(pc.havg * qinv);
+    q.ih = (float)(1.0 / (pc.havg * qinv));
     q.c1ih = (float)(pc.const1 / pc.havg);
-    q.eps2 = (float)pc.eps2;
+    q.eps2 = (float)(pc.eps2 * qinv * qinv);
     return q;
 }
 
-// dW/dr / r in FP32 from r^2 (so that grad W = f * dx).  The B-spline
+// dW/dr / r in FP32 from r^2 (so that grad W = f * dx; with r^2 and dx in
+// units of q, pc32's constants give f' with f' dx' = grad W).  The B-spline
 // derivative (MKernelBSpline.cpp:166-172) in branch-free form:
 //   dW/dz = c1 (3 (1-z)+^2 - 0.75 (2-z)+^2),
 // which is c1 (2.25 z^2 - 3 z) for z < 1, -3 c2 (2-z)^2 for 1 <= z < 2
@@ -140,28 +143,25 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
     const int mat_i = info_mat(fi);
     const Mat &mt = P.mat[mat_i];
     const double dt = ctl->dt;
-    float cs[D], ri[D], vi32[D];
+    float ri[D], vi32[D];
     float gs = 0.0f, gc = 0.0f;
     const float4 fa = rec32_at<0>(A.fs, (uint32_t)ic), fb = rec32_at<1>(A.fs, (uint32_t)ic);
     const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
-    const uint32_t pk_i = __float_as_uint(fb.w);
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        cs[d] = cell_size32(ctl, d);
         ri[d] = ri3[d];
         vi32[d] = vi3[d];
     }
-    const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
+    const PairConst32 q0 = pc32(P.pc[mat_i][mat_i], P.qinv);
     auto pair = [&](const float4 ra, const float4 rb) {
         const uint32_t pk_j = __float_as_uint(rb.w);
-        const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)]);
+        const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)], P.qinv);
         const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
-        float off[3];
-        cell_offs(pk_i, pk_j, off);
+        // separation and r^2 in units of q
         float dx[D], r2 = 0.0f;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            dx[d] = off[d] * cs[d] + (ri[d] - rj[d]);
+            dx[d] = qsep(ri[d], rj[d]);
             r2 += dx[d] * dx[d];
         }
         // the pair's share of tr D, sum over d of (v_j - v_i)_d * vol_j * dW/dx_d,
@@ -199,9 +199,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
             const double pr = p / (rho_i * rho_i);
             A.p[i] = p;
             A.prho2[i] = pr;
-            const int q = A.occ[i];
-            const int bc[3] = {code_ext(q, ctl->cmask[0]), code_ext(q, ctl->cmask[1]), code_ext(q, ctl->cmask[2])};
-            put_mrec32<D>(P, A, ctl, i, xr, A.m[i], vi, rho_i, pr, fi, bc);
+            put_mrec32<D>(P, A, i, xr, A.m[i], vi, rho_i, pr, fi);
         }
         A.rho_new[i] = rho_i;
         const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
@@ -240,21 +238,21 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const bool rig_i = fi & kRigid;
         const double rho_i64 = A.rho_new[i];
         const float rho_i = (float)rho_i64, pr_i = (float)A.prho2[i];
-        const float q_i = (float)(mi.nu * rho_i64);
+        // viscosities carry 1 / q: the Morris term's r . grad W / (r^2 + 0.01 h^2)
+        // is 1 / q times its value in units of q
+        const float q_i = (float)(mi.nu * P.qinv * rho_i64);
         const bool monaghan = mi.visc_type == 2;
-        float tcs[D], itcs[D], ri[D], vi[D], as[D], ac[D];
+        float ri[D], vi[D], as[D], ac[D];
         const float4 fa = rec32_at<0>(A.fm, (uint32_t)i), fb = rec32_at<1>(A.fm, (uint32_t)i);
         const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            tcs[d] = uniform_f(2.0f * cell_size32(ctl, d));
-            itcs[d] = uniform_f(1.0f / tcs[d]);
             ri[d] = ri3[d];
             vi[d] = vi3[d];
             as[d] = 0.0f;
             ac[d] = 0.0f;
         }
-        const PairConst32 q0 = pc32(P.pc[mat_i][mat_i]);
+        const PairConst32 q0 = pc32(P.pc[mat_i][mat_i], P.qinv);
         struct Rec { float4 a, b; float2 c; };
         auto load = [&](int j) {
             Rec r;
@@ -266,14 +264,14 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const float rho0_i = (float)mi.rho0, c2_i = (float)(mi.c * mi.c), m_i = P.mass32[mat_i];
         const float irho_i = (float)(1.0 / rho_i64);
         // one material: the partner's constants are the particle's own
-        const float nu_i = (float)mi.nu;
+        const float nu_i = (float)(mi.nu * P.qinv);
         const float mnu_i = m_i * nu_i;
         const float acbar_i = (float)(mi.alpha * mi.c);     // alpha_bar * c_bar for equal materials
         for_each_nbr_batched<G, true>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
             const uint32_t fj = KIND == 1 ? __float_as_uint(rb.w) : (KIND == 0 ? __float_as_uint(rec.c.y) : 0u);
             const int mat_j = ONEMAT ? mat_i : rec_mat(fj);
-            const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][mat_j]);
+            const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][mat_j], P.qinv);
             float mj, rho_j = 0.0f, prho2_j, ir_j;
             if (KIND == 2) {
                 mj = m_i;
@@ -296,12 +294,12 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             float dx[D], dv[D], r2 = 0.0f, vr = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                dx[d] = sep32(ri[d], rj[d], tcs[d], itcs[d]);
+                dx[d] = qsep(ri[d], rj[d]);
                 dv[d] = vi[d] - vj[d];
                 r2 += dx[d] * dx[d];
                 vr += dv[d] * dx[d];
             }
-            const float f = gradw_factor(r2, q);            // grad W = f * dx
+            const float f = gradw_factor(r2, q);            // grad W = f * dx (dx, r2 in units of q)
             float sdw = -mj * (prho2_j + pr_i);             // coefficient of grad W
             // viscous terms, skipped between two rigid particles; evaluated
             // for every pair and selected (no divergent branch per pair)
@@ -313,7 +311,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             if (KIND == 2) {
                 tv = mnu_i * (irho_i + ir_j) * (f * r2) * ir2e;
             } else {
-                const float qj = (ONEMAT ? nu_i : (float)P.mat[mat_j].nu) * rho_j;
+                const float qj = (ONEMAT ? nu_i : (float)(P.mat[mat_j].nu * P.qinv)) * rho_j;
                 tv = visc ? mj * (q_i + qj) * (f * r2) * (irho_i * ir_j * ir2e) : 0.0f;
             }
             if (monaghan) {

@@ -366,9 +366,7 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
         }
         const double vol = S.m[i] / S.rho[i];
         if (P.arm == 1) {
-            const int c = S.cell_of[i];
-            const int bc[3] = {code_ext(c, ctl->cmask[0]), code_ext(c, ctl->cmask[1]), code_ext(c, ctl->cmask[2])};
-            put_srec32<D>(T, ctl, j, xs, vol, vi, fi, bc);
+            put_srec32<D>(P, T, j, xs, vol, vi, fi);
         } else {
             put_srec<D>(T, j, xs, vol, vi, fi);
         }
@@ -606,11 +604,10 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
                 put_crec<3>(A, gi, T0, ginfo);
             }
             if (P.arm == 1) {
-                const int tb[3] = {i + g.off[0], j + g.off[1], k + g.off[2]};   // the image's cell
                 switch (P.ndim) {
-                case 1: put_srec32<1>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<1>(P, A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb); break;
-                case 2: put_srec32<2>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<2>(P, A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb); break;
-                default: put_srec32<3>(A, ctl, gi, gx, mass / rho, gv, ginfo, tb); put_mrec32<3>(P, A, ctl, gi, gx, mass, gv, rho, 0.0, ginfo, tb);
+                case 1: put_srec32<1>(P, A, gi, gx, mass / rho, gv, ginfo); put_mrec32<1>(P, A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
+                case 2: put_srec32<2>(P, A, gi, gx, mass / rho, gv, ginfo); put_mrec32<2>(P, A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
+                default: put_srec32<3>(P, A, gi, gx, mass / rho, gv, ginfo); put_mrec32<3>(P, A, gi, gx, mass, gv, rho, 0.0, ginfo);
                 }
             }
             A.gsrc[g0 + r] = src;

@@ -1,23 +1,18 @@
 // sph_rec32.hpp -- compact FP32 gather records of the FP32-compensated arm.
 //
-// Positions are stored relative to a cell corner, so that a pair separation
-// is rebuilt without cancellation of large coordinates (FP32 keeps them to
-// ~1e-7 of the cut-off).  origin(c) = gmin + c * cs32 with cs32 the cell size
-// rounded to FP32; neighbours lie less than one cell size apart on every axis.
-//   * continuity records: r = x - origin(c) of the particle's bin cell c; the
-//     packed word holds the low two bits of each cell coordinate, and the
-//     separation is off * cs32 + (r_i - r_j) with off in {-1, 0, 1} from them
-//     (off * cs32 is exact);
-//   * momentum records: R = x - origin(c & ~1), the corner of the even cell at
-//     or below c (0 <= R < 2 cs32); even-cell origins of neighbours differ by
-//     0 or +-2 cs32 exactly, so the separation is R_i - R_j - 2 cs32 rint((R_i -
-//     R_j) / (2 cs32)) (the subtraction of +-2 cs32 is exact, Sterbenz).  This
-//     form needs no cell bits, which frees the packed word of the one-material
-//     record for p/rho^2.
+// Positions are fixed-point integers: X = rint(x / q) modulo 2^32 in units of
+// q = 2^Params::qexp, the power of two with cut / q in [2^22, 2^23) for the
+// largest cut-off.  A pair separation is the 32-bit difference X_i - X_j
+// (wrap-around included: any two points closer than 2^31 q, ~256 cut-offs,
+// subtract correctly, whatever the size of the domain), exact in integers
+// and exact again when converted to FP32, since neighbours lie within 2^24 q.
+// The only rounding is the quantisation of each position, at most q / 2 =
+// 2^-24 of the cut-off: the precision FP32 has for coordinates relative to a
+// cell corner, without their cell bookkeeping.  The sweeps work in units of
+// q (separations, r^2); the pair constants are scaled to match (PairConst32),
+// the scalings being exact powers of two.
 //
-// Packed word: bits 0-3 material, bit 6 periodic image, bit 7 rigid, bytes
-// 1-3 (continuity records) the low two bits of the cell coordinates, one byte
-// per axis so that the three offsets come out of one byte-lane subtraction.
+// Packed word: bits 0-3 material, bit 6 periodic image, bit 7 rigid.
 #pragma once
 
 #include "sph_state.hpp"
@@ -27,17 +22,17 @@ namespace sph {
 constexpr uint32_t kRecMat = 0x0Fu, kRecGhost = 0x40u, kRecRigid = 0x80u;
 static_assert(kMaxMat < 16, "material fits the packed word");
 
-__device__ __forceinline__ float cell_size32(const Ctl *ctl, int d) { return (float)(1.0 / ctl->gsize[d]); }
-
-__device__ __forceinline__ double cell_origin(const Ctl *ctl, int d, int c)
+// X of one coordinate (qinv = 2^-qexp: the product is exact), its bits as a float
+__device__ __forceinline__ float fixq(double x, double qinv)
 {
-    return ctl->gmin[d] + (double)c * (double)cell_size32(ctl, d);
+    const long long k = (long long)rint(x * qinv);
+    return __uint_as_float((uint32_t)(unsigned long long)k);
 }
 
-// R of one coordinate: relative to the origin of the even cell at or below c
-__device__ __forceinline__ float rel32(const Ctl *ctl, int d, int c, double x)
+// separation component in units of q from two fixed-point coordinates
+__device__ __forceinline__ float qsep(float xi, float xj)
 {
-    return (float)(x - cell_origin(ctl, d, c & ~1));
+    return (float)(int)(__float_as_uint(xi) - __float_as_uint(xj));
 }
 
 __device__ __forceinline__ uint32_t pack32(uint32_t info)
@@ -45,32 +40,7 @@ __device__ __forceinline__ uint32_t pack32(uint32_t info)
     return (info & kRecMat) | ((info & kRigid) ? kRecRigid : 0u) | ((info & kGhost) ? kRecGhost : 0u);
 }
 
-// with the low two bits of the cell coordinates (continuity records)
-__device__ __forceinline__ uint32_t pack32c(uint32_t info, const int *bc)
-{
-    return pack32(info) | ((uint32_t)(bc[0] & 3) << 8) | ((uint32_t)(bc[1] & 3) << 16) |
-           ((uint32_t)(bc[2] & 3) << 24);
-}
-
-// Cell offsets (-1, 0, 1) of particle a relative to particle b on the three
-// axes: per byte (4 + ca - cb + 1) mod 4 = offset + 1, no borrow across bytes.
-__device__ __forceinline__ void cell_offs(uint32_t pa, uint32_t pb, float off[3])
-{
-    const uint32_t x = (((pa | 0x04040400u) - (pb & 0x03030300u)) + 0x01010100u) & 0x03030300u;
-    off[0] = (float)((x >> 8) & 0xFFu) - 1.0f;
-    off[1] = (float)((x >> 16) & 0xFFu) - 1.0f;
-    off[2] = (float)(x >> 24) - 1.0f;
-}
-
 __device__ __forceinline__ int rec_mat(uint32_t pk) { return (int)(pk & kRecMat); }
-
-// Separation component from two R values (see the header): tcs = 2 cs32,
-// itcs = 1 / tcs
-__device__ __forceinline__ float sep32(float ri, float rj, float tcs, float itcs)
-{
-    const float raw = ri - rj;
-    return raw - tcs * __builtin_rintf(raw * itcs);
-}
 
 // inverse of the cell-code deposit: the cell coordinate held in the bits m of v
 __device__ __forceinline__ int code_ext(int v, unsigned int m)
@@ -88,59 +58,67 @@ __device__ __forceinline__ int code_ext(int v, unsigned int m)
     return r;
 }
 
-// continuity record: {r[0..2], vol}, {v[0..2], packed with the cell bits}
+// The two 16-byte halves of a record side by side, one 32-byte sector per
+// particle (entries 2j and 2j + 1)
+template <int PART>
+__device__ __forceinline__ size_t rec32_index(int j)
+{
+    return 2 * (size_t)j + PART;
+}
+
+// continuity record: {X[0..2], vol}, {v[0..2], packed}
 template <int D>
-__device__ __forceinline__ void put_srec32(const Arrays &A, const Ctl *ctl, int j, const double *xs, double vol,
-                                           const double *v, uint32_t info, const int *bc)
+__device__ __forceinline__ void put_srec32(const Params &P, const Arrays &A, int j, const double *xs, double vol,
+                                           const double *v, uint32_t info)
 {
     float4 a = make_float4(0.f, 0.f, 0.f, (float)vol), b = make_float4(0.f, 0.f, 0.f, 0.f);
-    a.x = (float)(xs[0] - cell_origin(ctl, 0, bc[0]));
+    a.x = fixq(xs[0], P.qinv);
     b.x = (float)v[0];
-    if (D > 1) { a.y = (float)(xs[1] - cell_origin(ctl, 1, bc[1])); b.y = (float)v[1]; }
-    if (D > 2) { a.z = (float)(xs[2] - cell_origin(ctl, 2, bc[2])); b.z = (float)v[2]; }
-    b.w = __uint_as_float(pack32c(info, bc));
-    A.fs[2 * (size_t)j] = a;
-    A.fs[2 * (size_t)j + 1] = b;
+    if (D > 1) { a.y = fixq(xs[1], P.qinv); b.y = (float)v[1]; }
+    if (D > 2) { a.z = fixq(xs[2], P.qinv); b.z = (float)v[2]; }
+    b.w = __uint_as_float(pack32(info));
+    A.fs[rec32_index<0>(j)] = a;
+    A.fs[rec32_index<1>(j)] = b;
 }
 
 // momentum record, by Params::mcompact:
-//   0: {R[0..2], m}, {v[0..2], rho}, fmC = {p/rho^2, packed} (40 bytes)
-//   1: {R[0..2], rho - rho0}, {v[0..2], packed} (32 bytes; one mass per
+//   0: {X[0..2], m}, {v[0..2], rho}, fmC = {p/rho^2, packed} (40 bytes)
+//   1: {X[0..2], rho - rho0}, {v[0..2], packed} (32 bytes; one mass per
 //      material and the linear EOS: m and p/rho^2 follow from the material)
-//   2: {R[0..2], 1/rho}, {v[0..2], p/rho^2} (32 bytes; one material, no rigid
+//   2: {X[0..2], 1/rho}, {v[0..2], p/rho^2} (32 bytes; one material, no rigid
 //      body: nothing else differs between particles; images carry p = 0)
 template <int D>
-__device__ __forceinline__ void put_mrec32(const Params &P, const Arrays &A, const Ctl *ctl, int j, const double *x,
-                                           double m, const double *v, double rho, double prho2, uint32_t info,
-                                           const int *bc)
+__device__ __forceinline__ void put_mrec32(const Params &P, const Arrays &A, int j, const double *x, double m,
+                                           const double *v, double rho, double prho2, uint32_t info)
 {
     float4 a = make_float4(0.f, 0.f, 0.f, (float)m), b = make_float4(0.f, 0.f, 0.f, (float)rho);
-    a.x = rel32(ctl, 0, bc[0], x[0]);
+    a.x = fixq(x[0], P.qinv);
     b.x = (float)v[0];
-    if (D > 1) { a.y = rel32(ctl, 1, bc[1], x[1]); b.y = (float)v[1]; }
-    if (D > 2) { a.z = rel32(ctl, 2, bc[2], x[2]); b.z = (float)v[2]; }
+    if (D > 1) { a.y = fixq(x[1], P.qinv); b.y = (float)v[1]; }
+    if (D > 2) { a.z = fixq(x[2], P.qinv); b.z = (float)v[2]; }
     if (P.mcompact == 2) {
         a.w = (float)(1.0 / rho);
         b.w = (float)prho2;
-        A.fm[2 * (size_t)j] = a;
-        A.fm[2 * (size_t)j + 1] = b;
+        A.fm[rec32_index<0>(j)] = a;
+        A.fm[rec32_index<1>(j)] = b;
         return;
     }
     if (P.mcompact == 1) {
         a.w = (float)(rho - P.mat[info & kMatMask].rho0);
         b.w = __uint_as_float(pack32(info));
-        A.fm[2 * (size_t)j] = a;
-        A.fm[2 * (size_t)j + 1] = b;
+        A.fm[rec32_index<0>(j)] = a;
+        A.fm[rec32_index<1>(j)] = b;
         return;
     }
-    A.fm[2 * (size_t)j] = a;
-    A.fm[2 * (size_t)j + 1] = b;
+    A.fm[rec32_index<0>(j)] = a;
+    A.fm[rec32_index<1>(j)] = b;
     A.fmC[j] = make_float2((float)prho2, __uint_as_float(pack32(info)));
 }
 
-// Half PART (0 or 1) of particle j's 32-byte record: a 32-bit byte offset from
-// the wave-uniform base, so the load takes the base in scalar registers and
-// one vector register of offset (the arm limits scenes to 2^27 slots).
+// Half PART (0 or 1) of particle j's record: a 32-bit byte offset from the
+// wave-uniform base of the half, so the load takes the base in scalar
+// registers and one vector register of offset (the arm limits scenes to 2^27
+// slots).
 template <int PART>
 __device__ __forceinline__ float4 rec32_at(const float4 *base, uint32_t j)
 {

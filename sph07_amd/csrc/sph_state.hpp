@@ -113,6 +113,10 @@ struct Params {
     // are stored (sph_rec32.hpp)
     int mcompact;
     float mass32[kMaxMat + 1];
+    // FP32c record positions are fixed point in units of q = 2^qexp, the
+    // power of two with (largest cut-off) / q in [2^22, 2^23); qinv = 1 / q
+    int qexp;
+    double qinv;
     // FP64 momentum record: 0 {x, m}, {v, rho}, {p/rho^2, info} (80 bytes);
     // 1 {x, p/rho^2}, {v, rho} (64 bytes) for one material with one particle
     // mass (mass64), no rigid body and the Morris discretisation
@@ -144,10 +148,10 @@ struct Arrays {
     //   momentum:   rmA = {x[0..2], m}, rmB = {v[0..2], rho}, rmC = {p/rho^2, info}
     double4 *rsA, *rsB, *rmA, *rmB;
     double2 *rmC;
-    // FP32 records of the FP32-compensated arm (cell-relative positions),
+    // FP32 records of the FP32-compensated arm (fixed-point positions X),
     // two float4 per particle side by side (32 bytes, one sector):
-    //   continuity: fs[2j] = {r[0..2], vol}, fs[2j+1] = {v[0..2], packed}
-    //   momentum:   fm[2j] = {r[0..2], m}, fm[2j+1] = {v[0..2], rho}, fmC[j] = {p/rho^2, packed}
+    //   continuity: fs[2j] = {X[0..2], vol}, fs[2j+1] = {v[0..2], packed}
+    //   momentum:   fm[2j] = {X[0..2], m}, fm[2j+1] = {v[0..2], rho}, fmC[j] = {p/rho^2, packed}
     float4 *fs, *fm;
     float2 *fmC;
     // Classic discretisation (disctype 0): q, the diagonal artificial viscosity
