@@ -9,7 +9,8 @@
 // kernel gradient, pressure, viscous and Monaghan terms are FP32 with hardware
 // reciprocal and reciprocal square root; each particle's sums are accumulated
 // with the TwoSum error-free transformation (Knuth 1969) into (sum,
-// compensation) FP32 pairs and converted to FP64 once at the end.
+// compensation) FP32 pairs, two neighbours' terms per addition, and
+// converted to FP64 once at the end.
 //
 // Neighbour sets and their order are those of the FP64 arm (shared lists).
 #include "sph_kernels.hpp"
@@ -169,13 +170,16 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
         float dvx = 0.0f;
 #pragma unroll
         for (int d = 0; d < D; d++) dvx += (vj[d] - vi32[d]) * dx[d];
-        two_sum_acc(gs, gc, (gradw_factor(r2, q) * ra.w) * dvx);
+        return (gradw_factor(r2, q) * ra.w) * dvx;
     };
     if (act) {
         struct Rec { float4 a, b; };
         auto load = [&](int j) { return Rec{rec32_at<0>(A.fs, (uint32_t)j), rec32_at<1>(A.fs, (uint32_t)j)}; };
-        for_each_nbr_batched<G, true>(A.nbr, ic, A.nnbr[ic], P.nstride, load,
-                                      [&](const Rec &rec) { pair(rec.a, rec.b); });
+        // two pairs' terms per compensated addition (a padding entry's term
+        // with the particle itself is exactly +0)
+        for_each_nbr_pairs<G, true>(A.nbr, ic, A.nnbr[ic], P.nstride, load, [&](const Rec &a, const Rec &b) {
+            two_sum_acc(gs, gc, pair(a.a, a.b) + pair(b.a, b.b));
+        });
     }
     if (inr) {
         double rho_i = A.rho[i];
@@ -267,7 +271,8 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const float nu_i = (float)(mi.nu * P.qinv);
         const float mnu_i = m_i * nu_i;
         const float acbar_i = (float)(mi.alpha * mi.c);     // alpha_bar * c_bar for equal materials
-        for_each_nbr_batched<G, true>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
+        struct Term { float t[D]; };
+        auto term = [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
             const uint32_t fj = KIND == 1 ? __float_as_uint(rb.w) : (KIND == 0 ? __float_as_uint(rec.c.y) : 0u);
             const int mat_j = ONEMAT ? mat_i : rec_mat(fj);
@@ -329,8 +334,17 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
                 sdw += (visc && vr < 0.0f) ? pterm : 0.0f;
             }
             const float sf = sdw * f;
+            Term tm;
 #pragma unroll
-            for (int d = 0; d < D; d++) two_sum_acc(as[d], ac[d], sf * dx[d] + tv * dv[d]);
+            for (int d = 0; d < D; d++) tm.t[d] = sf * dx[d] + tv * dv[d];
+            return tm;
+        };
+        // two pairs' terms per compensated addition (a padding entry's term
+        // with the particle itself is exactly +0: r = 0 gives grad W = 0)
+        for_each_nbr_pairs<G, true>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &a, const Rec &b) {
+            const Term ta = term(a), tb = term(b);
+#pragma unroll
+            for (int d = 0; d < D; d++) two_sum_acc(as[d], ac[d], ta.t[d] + tb.t[d]);
         });
 #pragma unroll
         for (int d = 0; d < D; d++) ai[d] = (double)as[d] + (double)ac[d];

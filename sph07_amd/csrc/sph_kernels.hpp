@@ -67,6 +67,33 @@ __device__ __forceinline__ void for_each_nbr_batched(const int *nbr, int i, int 
     }
 }
 
+// As for_each_nbr_batched, consuming the entries two at a time in list
+// order: use2(rec_a, rec_b) for entries u and u + 1 whenever entry u is in
+// the list.  Entry u + 1 may be padding (the particle's own index): the
+// caller's pair term of a particle with itself must be exactly +0.
+template <int G, bool LOADS_FIRST = false, class Load, class Use2>
+__device__ __forceinline__ void for_each_nbr_pairs(const int *nbr, int i, int nn, int ncap, Load &&load, Use2 &&use2)
+{
+    static_assert(kNbrChunk % G == 0 && G % 2 == 0, "batch divides the chunk, entries go in twos");
+    for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
+        const int4 *p = reinterpret_cast<const int4 *>(nbr + nbr_slot(k0, i, ncap));
+        const int4 c0 = p[0], c1 = p[1];
+        const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        const int m = nn - k0;
+#pragma unroll
+        for (int g0 = 0; g0 < kNbrChunk; g0 += G) {
+            if (g0 >= m) break;
+            decltype(load(0)) rec[G];
+#pragma unroll
+            for (int u = 0; u < G; u++) rec[u] = load(e[g0 + u]);
+            if (LOADS_FIRST) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < G; u += 2)
+                if (g0 + u < m) use2(rec[u], rec[u + 1]);
+        }
+    }
+}
+
 #endif
 
 // masks of launch_strain
