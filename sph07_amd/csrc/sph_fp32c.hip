@@ -66,10 +66,12 @@ __device__ __forceinline__ PairConst32 pc32(const PairConst &pc, double qinv)
 // derivative (MKernelBSpline.cpp:166-172) in branch-free form:
 //   dW/dz = c1 (3 (1-z)+^2 - 0.75 (2-z)+^2),
 // which is c1 (2.25 z^2 - 3 z) for z < 1, -3 c2 (2-z)^2 for 1 <= z < 2
-// (c2 = c1 / 4) and 0 beyond; f = 0 at r = 0 as in the reference.
+// (c2 = c1 / 4) and 0 beyond; f = 0 at r = 0 as in the reference.  r^2 is
+// in units of q^2 and separations are whole quanta, so any r > 0 has r^2 >= 1:
+// r^2 = 0 takes rinv = 1, z = 0 and dW/dz = 3 - 0.75 * 4 = 0 exactly.
 __device__ __forceinline__ float gradw_factor(float r2, const PairConst32 &q)
 {
-    const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0f));
     const float z = r2 * rinv * q.ih;
     const float a = fmaxf(2.0f - z, 0.0f), b = fmaxf(1.0f - z, 0.0f);
     const float dwz = 3.0f * (b * b) - 0.75f * (a * a);
