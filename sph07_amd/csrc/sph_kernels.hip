@@ -638,115 +638,87 @@ __device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, i
     if (P.want_mindist) A.mindist[i] = mind;
 }
 
-// one particle's walk (the batched path, and the fallback of the cell kernel)
+// One particle's walk (the batched path, the fallback of the cell kernel for
+// oversized stencils, and a launch shape of its own).  Every lane steps
+// through the 3^D stencil cells in the same order, x -> y -> z ascending
+// (z fastest), so that a wave's lanes, whose particles lie in different
+// cells, run one loop body: per cell its periodic images newest first, then
+// its slots (descending deck number), the order of the reference's linked
+// lists.  A lane's loop over a cell's candidates has its own trip count.  The
+// next cell's range is loaded while the current cell's candidates are tested.
+// A stencil cell whose box lies beyond the largest cut-off (with a relative
+// margin far above FP64 rounding) holds no neighbour and is not walked.
+template <int D>
+struct WalkCell {
+    int s0, len, g0, ng;
+};
+
+template <int D>
+__device__ __forceinline__ WalkCell<D> walk_cell(const Params &P, const Arrays &A, const Ctl *ctl, int k,
+                                                 const int (&bc)[3], unsigned int q, const double (&dl)[3],
+                                                 const double (&dh)[3], double skip2)
+{
+    constexpr int W1 = D == 3 ? 3 : 1, W0 = D == 3 ? 9 : (D == 2 ? 3 : 1);
+    const int off[3] = {k / W0 - 1, D > 1 ? (k / W1) % 3 - 1 : 0, D > 2 ? k % 3 - 1 : 0};
+    WalkCell<D> w{0, 0, 0, 0};
+    bool in = true;
+    double box2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const int c = bc[d] + off[d];
+        in = in && c >= 0 && c < ctl->glim[d];
+        box2 += off[d] < 0 ? dl[d] : (off[d] > 0 ? dh[d] : 0.0);
+    }
+    if (!in || box2 > skip2) return w;
+    const unsigned int qc = code_step(q, ctl->cmask[0], off[0]) | code_step(q, ctl->cmask[1], off[1]) |
+                            code_step(q, ctl->cmask[2], off[2]);
+    w.s0 = A.cell_start[qc];
+    w.len = A.cell_start[qc + 1] - w.s0;
+    if (P.periodic) {
+        const int c = ctl->glim[2] * (ctl->glim[1] * (bc[0] + off[0]) + (bc[1] + off[1])) + (bc[2] + off[2]);
+        w.ng = A.gcount[c];
+        if (w.ng) w.g0 = A.gfirst[c];
+    }
+    return w;
+}
+
 template <int D>
 __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A, const int i)
 {
     Ctl *ctl = A.ctl;
     const uint32_t fi = A.info[i];
-    const double mi = A.m[i];
     if (!(fi & kActive)) { A.nnbr[i] = 0; return; }
     double xi[D];
 #pragma unroll
     for (int d = 0; d < D; d++) xi[d] = A.x[d][i];
-    int bc[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    int bc[3] = {0, 0, 0};
+    // squared distance from xi to the low and high faces of its own cell
+    double dl[3] = {0.0, 0.0, 0.0}, dh[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int d = 0; d < D; d++) {
         bc[d] = (int)(ctl->gsize[d] * (xi[d] - ctl->gmin[d]));
-        lo[d] = bc[d] - 1 > 0 ? bc[d] - 1 : 0;
-        hi[d] = bc[d] + 1 < ctl->glim[d] - 1 ? bc[d] + 1 : ctl->glim[d] - 1;
+        const double f0 = xi[d] - (ctl->gmin[d] + bc[d] / ctl->gsize[d]);
+        const double f1 = (ctl->gmin[d] + (bc[d] + 1) / ctl->gsize[d]) - xi[d];
+        dl[d] = f0 > 0.0 ? f0 * f0 : 0.0;
+        dh[d] = f1 > 0.0 ? f1 * f1 : 0.0;
     }
-    const int gl1 = ctl->glim[1], gl2 = ctl->glim[2];
+    const unsigned int q = (unsigned int)(code_dep(bc[0], ctl->cmask[0]) | code_dep(bc[1], ctl->cmask[1]) |
+                                          code_dep(bc[2], ctl->cmask[2]));
     const int mat_i = info_mat(fi);
     const bool onemat = P.nmat == 1;
     double cut2 = P.pc[mat_i][mat_i].cut2;
+    const double skip2 = P.cut2max * (1.0 + 1.0e-9);
     double mind = 1.0e20;
     int nn = 0;
-    // Cells x -> y -> z ascending (z fastest), walked as ONE loop over this
-    // particle's candidates, so that lanes of a wave whose particles lie in
-    // different cells still execute a single loop body.  The cursor (r, s, t)
-    // carries the code bits of each coordinate (q0 | q1 | q2 is the cell code),
-    // stepped by masked increments.  Following cells of a row whose codes are
-    // consecutive and that hold no images extend the open storage range.
-    // (Plain scalars, no lambdas: captured arrays would be kept in scratch.)
-    const unsigned int m0 = ctl->cmask[0], m1 = ctl->cmask[1], m2 = ctl->cmask[2];
-    const int lo0 = lo[0], lo1 = lo[1], lo2 = lo[2], hi0 = hi[0], hi1 = hi[1], hi2 = hi[2];
-    const int q1lo = code_dep(lo1, m1), q2lo = code_dep(lo2, m2);
-    int r = lo0, s = lo1, t = lo2;
-    int q0 = code_dep(lo0, m0), q1 = q1lo, q2 = q2lo;
-    // Squared distance from xi to the faces of its own cell: a stencil cell
-    // whose box lies beyond the largest cut-off (with a relative margin far
-    // above FP64 rounding) holds no neighbour and is not walked.
-    double dlo2[3] = {0.0, 0.0, 0.0}, dhi2[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-        const double f0 = xi[d] - (ctl->gmin[d] + bc[d] / ctl->gsize[d]);
-        const double f1 = (ctl->gmin[d] + (bc[d] + 1) / ctl->gsize[d]) - xi[d];
-        dlo2[d] = f0 > 0.0 ? f0 * f0 : 0.0;
-        dhi2[d] = f1 > 0.0 ? f1 * f1 : 0.0;
-    }
-    const double skip2 = P.cut2max * (1.0 + 1.0e-9);
-    const int bc0 = bc[0], bc1 = bc[1], bc2 = bc[2];
-    const double dl0 = dlo2[0], dl1 = dlo2[1], dl2 = dlo2[2], dh0 = dhi2[0], dh1 = dhi2[1], dh2 = dhi2[2];
-    const int inner = D == 3 ? 1 : (D == 2 ? 2 : 3);     // level of the innermost walked axis
-    int cells = 1;                 // level of the last cursor step; 0: walk finished
-    int gleft = 0, gbase = 0;      // images of the open cell still to test (newest first)
-    int jc = 0, e = 0;             // real particles still to test
     bool ok = true;
-    // The walk yields segments of consecutive slots: a cell's images (newest
-    // first), then merged real ranges.
-    while (ok) {
-        int base, cnt, dir;
-        if (gleft > 0) {
-            base = P.N + gbase + gleft - 1;
-            cnt = gleft;
-            dir = -1;
-            gleft = 0;
-        } else if (jc < e) {
-            base = jc;
-            cnt = e - jc;
-            dir = 1;
-            jc = e;
-        } else if (cells) {
-            int q = q0 | q1 | q2;
-            const double box2 = (r < bc0 ? dl0 : (r > bc0 ? dh0 : 0.0)) + (s < bc1 ? dl1 : (s > bc1 ? dh1 : 0.0)) +
-                                (t < bc2 ? dl2 : (t > bc2 ? dh2 : 0.0));
-            const bool near = box2 <= skip2;
-            if (near) {
-                if (P.periodic) {
-                    const int c = gl2 * (gl1 * r + s) + t;
-                    gleft = A.gcount[c];
-                    if (gleft) gbase = A.gfirst[c];
-                }
-                jc = A.cell_start[q];
-                e = A.cell_start[q + 1];
-            }
-            // step the cursor; merge the following cells of the row while possible
-            while (true) {
-                if (t < hi2) { t++; q2 = (int)((((unsigned)q2 | ~m2) + 1u) & m2); cells = 1; }
-                else {
-                    t = lo2; q2 = q2lo;
-                    if (s < hi1) { s++; q1 = (int)((((unsigned)q1 | ~m1) + 1u) & m1); cells = 2; }
-                    else {
-                        s = lo1; q1 = q1lo;
-                        if (r < hi0) { r++; q0 = (int)((((unsigned)q0 | ~m0) + 1u) & m0); cells = 3; }
-                        else cells = 0;
-                    }
-                }
-                if (!near || cells != inner || (q0 | q1 | q2) != q + 1) break;
-                if (P.periodic && A.gcount[gl2 * (gl1 * r + s) + t] != 0) break;
-                const double nb2 = (r < bc0 ? dl0 : (r > bc0 ? dh0 : 0.0)) + (s < bc1 ? dl1 : (s > bc1 ? dh1 : 0.0)) +
-                                   (t < bc2 ? dl2 : (t > bc2 ? dh2 : 0.0));
-                if (nb2 > skip2) break;
-                q++;
-                e = A.cell_start[q + 1];
-            }
-            continue;
-        } else {
-            break;
-        }
+    constexpr int NC = D == 3 ? 27 : (D == 2 ? 9 : 3);
+    WalkCell<D> w = walk_cell<D>(P, A, ctl, 0, bc, q, dl, dh, skip2);
+    for (int k = 0; k < NC && ok; k++) {
+        const WalkCell<D> cur = w;
+        if (k + 1 < NC) w = walk_cell<D>(P, A, ctl, k + 1, bc, q, dl, dh, skip2);
+        const int cnt = cur.ng + cur.len;
         for (int u = 0; u < cnt; u++) {
-            const int cand = base + dir * u;
+            const int cand = u < cur.ng ? P.N + cur.g0 + cur.ng - 1 - u : cur.s0 + (u - cur.ng);
             if (cand == i) continue;
             double d2 = 0.0;
 #pragma unroll
