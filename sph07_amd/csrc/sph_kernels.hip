@@ -648,6 +648,12 @@ __device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, i
 // next cell's range is loaded while the current cell's candidates are tested.
 // A stencil cell whose box lies beyond the largest cut-off (with a relative
 // margin far above FP64 rounding) holds no neighbour and is not walked.
+// candidates of a stencil cell whose positions the walk loads together
+#ifndef SPH_WALK_BATCH
+#define SPH_WALK_BATCH 2
+#endif
+constexpr int kWalkBatch = SPH_WALK_BATCH;
+
 template <int D>
 struct WalkCell {
     int s0, len, g0, ng;
@@ -717,24 +723,39 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
         const WalkCell<D> cur = w;
         if (k + 1 < NC) w = walk_cell<D>(P, A, ctl, k + 1, bc, q, dl, dh, skip2);
         const int cnt = cur.ng + cur.len;
-        for (int u = 0; u < cnt; u++) {
-            const int cand = u < cur.ng ? P.N + cur.g0 + cur.ng - 1 - u : cur.s0 + (u - cur.ng);
-            if (cand == i) continue;
-            double d2 = 0.0;
+        // kWalkBatch candidates per pass: their positions are loaded together
+        // (past the cell's last candidate a pass reloads that one), then they
+        // are tested and appended in order
+        for (int u0 = 0; u0 < cnt && ok; u0 += kWalkBatch) {
+            int cd[kWalkBatch];
+            double xc[kWalkBatch][D];
 #pragma unroll
-            for (int d = 0; d < D; d++) {
-                const double dd = xi[d] - A.x[d][cand];
-                d2 += dd * dd;
+            for (int b = 0; b < kWalkBatch; b++) {
+                const int u = u0 + b < cnt ? u0 + b : cnt - 1;
+                cd[b] = u < cur.ng ? P.N + cur.g0 + cur.ng - 1 - u : cur.s0 + (u - cur.ng);
+#pragma unroll
+                for (int d = 0; d < D; d++) xc[b][d] = A.x[d][cd[b]];
             }
-            if (!onemat) cut2 = P.pc[mat_i][info_mat(A.info[cand])].cut2;
-            if (d2 < cut2) {
-                if (P.want_mindist) {
-                    const double r1 = sqrt(d2);
-                    mind = mind < r1 ? mind : r1;
+#pragma unroll
+            for (int b = 0; b < kWalkBatch; b++) {
+                const int cand = cd[b];
+                if (u0 + b >= cnt || cand == i) continue;
+                double d2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    const double dd = xi[d] - xc[b][d];
+                    d2 += dd * dd;
                 }
-                if (nn >= P.maxnbr) { ok = false; break; }
-                A.nbr[nbr_slot(nn, i, P.nstride)] = cand;
-                nn++;
+                if (!onemat) cut2 = P.pc[mat_i][info_mat(A.info[cand])].cut2;
+                if (d2 < cut2) {
+                    if (P.want_mindist) {
+                        const double r1 = sqrt(d2);
+                        mind = mind < r1 ? mind : r1;
+                    }
+                    if (nn >= P.maxnbr) { ok = false; break; }
+                    A.nbr[nbr_slot(nn, i, P.nstride)] = cand;
+                    nn++;
+                }
             }
         }
     }
