@@ -911,12 +911,14 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             if (lane == 0) A.ovf[atomicAdd(&ctl->novf, 1)] = q;
             continue;
         }
-        if (kNbrBoxFilter) {
+        if (kNbrBoxFilter && ncand > 64) {
             // Drop the candidates farther than the largest cut-off from the
             // box spanned by the cell's own particles (parts of the edge and
             // corner cells, and of the faces where the particles do not fill
             // the cell): no target of the cell can reach them.  The margin is
             // far above FP64 rounding; order is kept (in-place compaction).
+            // A stencil of one chunk (the 2D scenes) is tested as it is: the
+            // filter could not save a chunk there.
             double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int d = 0; d < D; d++) {
@@ -1004,20 +1006,27 @@ __device__ __forceinline__ void nbr_cells_body(const Params &P, const Arrays &A,
             const int mat0 = info_mat((uint32_t)__builtin_amdgcn_readfirstlane((int)fi));
             const double cut2_0 = P.pc[mat0][mat0].cut2;
             const int self0 = own + (tg - c0);          // staged index of target 0
-            constexpr int NB = nbr_blk<ONEMAT, MIND>();
+            // 2D and 1D stencils mostly fit one chunk: two are held there
+            constexpr int NB = D == 3 ? nbr_blk<ONEMAT, MIND>() : 2;
             for (int kb = 0; kb < ncand; kb += 64 * NB) {
                 int cv[NB];
                 double xcv[NB][D];
                 int matcv[NB];
+                const int nb = min(NB, (ncand - kb + 63) >> 6);
 #pragma unroll
                 for (int c = 0; c < NB; c++) {
-                    const int k = kb + c * 64 + lane;
-                    cv[c] = k < ncand ? stage[k] : tg;      // lanes past the list read a valid slot
+                    cv[c] = tg;
 #pragma unroll
-                    for (int d = 0; d < D; d++) xcv[c][d] = A.x[d][cv[c]];
-                    matcv[c] = ONEMAT ? 0 : info_mat(A.info[cv[c]]);
+                    for (int d = 0; d < D; d++) xcv[c][d] = 0.0;
+                    matcv[c] = 0;
+                    if (c < nb) {                           // chunks holding candidates only
+                        const int k = kb + c * 64 + lane;
+                        cv[c] = k < ncand ? stage[k] : tg;  // lanes past the list read a valid slot
+#pragma unroll
+                        for (int d = 0; d < D; d++) xcv[c][d] = A.x[d][cv[c]];
+                        matcv[c] = ONEMAT ? 0 : info_mat(A.info[cv[c]]);
+                    }
                 }
-                const int nb = min(NB, (ncand - kb + 63) >> 6);
                 for (int t = 0; t < nt; t++) {
                     // target t's position: a broadcast read from LDS (LDS
                     // instructions, where readlane would take six VALU ones)
