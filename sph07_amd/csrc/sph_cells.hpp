@@ -21,6 +21,9 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 // deposits the low bits of v on the set bits of m (cell coordinate -> code bits)
 __device__ __forceinline__ int code_dep(int v, unsigned int m)
 {
+    if (m == 0u) return 0;
+    if (((m + (m & (0u - m))) & m) == 0u)                       // one run of bits (row-major order)
+        return (int)(((unsigned int)v << __builtin_ctz(m)) & m);
     int r = 0;
     while (m) {
         const unsigned int low = m & (0u - m);

@@ -365,7 +365,7 @@ template <int D, int G, bool ONEMAT> __global__ __launch_bounds__(kBlock) void k
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
     if (b < nb) strain32_body<D, G, ONEMAT>(P, A, b);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Params *Pt, const Arrays *At, const int2 *map)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     if (Pt[e.x].nmat == 1) strain32_body<D, kStrain32Batch, true>(Pt[e.x], At[e.x], e.y);
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
     if (b < nb) momentum32_body<D, G, KIND, ONEMAT>(P, A, b);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *Pt, const Arrays *At, const int2 *map)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     const bool one = Pt[e.x].nmat == 1;

@@ -16,6 +16,17 @@
 
 namespace sph {
 
+// A pointer into global memory (address space 1): loads and stores through it
+// are global instructions, where a generic pointer read from a table of
+// pointers would give flat ones.
+template <class T>
+using GPtr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ GPtr<T> gptr(T *p)
+{
+    return (GPtr<T>)p;
+}
+
 #ifndef SPH_NBR_BOX_FILTER
 #define SPH_NBR_BOX_FILTER 1
 #endif
@@ -654,41 +665,47 @@ __device__ __forceinline__ void nbr_epilogue(const Params &P, const Arrays &A, i
 #endif
 constexpr int kWalkBatch = SPH_WALK_BATCH;
 
-template <int D>
 struct WalkCell {
     int s0, len, g0, ng;
 };
 
+// the grid and cell tables the walk reads, loaded once per particle
+struct WalkGrid {
+    int glim[3];
+    unsigned int cmask[3];
+    int periodic;
+    GPtr<const int> cstart, gcount, gfirst;
+};
+
 template <int D>
-__device__ __forceinline__ WalkCell<D> walk_cell(const Params &P, const Arrays &A, const Ctl *ctl, int k,
-                                                 const int (&bc)[3], unsigned int q, const double (&dl)[3],
-                                                 const double (&dh)[3], double skip2)
+__device__ __forceinline__ WalkCell walk_cell(const WalkGrid &g, int k, const int (&bc)[3], unsigned int q,
+                                              const double (&dl)[3], const double (&dh)[3], double skip2)
 {
     constexpr int W1 = D == 3 ? 3 : 1, W0 = D == 3 ? 9 : (D == 2 ? 3 : 1);
     const int off[3] = {k / W0 - 1, D > 1 ? (k / W1) % 3 - 1 : 0, D > 2 ? k % 3 - 1 : 0};
-    WalkCell<D> w{0, 0, 0, 0};
+    WalkCell w{0, 0, 0, 0};
     bool in = true;
     double box2 = 0.0;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         const int c = bc[d] + off[d];
-        in = in && c >= 0 && c < ctl->glim[d];
+        in = in && c >= 0 && c < g.glim[d];
         box2 += off[d] < 0 ? dl[d] : (off[d] > 0 ? dh[d] : 0.0);
     }
     if (!in || box2 > skip2) return w;
-    const unsigned int qc = code_step(q, ctl->cmask[0], off[0]) | code_step(q, ctl->cmask[1], off[1]) |
-                            code_step(q, ctl->cmask[2], off[2]);
-    w.s0 = A.cell_start[qc];
-    w.len = A.cell_start[qc + 1] - w.s0;
-    if (P.periodic) {
-        const int c = ctl->glim[2] * (ctl->glim[1] * (bc[0] + off[0]) + (bc[1] + off[1])) + (bc[2] + off[2]);
-        w.ng = A.gcount[c];
-        if (w.ng) w.g0 = A.gfirst[c];
+    const unsigned int qc = code_step(q, g.cmask[0], off[0]) | code_step(q, g.cmask[1], off[1]) |
+                            code_step(q, g.cmask[2], off[2]);
+    w.s0 = g.cstart[qc];
+    w.len = g.cstart[qc + 1] - w.s0;
+    if (g.periodic) {
+        const int c = g.glim[2] * (g.glim[1] * (bc[0] + off[0]) + (bc[1] + off[1])) + (bc[2] + off[2]);
+        w.ng = g.gcount[c];
+        if (w.ng) w.g0 = g.gfirst[c];
     }
     return w;
 }
 
-template <int D>
+template <int D, bool ONEMAT, bool MIND>
 __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A, const int i)
 {
     Ctl *ctl = A.ctl;
@@ -708,20 +725,36 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
         dl[d] = f0 > 0.0 ? f0 * f0 : 0.0;
         dh[d] = f1 > 0.0 ? f1 * f1 : 0.0;
     }
-    const unsigned int q = (unsigned int)(code_dep(bc[0], ctl->cmask[0]) | code_dep(bc[1], ctl->cmask[1]) |
-                                          code_dep(bc[2], ctl->cmask[2]));
+    WalkGrid g;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        g.glim[d] = ctl->glim[d];
+        g.cmask[d] = ctl->cmask[d];
+    }
+    g.periodic = P.periodic;
+    g.cstart = gptr<const int>(A.cell_start);
+    g.gcount = gptr<const int>(A.gcount);
+    g.gfirst = gptr<const int>(A.gfirst);
+    const unsigned int q = (unsigned int)(code_dep(bc[0], g.cmask[0]) | code_dep(bc[1], g.cmask[1]) |
+                                          code_dep(bc[2], g.cmask[2]));
     const int mat_i = info_mat(fi);
-    const bool onemat = P.nmat == 1;
     double cut2 = P.pc[mat_i][mat_i].cut2;
     const double skip2 = P.cut2max * (1.0 + 1.0e-9);
+    const int nreal = P.N, maxnbr = P.maxnbr;
+    // positions read as global memory; the list written through a row
+    // pointer (entry k of this particle: row k / 8, column k % 8) that steps
+    // one chunk row per 8 appended entries
+    const GPtr<double> xg[3] = {gptr(A.x[0]), gptr(A.x[D > 1 ? 1 : 0]), gptr(A.x[D > 2 ? 2 : 0])};
+    const size_t rstride = (size_t)P.nstride * kNbrChunk;
+    GPtr<int> row = gptr(A.nbr) + (size_t)i * kNbrChunk;
     double mind = 1.0e20;
     int nn = 0;
     bool ok = true;
     constexpr int NC = D == 3 ? 27 : (D == 2 ? 9 : 3);
-    WalkCell<D> w = walk_cell<D>(P, A, ctl, 0, bc, q, dl, dh, skip2);
+    WalkCell w = walk_cell<D>(g, 0, bc, q, dl, dh, skip2);
     for (int k = 0; k < NC && ok; k++) {
-        const WalkCell<D> cur = w;
-        if (k + 1 < NC) w = walk_cell<D>(P, A, ctl, k + 1, bc, q, dl, dh, skip2);
+        const WalkCell cur = w;
+        if (k + 1 < NC) w = walk_cell<D>(g, k + 1, bc, q, dl, dh, skip2);
         const int cnt = cur.ng + cur.len;
         // kWalkBatch candidates per pass: their positions are loaded together
         // (past the cell's last candidate a pass reloads that one), then they
@@ -732,9 +765,9 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
 #pragma unroll
             for (int b = 0; b < kWalkBatch; b++) {
                 const int u = u0 + b < cnt ? u0 + b : cnt - 1;
-                cd[b] = u < cur.ng ? P.N + cur.g0 + cur.ng - 1 - u : cur.s0 + (u - cur.ng);
+                cd[b] = u < cur.ng ? nreal + cur.g0 + cur.ng - 1 - u : cur.s0 + (u - cur.ng);
 #pragma unroll
-                for (int d = 0; d < D; d++) xc[b][d] = A.x[d][cd[b]];
+                for (int d = 0; d < D; d++) xc[b][d] = xg[d][cd[b]];
             }
 #pragma unroll
             for (int b = 0; b < kWalkBatch; b++) {
@@ -746,15 +779,16 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
                     const double dd = xi[d] - xc[b][d];
                     d2 += dd * dd;
                 }
-                if (!onemat) cut2 = P.pc[mat_i][info_mat(A.info[cand])].cut2;
+                if (!ONEMAT) cut2 = P.pc[mat_i][info_mat(A.info[cand])].cut2;
                 if (d2 < cut2) {
-                    if (P.want_mindist) {
+                    if (MIND) {
                         const double r1 = sqrt(d2);
                         mind = mind < r1 ? mind : r1;
                     }
-                    if (nn >= P.maxnbr) { ok = false; break; }
-                    A.nbr[nbr_slot(nn, i, P.nstride)] = cand;
+                    if (nn >= maxnbr) { ok = false; break; }
+                    row[nn & (kNbrChunk - 1)] = cand;
                     nn++;
+                    if ((nn & (kNbrChunk - 1)) == 0) row += rstride;
                 }
             }
         }
@@ -767,13 +801,27 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
     nbr_epilogue<D>(P, A, i, nn, mind);
 }
 
+// the walk specialised for one material (one cut-off) and for the
+// minimum-distance output (wave-uniform choice)
+template <int D>
+__device__ __forceinline__ void neighbours_walk(const Params &P, const Arrays &A, const int i)
+{
+    if (P.nmat == 1) {
+        if (P.want_mindist) neighbours_one<D, true, true>(P, A, i);
+        else neighbours_one<D, true, false>(P, A, i);
+    } else {
+        if (P.want_mindist) neighbours_one<D, false, true>(P, A, i);
+        else neighbours_one<D, false, false>(P, A, i);
+    }
+}
+
 template <int D>
 __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A, const int bid)
 {
     if (CTL_STOP(A.ctl)) return;
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
-    neighbours_one<D>(P, A, i);
+    neighbours_walk<D>(P, A, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1831,26 +1879,26 @@ __device__ __forceinline__ void finish_body(const Params &P, const Arrays &A, in
 #define SCENE_OF(e) const Params &Pb = Pt[(e)]; const Arrays &Ab = At[(e)]
 
 __global__ void k_grid_setup(Params P, Ctl *ctl) { grid_setup_body(P, ctl); }
-__global__ void kb_grid_setup(const Params *Pt, const Arrays *At, const int *sc, int ns)
+__global__ void kb_grid_setup(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc, int ns)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ns) { SCENE_OF(sc[t]); grid_setup_body(Pb, Ab.ctl); }
 }
 __global__ void k_zero_cells(Params P, Arrays A) { zero_cells_body(P, A, blockIdx.x, gridDim.x); }
-__global__ void kb_zero_cells(const Params *Pt, const Arrays *At, const int *sc, int nb)
+__global__ void kb_zero_cells(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc, int nb)
 {
     SCENE_OF(sc[blockIdx.x / nb]);
     zero_cells_body(Pb, Ab, blockIdx.x % nb, nb);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, Arrays A) { bin_count_body<D>(P, A, blockIdx.x); }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_bin_count(const Params *Pt, const Arrays *At, const int2 *map)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_bin_count(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
     bin_count_body<D>(Pb, Ab, e.y);
 }
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(Params P, Arrays A) { bin_scatter_body(P, A, blockIdx.x); }
-__global__ __launch_bounds__(kBlock) void kb_bin_scatter(const Params *Pt, const Arrays *At, const int2 *map)
+__global__ __launch_bounds__(kBlock) void kb_bin_scatter(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
@@ -1860,19 +1908,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *in, const int
 {
     scan_reduce_body(in, n_ptr, aux, ctl, blockIdx.x);
 }
-__global__ __launch_bounds__(kBlock) void kb_scan_reduce(const Arrays *At, const int2 *map)
+__global__ __launch_bounds__(kBlock) void kb_scan_reduce(const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     const Arrays &Ab = At[e.x];
     scan_reduce_body(Ab.cell_count, &Ab.ctl->nbucket, Ab.scan_aux, Ab.ctl, e.y);
 }
-__global__ __launch_bounds__(kBlock) void kb_scan_aux(const Params *Pt, const Arrays *At, const int *sc)
+__global__ __launch_bounds__(kBlock) void kb_scan_aux(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc)
 {
     SCENE_OF(sc[blockIdx.x]);
     const int nb = (Pb.ncell_cap + kScanTile - 1) / kScanTile;
     scan_aux_body(Ab.scan_aux, nb, &Ab.ctl->nbucket, Ab.cell_start, Ab.ctl);
 }
-__global__ __launch_bounds__(kBlock) void kb_scan_down(const Arrays *At, const int2 *map)
+__global__ __launch_bounds__(kBlock) void kb_scan_down(const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     const Arrays &Ab = At[e.x];
@@ -1884,7 +1932,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_down_clear(int *in, int *out, c
     scan_down_clear_body(in, out, n_ptr, aux, nblocks, ctl, blockIdx.x);
 }
 __global__ void k_reset_ngp(Ctl *ctl) { reset_ngp_body(ctl); }
-__global__ void kb_reset_ngp(const Arrays *At, const int *sc, int ns)
+__global__ void kb_reset_ngp(const Arrays *__restrict__ At, const int *sc, int ns)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ns) reset_ngp_body(At[sc[t]].ctl);
@@ -1905,17 +1953,17 @@ __device__ __forceinline__ bool has_pass(const Params &P, int m, int side)
     const int code = (m == 0) ? P.bcode[0][0] : P.bcode[side][m];
     return code == 2;
 }
-__global__ __launch_bounds__(kBlock) void kb_ghost_count(const Params *Pt, const Arrays *At, const int *sc, int nb, int m, int side)
+__global__ __launch_bounds__(kBlock) void kb_ghost_count(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc, int nb, int m, int side)
 {
     SCENE_OF(sc[blockIdx.x / nb]);
     if (has_pass(Pb, m, side)) ghost_count_body(Pb, Ab, m, side, blockIdx.x % nb, nb);
 }
-__global__ __launch_bounds__(kBlock) void kb_ghost_scan(const Params *Pt, const Arrays *At, const int *sc, int m, int side)
+__global__ __launch_bounds__(kBlock) void kb_ghost_scan(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc, int m, int side)
 {
     SCENE_OF(sc[blockIdx.x]);
     if (has_pass(Pb, m, side)) ghost_scan_body(Pb, Ab, m, side);
 }
-__global__ __launch_bounds__(kBlock) void kb_ghost_emit(const Params *Pt, const Arrays *At, const int *sc, int nb, int m, int side)
+__global__ __launch_bounds__(kBlock) void kb_ghost_emit(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc, int nb, int m, int side)
 {
     SCENE_OF(sc[blockIdx.x / nb]);
     if (has_pass(Pb, m, side)) ghost_emit_body(Pb, Ab, m, side, blockIdx.x % nb, nb);
@@ -1934,7 +1982,7 @@ template <int D> __global__ __launch_bounds__(kBlock) void k_accel_sq(Params P, 
     for (int d = 0; d < D; d++) s += A.a[d][i] * A.a[d][i];
     A.aa[i] = s;
 }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Params *Pt, const Arrays *At, const Arrays *At2, const int2 *map)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_permute(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const Arrays *__restrict__ At2, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     permute_body<D>(Pt[e.x], At[e.x], At2[e.x], e.y, false);   // batched scenes are Morris (a rewritten)
@@ -1968,7 +2016,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(nbr_wave
 // (the scene's waves split its slots as in k_nbr_cells); the variant is
 // chosen per scene (uniform per block).
 template <int D>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES - 1, 8))) void kb_nbr_cells(const Params *Pt, const Arrays *At, const int4 *map)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES - 1, 8))) void kb_nbr_cells(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int4 *map)
 {
     __shared__ int stage[kCellWaves][kCandCap];
     __shared__ double tpos[kCellWaves][kNbrTposLds ? 3 * 64 : 1];
@@ -2002,12 +2050,12 @@ template <int D> __global__ __launch_bounds__(kBlock) void k_nbr_ovf(Params P, A
     for (int w = blockIdx.x; w < n; w += gridDim.x) {
         const int q = A.ovf[w];
         const int c0 = A.cell_start[q], c1 = A.cell_start[q + 1];
-        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(P, A, i);
+        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_walk<D>(P, A, i);
     }
 }
 // overflow cells of the batched cell kernel: kOvfBlocksPerScene blocks per scene
 constexpr int kOvfBlocksPerScene = 8;
-template <int D> __global__ __launch_bounds__(kBlock) void kb_nbr_ovf(const Params *Pt, const Arrays *At, const int *sc)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_nbr_ovf(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc)
 {
     const int s = sc[blockIdx.x / kOvfBlocksPerScene], part = (int)blockIdx.x % kOvfBlocksPerScene;
     SCENE_OF(s);
@@ -2017,10 +2065,10 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_nbr_ovf(const Para
     for (int w = part; w < n; w += kOvfBlocksPerScene) {
         const int q = Ab.ovf[w];
         const int c0 = Ab.cell_start[q], c1 = Ab.cell_start[q + 1];
-        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_one<D>(Pb, Ab, i);
+        for (int i = c0 + (int)threadIdx.x; i < c1; i += kBlock) neighbours_walk<D>(Pb, Ab, i);
     }
 }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *Pt, const Arrays *At, const int2 *map)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
@@ -2028,21 +2076,21 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_neighbours(const P
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_shift(Params P, Arrays A) { shift_body<D>(P, A, blockIdx.x); }
 template <int D, int G> __global__ __launch_bounds__(kBlock) void k_strain(Params P, Arrays A, int mask) { XCD_BID(P); strain_body<D, G>(P, A, mask, bid_); }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_strain(const Params *Pt, const Arrays *At, const int2 *map, int mask)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_strain(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map, int mask)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
     strain_body<D>(Pb, Ab, mask, e.y);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_normals(Params P, Arrays A) { XCD_BID(P); normals_body<D>(P, A, bid_); }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_normals(const Params *Pt, const Arrays *At, const int2 *map)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_normals(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
     if (Pb.st) normals_body<D>(Pb, Ab, e.y);
 }
 template <int D> __global__ __launch_bounds__(kBlock) void k_curvature(Params P, Arrays A) { XCD_BID(P); curvature_body<D>(P, A, bid_); }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_curvature(const Params *Pt, const Arrays *At, const int2 *map)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_curvature(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
@@ -2051,7 +2099,7 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_curvature(const Pa
 __global__ __launch_bounds__(64) void k_advance(Params P, Arrays A) { advance_body(P, A); }
 __global__ __launch_bounds__(64) void k_crit_fold(Arrays A) { crit_fold_body(A); }
 // one wave per scene
-__global__ __launch_bounds__(64) void kb_advance(const Params *Pt, const Arrays *At, const int *sc)
+__global__ __launch_bounds__(64) void kb_advance(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc)
 {
     SCENE_OF(sc[blockIdx.x]);
     advance_body(Pb, Ab);
@@ -2059,7 +2107,7 @@ __global__ __launch_bounds__(64) void kb_advance(const Params *Pt, const Arrays 
 template <int D, int G, bool CMP> __global__ __launch_bounds__(kBlock) void k_momentum(Params P, Arrays A) { XCD_BID(P); momentum_body<D, G, CMP>(P, A, bid_); }
 template <int D, int G> __global__ __launch_bounds__(kBlock) void k_strain_classic(Params P, Arrays A, int mask) { XCD_BID(P); strain_classic_body<D, G>(P, A, mask, bid_); }
 template <int D, int G> __global__ __launch_bounds__(kBlock) void k_momentum_classic(Params P, Arrays A) { XCD_BID(P); momentum_classic_body<D, G>(P, A, bid_); }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Params *Pt, const Arrays *At, const int2 *map)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
@@ -2067,14 +2115,14 @@ template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum(const Par
     else momentum_body<D, kMomentum64Batch, false>(Pb, Ab, e.y);
 }
 __global__ __launch_bounds__(kRigidBlock) void k_rigid_reduce(Params P, Arrays A, int kin) { rigid_reduce_body(P, A, kin); }
-__global__ __launch_bounds__(kRigidBlock) void kb_rigid_reduce(const Params *Pt, const Arrays *At, const int *sc, int kin)
+__global__ __launch_bounds__(kRigidBlock) void kb_rigid_reduce(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc, int kin)
 {
     SCENE_OF(sc[blockIdx.x]);
     if (Pb.rigid) rigid_reduce_body(Pb, Ab, kin);
 }
 __global__ void k_rigid_kin(Ctl *ctl) { rigid_kin_body(ctl); }
 template <int D> __global__ __launch_bounds__(kBlock) void k_integrate(Params P, Arrays A, int mask) { integrate_body<D>(P, A, mask, blockIdx.x); }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_integrate(const Params *Pt, const Arrays *At, const int2 *map, int mask)
+template <int D> __global__ __launch_bounds__(kBlock) void kb_integrate(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map, int mask)
 {
     const int2 e = map[blockIdx.x];
     SCENE_OF(e.x);
@@ -2085,7 +2133,7 @@ __global__ __launch_bounds__(64) void k_finish(Params P, Arrays A, int bbox, int
     finish_body(P, A, bbox, end, next_grid);
 }
 // one wave per scene
-__global__ __launch_bounds__(64) void kb_finish(const Params *Pt, const Arrays *At, const int *sc, int bbox, int end)
+__global__ __launch_bounds__(64) void kb_finish(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int *sc, int bbox, int end)
 {
     SCENE_OF(sc[blockIdx.x]);
     finish_body(Pb, Ab, bbox, end);
@@ -2453,7 +2501,7 @@ void launch_batch_step(const BatchView &B, hipStream_t st)
 }
 
 // gathers the control blocks of a batch into one array for a single copy
-__global__ void kb_gather_ctl(const Arrays *At, const int *sc, int ns, Ctl *out)
+__global__ void kb_gather_ctl(const Arrays *__restrict__ At, const int *sc, int ns, Ctl *out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ns) out[t] = *At[sc[t]].ctl;
