@@ -264,7 +264,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             Rec r;
             r.a = rec32_at<0>(A.fm, (uint32_t)j);
             r.b = rec32_at<1>(A.fm, (uint32_t)j);
-            if (KIND == 0) r.c = A.fmC[j];
+            if (KIND == 0) r.c = gload(A.fmC + j);
             return r;
         };
         const float rho0_i = (float)mi.rho0, c2_i = (float)(mi.c * mi.c), m_i = P.mass32[mat_i];

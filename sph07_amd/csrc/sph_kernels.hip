@@ -16,17 +16,6 @@
 
 namespace sph {
 
-// A pointer into global memory (address space 1): loads and stores through it
-// are global instructions, where a generic pointer read from a table of
-// pointers would give flat ones.
-template <class T>
-using GPtr = __attribute__((address_space(1))) T *;
-template <class T>
-__device__ __forceinline__ GPtr<T> gptr(T *p)
-{
-    return (GPtr<T>)p;
-}
-
 #ifndef SPH_NBR_BOX_FILTER
 #define SPH_NBR_BOX_FILTER 1
 #endif
@@ -1205,7 +1194,7 @@ __device__ __forceinline__ void strain_body(const Params &P, const Arrays &A, in
                 g[d] = 0.0;
             }
             struct Rec { double4 a, b; };
-            auto load = [&](int j) { return Rec{A.rsA[j], A.rsB[j]}; };
+            auto load = [&](int j) { return Rec{gload(A.rsA + j), gload(A.rsB + j)}; };
             for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
                 const double4 ra = rec.a, rb = rec.b;
                 const double volj = ra.w;
@@ -1397,7 +1386,8 @@ __device__ __forceinline__ void momentum_body(const Params &P, const Arrays &A, 
         }
         struct Rec { double4 a, b; double2 c; };
         auto load = [&](int j) {
-            return CMP ? Rec{A.rmA[j], A.rmB[j], make_double2(0.0, 0.0)} : Rec{A.rmA[j], A.rmB[j], A.rmC[j]};
+            return CMP ? Rec{gload(A.rmA + j), gload(A.rmB + j), make_double2(0.0, 0.0)}
+                       : Rec{gload(A.rmA + j), gload(A.rmB + j), gload(A.rmC + j)};
         };
         for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
             const double4 ra = rec.a, rb = rec.b;
@@ -1485,7 +1475,7 @@ __device__ __forceinline__ void strain_classic_body(const Params &P, const Array
                 for (int e = 0; e < D; e++) g[d][e] = 0.0;
             }
             struct Rec { double4 a, b; };
-            auto load = [&](int j) { return Rec{A.rsA[j], A.rsB[j]}; };
+            auto load = [&](int j) { return Rec{gload(A.rsA + j), gload(A.rsB + j)}; };
             for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
                 const double4 ra = rec.a, rb = rec.b;
                 const double volj = ra.w;
@@ -1587,7 +1577,7 @@ __device__ __forceinline__ void momentum_classic_body(const Params &P, const Arr
 #pragma unroll
     for (int d = 0; d < D; d++) ai[d] = 0.0;
     struct Rec { double4 a, b, c; };
-    auto load = [&](int j) { return Rec{A.rmA[j], A.rmB[j], A.rcC[j]}; };
+    auto load = [&](int j) { return Rec{gload(A.rmA + j), gload(A.rmB + j), gload(A.rcC + j)}; };
     for_each_nbr_batched<G>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &rec) {
         const double4 ra = rec.a, rb = rec.b, rc = rec.c;
         const int mat_j = P.nmat == 1 ? mat_i : info_mat((uint32_t)rb.w);

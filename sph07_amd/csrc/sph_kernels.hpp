@@ -20,14 +20,40 @@ __device__ __forceinline__ int xcd_block(int nb)
 #endif
 
 #ifdef __HIPCC__
+// A pointer into global memory (address space 1): loads and stores through it
+// are global instructions (a 32-bit offset from a base in scalar registers),
+// where a generic pointer read from a table of pointers (the batched kernels)
+// would give flat ones.
+template <class T>
+using GPtr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ GPtr<T> gptr(T *p)
+{
+    return (GPtr<T>)p;
+}
+
+// *p as a global load (T a vector type of 8, 16 or 32 bytes, naturally aligned)
+template <class T>
+__device__ __forceinline__ T gload(const T *p)
+{
+    typedef unsigned int W __attribute__((ext_vector_type(sizeof(T) / 4)));
+    return __builtin_bit_cast(T, *reinterpret_cast<GPtr<const W>>(gptr(p)));
+}
+
+// the first of the two 16-byte halves of chunk row k0 / 8 of particle i's list
+__device__ __forceinline__ const int4 *nbr_chunk(const int *nbr, int k0, int i, int ncap)
+{
+    return reinterpret_cast<const int4 *>(nbr + nbr_slot(k0, i, ncap));
+}
+
 // Calls f(j) for the nn entries of particle i in list order, reading the list
 // one chunk (two 16-byte loads) at a time.
 template <class F>
 __device__ __forceinline__ void for_each_nbr(const int *nbr, int i, int nn, int ncap, F &&f)
 {
     for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
-        const int4 *p = reinterpret_cast<const int4 *>(nbr + nbr_slot(k0, i, ncap));
-        const int4 c0 = p[0], c1 = p[1];
+        const int4 *p = nbr_chunk(nbr, k0, i, ncap);
+        const int4 c0 = gload(p), c1 = gload(p + 1);
         const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
         const int m = nn - k0;
 #pragma unroll
@@ -49,8 +75,8 @@ __device__ __forceinline__ void for_each_nbr_batched(const int *nbr, int i, int 
 {
     static_assert(kNbrChunk % G == 0, "batch divides the chunk");
     for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
-        const int4 *p = reinterpret_cast<const int4 *>(nbr + nbr_slot(k0, i, ncap));
-        const int4 c0 = p[0], c1 = p[1];
+        const int4 *p = nbr_chunk(nbr, k0, i, ncap);
+        const int4 c0 = gload(p), c1 = gload(p + 1);
         const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
         const int m = nn - k0;
 #pragma unroll
@@ -76,8 +102,8 @@ __device__ __forceinline__ void for_each_nbr_pairs(const int *nbr, int i, int nn
 {
     static_assert(kNbrChunk % G == 0 && G % 2 == 0, "batch divides the chunk, entries go in twos");
     for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
-        const int4 *p = reinterpret_cast<const int4 *>(nbr + nbr_slot(k0, i, ncap));
-        const int4 c0 = p[0], c1 = p[1];
+        const int4 *p = nbr_chunk(nbr, k0, i, ncap);
+        const int4 c0 = gload(p), c1 = gload(p + 1);
         const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
         const int m = nn - k0;
 #pragma unroll

@@ -15,7 +15,7 @@
 // Packed word: bits 0-3 material, bit 6 periodic image, bit 7 rigid.
 #pragma once
 
-#include "sph_state.hpp"
+#include "sph_kernels.hpp"
 
 namespace sph {
 
@@ -122,7 +122,7 @@ __device__ __forceinline__ void put_mrec32(const Params &P, const Arrays &A, int
 template <int PART>
 __device__ __forceinline__ float4 rec32_at(const float4 *base, uint32_t j)
 {
-    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + ((j << 5) + PART * 16u));
+    return gload(reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + ((j << 5) + PART * 16u)));
 }
 
 }  // namespace sph
