@@ -1656,6 +1656,8 @@ struct Batch {
     int cap_slots = 0;
     int *dscenes = nullptr;
     int2 *dpmap = nullptr, *dtmap = nullptr, *dwmap = nullptr;
+    int2 *dvmap[kSweep32Variants] = {};
+    int cap_vmap[kSweep32Variants] = {};
     int4 *dcmap = nullptr;
     int cap_scenes = 0, cap_pmap = 0, cap_tmap = 0, cap_cmap = 0, cap_wmap = 0;
     Ctl *dctl = nullptr, *hctl = nullptr; // gathered control blocks
@@ -1727,14 +1729,18 @@ int batch_rebuild(Batch *b)
     // Arrays table has the same capacity as the Params table
     static_assert(sizeof(Arrays) > 0, "");
     std::vector<int> scenes;
-    std::vector<int2> pmap, tmap, wmap;
+    std::vector<int2> pmap, tmap, wmap, vmap[kSweep32Variants];
     std::vector<int4> cmap;
     int any_p = 0, any_st = 0, any_r = 0, mask = 0;
     for (int s : b->live) {
         Context *c = b->slots[s];
         c->P.corder = b->corder;           // the next step sorts in this order
         scenes.push_back(s);
-        for (int lb = 0; lb < (c->N + kBlock - 1) / kBlock; lb++) pmap.push_back(make_int2(s, lb));
+        const int vv = sweep32_variant(c->P);
+        for (int lb = 0; lb < (c->N + kBlock - 1) / kBlock; lb++) {
+            pmap.push_back(make_int2(s, lb));
+            vmap[vv].push_back(make_int2(s, lb));
+        }
         // cell_slots 0: the per-particle walk for every scene (a tuning candidate)
         if (cell_lists_fit(c->P) && b->cell_slots > 0) {
             const int nb = batch_cell_blocks(c->N, b->cell_slots);
@@ -1754,6 +1760,8 @@ int batch_rebuild(Batch *b)
     }
     if (grow_dev(b, &b->dscenes, &b->cap_scenes, (int)scenes.size() + 1)) return SPH_ERR_HIP;
     if (grow_dev(b, &b->dpmap, &b->cap_pmap, (int)pmap.size() + 1)) return SPH_ERR_HIP;
+    for (int vv = 0; vv < kSweep32Variants; vv++)
+        if (grow_dev(b, &b->dvmap[vv], &b->cap_vmap[vv], (int)vmap[vv].size() + 1)) return SPH_ERR_HIP;
     if (grow_dev(b, &b->dtmap, &b->cap_tmap, (int)tmap.size() + 1)) return SPH_ERR_HIP;
     if (grow_dev(b, &b->dcmap, &b->cap_cmap, (int)cmap.size() + 1)) return SPH_ERR_HIP;
     if (grow_dev(b, &b->dwmap, &b->cap_wmap, (int)wmap.size() + 1)) return SPH_ERR_HIP;
@@ -1782,6 +1790,9 @@ int batch_rebuild(Batch *b)
     if (!scenes.empty()) {
         BCHK(b, hipMemcpy(b->dscenes, scenes.data(), scenes.size() * sizeof(int), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->dpmap, pmap.data(), pmap.size() * sizeof(int2), hipMemcpyHostToDevice));
+        for (int vv = 0; vv < kSweep32Variants; vv++)
+            if (!vmap[vv].empty())
+                BCHK(b, hipMemcpy(b->dvmap[vv], vmap[vv].data(), vmap[vv].size() * sizeof(int2), hipMemcpyHostToDevice));
         BCHK(b, hipMemcpy(b->dtmap, tmap.data(), tmap.size() * sizeof(int2), hipMemcpyHostToDevice));
         if (!cmap.empty())
             BCHK(b, hipMemcpy(b->dcmap, cmap.data(), cmap.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -1794,6 +1805,10 @@ int batch_rebuild(Batch *b)
     v.A2 = b->dA2;
     v.scenes = b->dscenes;
     v.pmap = b->dpmap;
+    for (int vv = 0; vv < kSweep32Variants; vv++) {
+        v.vmap[vv] = b->dvmap[vv];
+        v.nvblocks[vv] = (int)vmap[vv].size();
+    }
     v.tmap = b->dtmap;
     v.nscene = (int)scenes.size();
     v.npart_blocks = (int)pmap.size();
@@ -2187,6 +2202,7 @@ void sph_batch_destroy(void *batch)
     hipFree(b->dA2);
     hipFree(b->dscenes);
     hipFree(b->dpmap);
+    for (int2 *m : b->dvmap) hipFree(m);
     hipFree(b->dtmap);
     hipFree(b->dcmap);
     hipFree(b->dwmap);

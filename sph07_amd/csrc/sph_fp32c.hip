@@ -365,11 +365,12 @@ template <int D, int G, bool ONEMAT> __global__ __launch_bounds__(kBlock) void k
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
     if (b < nb) strain32_body<D, G, ONEMAT>(P, A, b);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_strain32(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
+template <int D, bool ONEMAT>
+__global__ __launch_bounds__(kBlock) void kb_strain32(const Params *__restrict__ Pt, const Arrays *__restrict__ At,
+                                                      const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    if (Pt[e.x].nmat == 1) strain32_body<D, kStrain32Batch, true>(Pt[e.x], At[e.x], e.y);
-    else strain32_body<D>(Pt[e.x], At[e.x], e.y);
+    strain32_body<D, kStrain32Batch, ONEMAT>(Pt[e.x], At[e.x], e.y);
 }
 template <int D, int G, int KIND, bool ONEMAT>
 __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
@@ -377,37 +378,59 @@ __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
     if (b < nb) momentum32_body<D, G, KIND, ONEMAT>(P, A, b);
 }
-template <int D> __global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int2 *map)
+template <int D, int KIND, bool ONEMAT>
+__global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *__restrict__ Pt, const Arrays *__restrict__ At,
+                                                        const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    const bool one = Pt[e.x].nmat == 1;
-    switch (Pt[e.x].mcompact) {
-    case 2: momentum32_body<D, kMomentum32Batch, 2, true>(Pt[e.x], At[e.x], e.y); break;
-    case 1:
-        if (one) momentum32_body<D, kMomentum32Batch, 1, true>(Pt[e.x], At[e.x], e.y);
-        else momentum32_body<D, kMomentum32Batch, 1, false>(Pt[e.x], At[e.x], e.y);
-        break;
-    default:
-        if (one) momentum32_body<D, kMomentum32Batch, 0, true>(Pt[e.x], At[e.x], e.y);
-        else momentum32_body<D, kMomentum32Batch, 0, false>(Pt[e.x], At[e.x], e.y);
+    momentum32_body<D, kMomentum32Batch, KIND, ONEMAT>(Pt[e.x], At[e.x], e.y);
+}
+
+template <int D>
+static void batch_strain32_d(const BatchView &B, hipStream_t st)
+{
+    for (int v = 0; v < kSweep32Variants; v++) {
+        if (!B.nvblocks[v]) continue;
+        const dim3 grid(B.nvblocks[v]), blk(kBlock);
+        if (v == 0 || v == 1 || v == 3)     // one material
+            hipLaunchKernelGGL((kb_strain32<D, true>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]);
+        else
+            hipLaunchKernelGGL((kb_strain32<D, false>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]);
     }
 }
 
+template <int D>
+static void batch_momentum32_d(const BatchView &B, hipStream_t st)
+{
+    for (int v = 0; v < kSweep32Variants; v++) {
+        if (!B.nvblocks[v]) continue;
+        const dim3 grid(B.nvblocks[v]), blk(kBlock);
+        switch (v) {
+        case 0: hipLaunchKernelGGL((kb_momentum32<D, 2, true>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]); break;
+        case 1: hipLaunchKernelGGL((kb_momentum32<D, 1, true>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]); break;
+        case 2: hipLaunchKernelGGL((kb_momentum32<D, 1, false>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]); break;
+        case 3: hipLaunchKernelGGL((kb_momentum32<D, 0, true>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]); break;
+        default: hipLaunchKernelGGL((kb_momentum32<D, 0, false>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]); break;
+        }
+    }
+}
+
+// one launch per record variant present in the batch (sweep32_variant)
 void launch_batch_strain32(const BatchView &B, hipStream_t st)
 {
     switch (B.ndim) {
-    case 1: hipLaunchKernelGGL(kb_strain32<1>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
-    case 2: hipLaunchKernelGGL(kb_strain32<2>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
-    default: hipLaunchKernelGGL(kb_strain32<3>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
+    case 1: batch_strain32_d<1>(B, st); break;
+    case 2: batch_strain32_d<2>(B, st); break;
+    default: batch_strain32_d<3>(B, st); break;
     }
 }
 
 void launch_batch_momentum32(const BatchView &B, hipStream_t st)
 {
     switch (B.ndim) {
-    case 1: hipLaunchKernelGGL(kb_momentum32<1>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
-    case 2: hipLaunchKernelGGL(kb_momentum32<2>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
-    default: hipLaunchKernelGGL(kb_momentum32<3>, dim3(B.npart_blocks), dim3(kBlock), 0, st, B.P, B.A2, B.pmap); break;
+    case 1: batch_momentum32_d<1>(B, st); break;
+    case 2: batch_momentum32_d<2>(B, st); break;
+    default: batch_momentum32_d<3>(B, st); break;
     }
 }
 

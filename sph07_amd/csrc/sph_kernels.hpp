@@ -175,6 +175,15 @@ void launch_kernel_eval(int ndim, const PairConst &pc, int n, const double *xi, 
 // Device tables of one batched launch: per-scene parameter and array blocks,
 // the scene list, and block maps (scene, local block) over particles and
 // scan tiles.  All scenes share ndim and arithmetic arm.
+// record variants of the FP32c sweeps: the momentum record (Params::mcompact)
+// and whether the scene has one material
+constexpr int kSweep32Variants = 5;
+inline int sweep32_variant(const Params &P)
+{
+    const bool one = P.nmat == 1;
+    return P.mcompact == 2 ? 0 : (P.mcompact == 1 ? (one ? 1 : 2) : (one ? 3 : 4));
+}
+
 struct BatchView {
     const Params *P;
     const Arrays *A;         // current buffer sets (binning reads these)
@@ -187,6 +196,11 @@ struct BatchView {
     // of the per-particle walk
     const int4 *cmap;
     const int2 *wmap;
+    // particle blocks of the FP32c sweeps by record variant (sweep32_variant):
+    // each variant has a kernel of its own, so that a kernel holds the
+    // registers of its own variant, not the largest one's
+    const int2 *vmap[kSweep32Variants];
+    int nvblocks[kSweep32Variants];
     int nscene, npart_blocks, ntile_blocks, ncell_blocks, nwalk_blocks;
     int zero_nb, slab_nb;    // fixed blocks per scene for grid-stride kernels
     int ndim, arm;
