@@ -927,7 +927,7 @@ int sph_ghost_setup(void *ctx)
 int sph_linked_list_neighbours(void *ctx)
 {
     Context *c = ctx_of(ctx);
-    return run_phase(c, [&] { launch_neighbours(c->P, cur(c), c->st); });
+    return run_phase(c, [&] { launch_neighbours(c->P, cur(c), c->st, c->tune.cell_slots); });
 }
 
 int sph_strain(void *ctx)

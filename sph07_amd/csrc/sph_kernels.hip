@@ -694,7 +694,9 @@ __device__ __forceinline__ WalkCell walk_cell(const WalkGrid &g, int k, const in
     return w;
 }
 
-template <int D, bool ONEMAT, bool MIND>
+// FIT: every list and position byte offset fits 32 bits (lists_fit32), so
+// the loads and stores take a 32-bit offset from a base in scalar registers
+template <int D, bool ONEMAT, bool MIND, bool FIT>
 __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A, const int i)
 {
     Ctl *ctl = A.ctl;
@@ -727,81 +729,105 @@ __device__ __forceinline__ void neighbours_one(const Params &P, const Arrays &A,
     const unsigned int q = (unsigned int)(code_dep(bc[0], g.cmask[0]) | code_dep(bc[1], g.cmask[1]) |
                                           code_dep(bc[2], g.cmask[2]));
     const int mat_i = info_mat(fi);
-    double cut2 = P.pc[mat_i][mat_i].cut2;
+    const double cut2_0 = P.pc[mat_i][mat_i].cut2;
     const double skip2 = P.cut2max * (1.0 + 1.0e-9);
-    const int nreal = P.N, maxnbr = P.maxnbr;
-    // positions read as global memory; the list written through a row
-    // pointer (entry k of this particle: row k / 8, column k % 8) that steps
-    // one chunk row per 8 appended entries
-    const GPtr<double> xg[3] = {gptr(A.x[0]), gptr(A.x[D > 1 ? 1 : 0]), gptr(A.x[D > 2 ? 2 : 0])};
+    const int nreal = P.N;
+    // list entry k of slot i: entry i * 8 + (k / 8) * nstride * 8 + k % 8;
+    // entries past the capacity (an overflow, reported below) all land in
+    // the spare chunk row past it
+    const unsigned int maxr = (unsigned int)nbr_round(P.maxnbr), cshift = (unsigned int)P.nshift - 2u;
+    const unsigned int tbase = (unsigned int)i * kNbrChunk;
     const size_t rstride = (size_t)P.nstride * kNbrChunk;
-    GPtr<int> row = gptr(A.nbr) + (size_t)i * kNbrChunk;
+    const GPtr<char> lst = gptr(reinterpret_cast<char *>(A.nbr));
+    const GPtr<const char> xb[3] = {gptr(reinterpret_cast<const char *>(A.x[0])),
+                                    gptr(reinterpret_cast<const char *>(A.x[D > 1 ? 1 : 0])),
+                                    gptr(reinterpret_cast<const char *>(A.x[D > 2 ? 2 : 0]))};
+    auto pos = [&](int j, int d) -> double {
+        if (FIT) return *reinterpret_cast<GPtr<const double>>(xb[d] + ((unsigned int)j << 3));
+        return reinterpret_cast<GPtr<const double>>(xb[d])[j];
+    };
     double mind = 1.0e20;
-    int nn = 0;
-    bool ok = true;
+    unsigned int nn = 0;
+    // valid: the candidate is in the cell (the test runs either way, so that
+    // the loaded positions are consumed in every pass and no wait for the
+    // list stores lands at the top of the next pass)
+    auto test = [&](int cand, const double (&xc)[D], bool valid) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const double dd = xi[d] - xc[d];
+            d2 += dd * dd;
+        }
+        const double cut2 = ONEMAT ? cut2_0 : P.pc[mat_i][info_mat(A.info[cand])].cut2;
+        if (d2 < cut2 && cand != i && valid) {
+            if (MIND) {
+                const double r1 = sqrt(d2);
+                mind = mind < r1 ? mind : r1;
+            }
+            const unsigned int ps = nn < maxr ? nn : maxr;
+            if (FIT)
+                *reinterpret_cast<GPtr<int>>(lst + ((tbase + ((ps >> 3) << cshift) + (ps & 7u)) << 2)) = cand;
+            else
+                reinterpret_cast<GPtr<int>>(lst)[(size_t)(ps >> 3) * rstride + tbase + (ps & 7u)] = cand;
+            nn++;
+        }
+    };
     constexpr int NC = D == 3 ? 27 : (D == 2 ? 9 : 3);
     WalkCell w = walk_cell<D>(g, 0, bc, q, dl, dh, skip2);
-    for (int k = 0; k < NC && ok; k++) {
+    for (int k = 0; k < NC; k++) {
         const WalkCell cur = w;
         if (k + 1 < NC) w = walk_cell<D>(g, k + 1, bc, q, dl, dh, skip2);
-        const int cnt = cur.ng + cur.len;
-        // kWalkBatch candidates per pass: their positions are loaded together
-        // (past the cell's last candidate a pass reloads that one), then they
-        // are tested and appended in order
-        for (int u0 = 0; u0 < cnt && ok; u0 += kWalkBatch) {
+        // the cell's images, newest first (one at a time: only the cells
+        // next to a periodic boundary have any)
+        for (int u = 0; u < cur.ng; u++) {
+            const int cand = nreal + cur.g0 + cur.ng - 1 - u;
+            double xc[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) xc[d] = pos(cand, d);
+            test(cand, xc, true);
+        }
+        // its slots, kWalkBatch per pass: their positions are loaded before
+        // any is tested (past the cell's last slot a pass reloads that one),
+        // then they are tested and appended in order
+        for (int u0 = 0; u0 < cur.len; u0 += kWalkBatch) {
             int cd[kWalkBatch];
             double xc[kWalkBatch][D];
 #pragma unroll
             for (int b = 0; b < kWalkBatch; b++) {
-                const int u = u0 + b < cnt ? u0 + b : cnt - 1;
-                cd[b] = u < cur.ng ? nreal + cur.g0 + cur.ng - 1 - u : cur.s0 + (u - cur.ng);
+                cd[b] = cur.s0 + (u0 + b < cur.len ? u0 + b : cur.len - 1);
 #pragma unroll
-                for (int d = 0; d < D; d++) xc[b][d] = xg[d][cd[b]];
+                for (int d = 0; d < D; d++) xc[b][d] = pos(cd[b], d);
             }
 #pragma unroll
-            for (int b = 0; b < kWalkBatch; b++) {
-                const int cand = cd[b];
-                if (u0 + b >= cnt || cand == i) continue;
-                double d2 = 0.0;
-#pragma unroll
-                for (int d = 0; d < D; d++) {
-                    const double dd = xi[d] - xc[b][d];
-                    d2 += dd * dd;
-                }
-                if (!ONEMAT) cut2 = P.pc[mat_i][info_mat(A.info[cand])].cut2;
-                if (d2 < cut2) {
-                    if (MIND) {
-                        const double r1 = sqrt(d2);
-                        mind = mind < r1 ? mind : r1;
-                    }
-                    if (nn >= maxnbr) { ok = false; break; }
-                    row[nn & (kNbrChunk - 1)] = cand;
-                    nn++;
-                    if ((nn & (kNbrChunk - 1)) == 0) row += rstride;
-                }
-            }
+            for (int b = 0; b < kWalkBatch; b++) test(cd[b], xc[b], u0 + b < cur.len);
         }
     }
-    if (!ok) {
+    if (nn > (unsigned int)P.maxnbr) {
         set_err(ctl, kErrNbrOverflow, A.orig[i]);
-        A.nnbr[i] = nn;
+        A.nnbr[i] = P.maxnbr;
         return;
     }
-    nbr_epilogue<D>(P, A, i, nn, mind);
+    nbr_epilogue<D>(P, A, i, (int)nn, mind);
 }
 
-// the walk specialised for one material (one cut-off) and for the
-// minimum-distance output (wave-uniform choice)
+// the walk specialised for one material (one cut-off), for the
+// minimum-distance output and for 32-bit offsets (wave-uniform choices)
+template <int D, bool FIT>
+__device__ __forceinline__ void neighbours_walk_fit(const Params &P, const Arrays &A, const int i)
+{
+    if (P.nmat == 1) {
+        if (P.want_mindist) neighbours_one<D, true, true, FIT>(P, A, i);
+        else neighbours_one<D, true, false, FIT>(P, A, i);
+    } else {
+        if (P.want_mindist) neighbours_one<D, false, true, FIT>(P, A, i);
+        else neighbours_one<D, false, false, FIT>(P, A, i);
+    }
+}
 template <int D>
 __device__ __forceinline__ void neighbours_walk(const Params &P, const Arrays &A, const int i)
 {
-    if (P.nmat == 1) {
-        if (P.want_mindist) neighbours_one<D, true, true>(P, A, i);
-        else neighbours_one<D, true, false>(P, A, i);
-    } else {
-        if (P.want_mindist) neighbours_one<D, false, true>(P, A, i);
-        else neighbours_one<D, false, false>(P, A, i);
-    }
+    if (lists_fit32(P)) neighbours_walk_fit<D, true>(P, A, i);
+    else neighbours_walk_fit<D, false>(P, A, i);
 }
 
 template <int D>
@@ -2207,7 +2233,7 @@ void launch_ghost_setup(const Params &P, const Arrays &A, hipStream_t st)
 
 bool cell_lists_fit(const Params &P)
 {
-    return (unsigned long long)P.nstride * (unsigned long long)nbr_rows(P.maxnbr) * 4ull <= 0xffffffffull;
+    return lists_fit32(P);
 }
 
 int batch_cell_blocks(int n, int slots)

@@ -134,6 +134,14 @@ struct Params {
     PairConst pc[kMaxMat + 1][kMaxMat + 1];    // 1-based
 };
 
+// whether the neighbour lists (nstride * nbr_rows(maxnbr) entries of 4 bytes)
+// fit 32-bit byte offsets; the per-particle arrays (fewer than nstride
+// entries of at most 8 bytes) then fit too
+inline __host__ __device__ bool lists_fit32(const Params &P)
+{
+    return (unsigned long long)P.nstride * (unsigned long long)nbr_rows(P.maxnbr) * 4ull <= 0xffffffffull;
+}
+
 struct Arrays {
     double *x[3], *xs[3], *v[3], *a[3];
     double *rho, *rho_new, *p, *prho2, *m;
