@@ -222,6 +222,158 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
     }
 }
 
+// The continuity sweep with two lanes per particle.  A 32-byte record takes
+// two 16-byte loads, and the sweep is bound by the vector L1's tag lookups
+// (one cache line per lookup, about one lookup per cycle per CU): with one
+// lane per particle each load instruction touches up to 64 records.  Here
+// lanes 2t and 2t + 1 both serve particle t and take its list entries two
+// at a time: the first load gives lane 2t half 0 and lane 2t + 1 half 1 of
+// entry a's record, the second gives lane 2t half 1 and lane 2t + 1 half 0
+// of entry b's record, so every load touches one record per lane pair.  A
+// swap of one half between the two lanes (DPP) completes record a in lane 2t
+// and record b in lane 2t + 1; each lane computes its pair's term, and the
+// sum of the two terms (the same in both lanes: addition commutes) goes into
+// both lanes' compensated accumulators, exactly the sum and order of the
+// one-lane form.  Block: kBlock particles on 2 * kBlock lanes.  Taken in 3D
+// (the 3D dam break: 239 -> 229 us per sweep); the 2D scenes, with a third of
+// the neighbours, run faster with one lane per particle (the ensemble's
+// batched step lost 8 % with two).
+#ifndef SPH_STRAIN32_PAIRS
+#define SPH_STRAIN32_PAIRS 1
+#endif
+template <int D>
+constexpr bool strain32_pairs()
+{
+    return SPH_STRAIN32_PAIRS != 0 && D == 3;
+}
+template <int D>
+constexpr int strain32_threads()
+{
+    return strain32_pairs<D>() ? 2 * kBlock : kBlock;
+}
+
+// v from the other lane of the lane pair (lanes 2t and 2t + 1 swap)
+__device__ __forceinline__ float pair_swap(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+
+// half `part` (0 or 1) of record j
+__device__ __forceinline__ float4 rec32_half(const float4 *base, uint32_t j, uint32_t part)
+{
+    return gload(reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + ((j << 5) + (part << 4))));
+}
+
+template <int D, int G = kStrain32Batch, bool ONEMAT = false>
+__device__ __forceinline__ void strain32_pair_body(const Params &P, const Arrays &A, const int bid)
+{
+    static_assert(G % 2 == 0 && kNbrChunk % G == 0, "entries go in twos");
+    Ctl *ctl = A.ctl;
+    if (CTL_STOP32(ctl)) return;
+    const uint32_t par = threadIdx.x & 1u;
+    const int i = bid * kBlock + (int)(threadIdx.x >> 1);
+    const bool inr = i < P.N;
+    const int ic = inr ? i : P.N - 1;
+    double crit = 1.0e20;
+    const uint32_t fi = A.info[ic];
+    const bool act = inr && (fi & kActive);
+    const int mat_i = info_mat(fi);
+    const Mat &mt = P.mat[mat_i];
+    const double dt = ctl->dt;
+    float ri[D], vi32[D];
+    float gs = 0.0f, gc = 0.0f;
+    const float4 fa = rec32_at<0>(A.fs, (uint32_t)ic), fb = rec32_at<1>(A.fs, (uint32_t)ic);
+    const float ri3[3] = {fa.x, fa.y, fa.z}, vi3[3] = {fb.x, fb.y, fb.z};
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        ri[d] = ri3[d];
+        vi32[d] = vi3[d];
+    }
+    const PairConst32 q0 = pc32(P.pc[mat_i][mat_i], P.qinv);
+    // the pair term of the one-lane form
+    auto pair = [&](const float4 ra, const float4 rb) {
+        const uint32_t pk_j = __float_as_uint(rb.w);
+        const PairConst32 q = ONEMAT ? q0 : pc32(P.pc[mat_i][rec_mat(pk_j)], P.qinv);
+        const float rj[3] = {ra.x, ra.y, ra.z}, vj[3] = {rb.x, rb.y, rb.z};
+        float dx[D], r2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            dx[d] = qsep(ri[d], rj[d]);
+            r2 += dx[d] * dx[d];
+        }
+        float dvx = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; d++) dvx += (vj[d] - vi32[d]) * dx[d];
+        return (gradw_factor(r2, q) * ra.w) * dvx;
+    };
+    if (act) {
+        // both lanes of a pair have the same particle, so the same trip counts
+        const int nn = A.nnbr[ic];
+        for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
+            const int4 *lp = nbr_chunk(A.nbr, k0, ic, P.nstride);
+            const int4 c0 = gload(lp), c1 = gload(lp + 1);
+            const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            const int m = nn - k0;
+#pragma unroll
+            for (int g0 = 0; g0 < kNbrChunk; g0 += G) {
+                if (g0 >= m) break;
+                // entries past nn are padding (the particle itself: a term of +0)
+                float4 ha[G / 2], hb[G / 2];
+#pragma unroll
+                for (int u = 0; u < G / 2; u++) {
+                    ha[u] = rec32_half(A.fs, (uint32_t)e[g0 + 2 * u], par);
+                    hb[u] = rec32_half(A.fs, (uint32_t)e[g0 + 2 * u + 1], par ^ 1u);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < G / 2; u++) {
+                    if (g0 + 2 * u >= m) break;
+                    // lane 2t: record a = (ha, swapped ha of lane 2t + 1);
+                    // lane 2t + 1: record b = (hb, swapped hb of lane 2t)
+                    const float4 own = par ? hb[u] : ha[u], snd = par ? ha[u] : hb[u];
+                    const float4 oth = make_float4(pair_swap(snd.x), pair_swap(snd.y), pair_swap(snd.z), pair_swap(snd.w));
+                    const float t = pair(own, oth);
+                    two_sum_acc(gs, gc, t + pair_swap(t));
+                }
+            }
+        }
+    }
+    if (inr && par == 0) {
+        double rho_i = A.rho[i];
+        if (act) {
+            const double tr = (double)gs + (double)gc;
+            A.trd[i] = tr;
+            double xi[D], vi[D], xr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+#pragma clang fp contract(off)
+                vi[d] = A.v[d][i];
+                xi[d] = A.x[d][i] - 0.5 * vi[d] * dt;
+                xr[d] = xi[d] + 0.5 * vi[d] * dt;
+                A.x[d][i] = xr[d];
+            }
+            rho_i = rho_i * (1.0 - tr * dt);
+            const double p = eos_p32(mt, rho_i);
+            const double pr = p / (rho_i * rho_i);
+            A.p[i] = p;
+            A.prho2[i] = pr;
+            put_mrec32<D>(P, A, i, xr, A.m[i], vi, rho_i, pr, fi);
+        }
+        A.rho_new[i] = rho_i;
+        const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
+        if (act && !(fi & kRigid)) crit = ct;
+    }
+    __shared__ double red[2 * kBlock / 64];
+    crit = wave_min32(crit);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = crit;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mn = red[0];
+        for (int k = 1; k < 2 * kBlock / 64; k++) mn = fmin(mn, red[k]);
+        atomicMin(A.bbox + (size_t)(bid % kBboxSlots) * kBboxStride + kCritSlot, dkey(mn));
+    }
+}
+
 // momentum sweep: pressure gradient, Morris viscosity, Monaghan term.
 // KIND: the momentum record of Params::mcompact (sph_rec32.hpp): 0 full
 // (40 bytes), 1 {R, rho - rho0}, {v, packed}, 2 {R, 1/rho}, {v, p/rho^2}
@@ -360,17 +512,20 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
     for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
 }
 
-template <int D, int G, bool ONEMAT> __global__ __launch_bounds__(kBlock) void k_strain32(Params P, Arrays A)
+template <int D, int G, bool ONEMAT> __global__ __launch_bounds__(strain32_threads<D>()) void k_strain32(Params P, Arrays A)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) strain32_body<D, G, ONEMAT>(P, A, b);
+    if (b >= nb) return;
+    if constexpr (strain32_pairs<D>()) strain32_pair_body<D, G, ONEMAT>(P, A, b);
+    else strain32_body<D, G, ONEMAT>(P, A, b);
 }
 template <int D, bool ONEMAT>
-__global__ __launch_bounds__(kBlock) void kb_strain32(const Params *__restrict__ Pt, const Arrays *__restrict__ At,
-                                                      const int2 *map)
+__global__ __launch_bounds__(strain32_threads<D>()) void kb_strain32(const Params *__restrict__ Pt, const Arrays *__restrict__ At,
+                                                                     const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    strain32_body<D, kStrain32Batch, ONEMAT>(Pt[e.x], At[e.x], e.y);
+    if constexpr (strain32_pairs<D>()) strain32_pair_body<D, kStrain32Batch, ONEMAT>(Pt[e.x], At[e.x], e.y);
+    else strain32_body<D, kStrain32Batch, ONEMAT>(Pt[e.x], At[e.x], e.y);
 }
 template <int D, int G, int KIND, bool ONEMAT>
 __global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
@@ -391,7 +546,7 @@ static void batch_strain32_d(const BatchView &B, hipStream_t st)
 {
     for (int v = 0; v < kSweep32Variants; v++) {
         if (!B.nvblocks[v]) continue;
-        const dim3 grid(B.nvblocks[v]), blk(kBlock);
+        const dim3 grid(B.nvblocks[v]), blk(strain32_threads<D>());
         if (v == 0 || v == 1 || v == 3)     // one material
             hipLaunchKernelGGL((kb_strain32<D, true>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]);
         else
@@ -439,11 +594,11 @@ static inline int nblk32(int n) { return (n + kBlock - 1) / kBlock; }
 template <int G, bool ONEMAT>
 static void strain32_gm(const Params &P, const Arrays &A, hipStream_t st)
 {
-    const dim3 grid(xcd_grid(nblk32(P.N))), blk(kBlock);
+    const dim3 grid(xcd_grid(nblk32(P.N)));
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL((k_strain32<1, G, ONEMAT>), grid, blk, 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL((k_strain32<2, G, ONEMAT>), grid, blk, 0, st, P, A); break;
-    default: hipLaunchKernelGGL((k_strain32<3, G, ONEMAT>), grid, blk, 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_strain32<1, G, ONEMAT>), grid, dim3(strain32_threads<1>()), 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL((k_strain32<2, G, ONEMAT>), grid, dim3(strain32_threads<2>()), 0, st, P, A); break;
+    default: hipLaunchKernelGGL((k_strain32<3, G, ONEMAT>), grid, dim3(strain32_threads<3>()), 0, st, P, A); break;
     }
 }
 
