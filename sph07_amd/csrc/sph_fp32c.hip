@@ -9,7 +9,8 @@
 // kernel gradient, pressure, viscous and Monaghan terms are FP32 with hardware
 // reciprocal and reciprocal square root; each particle's sums are accumulated
 // with the TwoSum error-free transformation (Knuth 1969) into (sum,
-// compensation) FP32 pairs, two neighbours' terms per addition, and
+// compensation) FP32 pairs (two neighbours' terms per addition in the
+// continuity sweep, four in the momentum sweep), and
 // converted to FP64 once at the end.
 //
 // Neighbour sets and their order are those of the FP64 arm (shared lists).
@@ -378,13 +379,18 @@ __device__ __forceinline__ void strain32_pair_body(const Params &P, const Arrays
 // KIND: the momentum record of Params::mcompact (sph_rec32.hpp): 0 full
 // (40 bytes), 1 {R, rho - rho0}, {v, packed}, 2 {R, 1/rho}, {v, p/rho^2}
 // (one material, no rigid body; implies ONEMAT).
-template <int D, int G = kMomentum32Batch, int KIND = 0, bool ONEMAT = false>
+// PAIRS: two lanes per particle, as in the continuity sweep's lane pairs
+// (the same sums in the same order: four terms per compensated addition,
+// (t_a + t_b) + (t_c + t_d), with t_a, t_c from lane 2t and t_b, t_d from
+// lane 2t + 1); block: kBlock particles on 2 * kBlock lanes.
+template <int D, int G = kMomentum32Batch, int KIND = 0, bool ONEMAT = false, bool PAIRS = false>
 __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A, const int bid)
 {
     static_assert(KIND != 2 || ONEMAT, "the one-material record has no material bits");
     const Ctl *ctl = A.ctl;
     if (CTL_STOP32(ctl)) return;
-    const int i = bid * kBlock + threadIdx.x;
+    const uint32_t par = PAIRS ? (threadIdx.x & 1u) : 0u;
+    const int i = bid * kBlock + (int)(PAIRS ? threadIdx.x >> 1 : threadIdx.x);
     if (i >= P.N) return;
     const uint32_t fi = A.info[i];
     double ai[D];
@@ -425,6 +431,11 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         const float nu_i = (float)(mi.nu * P.qinv);
         const float mnu_i = m_i * nu_i;
         const float acbar_i = (float)(mi.alpha * mi.c);     // alpha_bar * c_bar for equal materials
+        // one material and the material's mass (not the per-particle masses
+        // of the 40-byte record): m_j alpha_bar c_bar havg of the Monaghan
+        // term, a constant
+        constexpr bool kConstMach = ONEMAT && KIND != 0;
+        const float mach_i = m_i * acbar_i * q0.havg;
         struct Term { float t[D]; };
         auto term = [&](const Rec &rec) {
             const float4 ra = rec.a, rb = rec.b;
@@ -480,11 +491,11 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
                     const Mat &mj_ = P.mat[mat_j];
                     acbar = (float)(0.5 * (mi.alpha + mj_.alpha)) * (float)(0.5 * (mi.c + mj_.c));
                 }
-                const float mu = q.havg * vr * ir2e;
                 // 1 / rho_bar = 2 / (rho_i + rho_j) = 2 (1/rho_i)(1/rho_j) / (1/rho_i + 1/rho_j)
                 const float irbar = KIND == 2 ? 2.0f * irho_i * ir_j * __builtin_amdgcn_rcpf(irho_i + ir_j)
                                               : 2.0f * __builtin_amdgcn_rcpf(rho_i + rho_j);
-                const float pterm = mj * (acbar * mu) * irbar;
+                // m_j alpha_bar c_bar mu / rho_bar, mu = havg (v . r) / (r^2 + 0.01 havg^2)
+                const float pterm = kConstMach ? mach_i * (vr * ir2e) * irbar : mj * (acbar * (q.havg * vr * ir2e)) * irbar;
                 sdw += (visc && vr < 0.0f) ? pterm : 0.0f;
             }
             const float sf = sdw * f;
@@ -493,13 +504,58 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             for (int d = 0; d < D; d++) tm.t[d] = sf * dx[d] + tv * dv[d];
             return tm;
         };
-        // two pairs' terms per compensated addition (a padding entry's term
-        // with the particle itself is exactly +0: r = 0 gives grad W = 0)
-        for_each_nbr_pairs<G, true>(A.nbr, i, A.nnbr[i], P.nstride, load, [&](const Rec &a, const Rec &b) {
-            const Term ta = term(a), tb = term(b);
+        // four pairs' terms per compensated addition, summed as two sums of
+        // two (a padding entry's term with the particle itself is exactly +0:
+        // r = 0 gives grad W = 0); at least four records in flight per lane
+        constexpr int GQ = G < 4 ? 4 : G;
+        if constexpr (!PAIRS) {
+            for_each_nbr_quads<GQ, true>(A.nbr, i, A.nnbr[i], P.nstride, load,
+                                         [&](const Rec &a, const Rec &b, const Rec &c, const Rec &e) {
+                const Term ta = term(a), tb = term(b), tc = term(c), te = term(e);
 #pragma unroll
-            for (int d = 0; d < D; d++) two_sum_acc(as[d], ac[d], ta.t[d] + tb.t[d]);
-        });
+                for (int d = 0; d < D; d++) two_sum_acc(as[d], ac[d], (ta.t[d] + tb.t[d]) + (tc.t[d] + te.t[d]));
+            });
+        } else {
+            // entries a, b of an entry pair: lane 2t takes half 0 of a's record
+            // and half 1 of b's, lane 2t + 1 the other halves; one swap gives
+            // lane 2t record a and lane 2t + 1 record b
+            const int nn = A.nnbr[i];
+            for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
+                const int4 *lp = nbr_chunk(A.nbr, k0, i, P.nstride);
+                const int4 c0 = gload(lp), c1 = gload(lp + 1);
+                const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+                const int m = nn - k0;
+#pragma unroll
+                for (int g0 = 0; g0 < kNbrChunk; g0 += GQ) {
+                    if (g0 >= m) break;
+                    float4 ha[GQ / 2], hb[GQ / 2];
+                    float2 hc[GQ / 2];
+#pragma unroll
+                    for (int u = 0; u < GQ / 2; u++) {
+                        ha[u] = rec32_half(A.fm, (uint32_t)e[g0 + 2 * u], par);
+                        hb[u] = rec32_half(A.fm, (uint32_t)e[g0 + 2 * u + 1], par ^ 1u);
+                        if (KIND == 0) hc[u] = gload(A.fmC + e[g0 + 2 * u + (int)par]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    auto own = [&](int w) {
+                        const float4 o = par ? hb[w] : ha[w], sn = par ? ha[w] : hb[w];
+                        Rec r;
+                        r.a = o;
+                        r.b = make_float4(pair_swap(sn.x), pair_swap(sn.y), pair_swap(sn.z), pair_swap(sn.w));
+                        if (KIND == 0) r.c = hc[w];
+                        return r;
+                    };
+#pragma unroll
+                    for (int u = 0; u < GQ / 2; u += 2) {
+                        if (g0 + 2 * u >= m) break;
+                        const Term t1 = term(own(u)), t2 = term(own(u + 1));
+#pragma unroll
+                        for (int d = 0; d < D; d++)
+                            two_sum_acc(as[d], ac[d], (t1.t[d] + pair_swap(t1.t[d])) + (t2.t[d] + pair_swap(t2.t[d])));
+                    }
+                }
+            }
+        }
 #pragma unroll
         for (int d = 0; d < D; d++) ai[d] = (double)as[d] + (double)ac[d];
         if (P.st) {
@@ -508,6 +564,7 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
             for (int d = 0; d < D; d++) ai[d] = ai[d] + fst * A.nrm[d][i];
         }
     }
+    if (PAIRS && par) return;          // both lanes hold the sums; one stores them
 #pragma unroll
     for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
 }
@@ -528,17 +585,17 @@ __global__ __launch_bounds__(strain32_threads<D>()) void kb_strain32(const Param
     else strain32_body<D, kStrain32Batch, ONEMAT>(Pt[e.x], At[e.x], e.y);
 }
 template <int D, int G, int KIND, bool ONEMAT>
-__global__ __launch_bounds__(kBlock) void k_momentum32(Params P, Arrays A)
+__global__ __launch_bounds__(strain32_threads<D>()) void k_momentum32(Params P, Arrays A)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) momentum32_body<D, G, KIND, ONEMAT>(P, A, b);
+    if (b < nb) momentum32_body<D, G, KIND, ONEMAT, strain32_pairs<D>()>(P, A, b);
 }
 template <int D, int KIND, bool ONEMAT>
-__global__ __launch_bounds__(kBlock) void kb_momentum32(const Params *__restrict__ Pt, const Arrays *__restrict__ At,
-                                                        const int2 *map)
+__global__ __launch_bounds__(strain32_threads<D>()) void kb_momentum32(const Params *__restrict__ Pt, const Arrays *__restrict__ At,
+                                                                      const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    momentum32_body<D, kMomentum32Batch, KIND, ONEMAT>(Pt[e.x], At[e.x], e.y);
+    momentum32_body<D, kMomentum32Batch, KIND, ONEMAT, strain32_pairs<D>()>(Pt[e.x], At[e.x], e.y);
 }
 
 template <int D>
@@ -559,7 +616,7 @@ static void batch_momentum32_d(const BatchView &B, hipStream_t st)
 {
     for (int v = 0; v < kSweep32Variants; v++) {
         if (!B.nvblocks[v]) continue;
-        const dim3 grid(B.nvblocks[v]), blk(kBlock);
+        const dim3 grid(B.nvblocks[v]), blk(strain32_threads<D>());
         switch (v) {
         case 0: hipLaunchKernelGGL((kb_momentum32<D, 2, true>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]); break;
         case 1: hipLaunchKernelGGL((kb_momentum32<D, 1, true>), grid, blk, 0, st, B.P, B.A2, B.vmap[v]); break;
@@ -613,11 +670,11 @@ static void strain32_g(const Params &P, const Arrays &A, hipStream_t st)
 template <int G, int KIND, bool ONEMAT>
 static void momentum32_gc(const Params &P, const Arrays &A, hipStream_t st)
 {
-    const dim3 grid(xcd_grid(nblk32(P.N))), blk(kBlock);
+    const dim3 grid(xcd_grid(nblk32(P.N)));
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL((k_momentum32<1, G, KIND, ONEMAT>), grid, blk, 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL((k_momentum32<2, G, KIND, ONEMAT>), grid, blk, 0, st, P, A); break;
-    default: hipLaunchKernelGGL((k_momentum32<3, G, KIND, ONEMAT>), grid, blk, 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_momentum32<1, G, KIND, ONEMAT>), grid, dim3(strain32_threads<1>()), 0, st, P, A); break;
+    case 2: hipLaunchKernelGGL((k_momentum32<2, G, KIND, ONEMAT>), grid, dim3(strain32_threads<2>()), 0, st, P, A); break;
+    default: hipLaunchKernelGGL((k_momentum32<3, G, KIND, ONEMAT>), grid, dim3(strain32_threads<3>()), 0, st, P, A); break;
     }
 }
 

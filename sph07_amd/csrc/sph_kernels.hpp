@@ -120,6 +120,33 @@ __device__ __forceinline__ void for_each_nbr_pairs(const int *nbr, int i, int nn
     }
 }
 
+
+// As for_each_nbr_pairs, four entries at a time: use4(ra, rb, rc, rd) for
+// entries u .. u + 3 whenever entry u is in the list (the entries past it up
+// to the chunk boundary are padding, whose terms must be exactly +0).
+template <int G, bool LOADS_FIRST = false, class Load, class Use4>
+__device__ __forceinline__ void for_each_nbr_quads(const int *nbr, int i, int nn, int ncap, Load &&load, Use4 &&use4)
+{
+    static_assert(kNbrChunk % G == 0 && G % 4 == 0, "batch divides the chunk, entries go in fours");
+    for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
+        const int4 *p = nbr_chunk(nbr, k0, i, ncap);
+        const int4 c0 = gload(p), c1 = gload(p + 1);
+        const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        const int m = nn - k0;
+#pragma unroll
+        for (int g0 = 0; g0 < kNbrChunk; g0 += G) {
+            if (g0 >= m) break;
+            decltype(load(0)) rec[G];
+#pragma unroll
+            for (int u = 0; u < G; u++) rec[u] = load(e[g0 + u]);
+            if (LOADS_FIRST) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < G; u += 4)
+                if (g0 + u < m) use4(rec[u], rec[u + 1], rec[u + 2], rec[u + 3]);
+        }
+    }
+}
+
 #endif
 
 // masks of launch_strain
