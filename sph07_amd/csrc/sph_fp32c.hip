@@ -280,7 +280,7 @@ __device__ __forceinline__ float4 rec32_half(const float4 *base, uint32_t j, uin
 template <int D, int G = kStrain32Batch, bool ONEMAT = false>
 __device__ __forceinline__ void strain32_pair_body(const Params &P, const Arrays &A, const int bid)
 {
-    static_assert(G % 2 == 0 && kNbrChunk % G == 0, "entries go in twos");
+    static_assert(G == 2 || G == 4 || G == 8, "records in flight per lane");
     Ctl *ctl = A.ctl;
     if (CTL_STOP32(ctl)) return;
     const uint32_t par = threadIdx.x & 1u;
@@ -320,26 +320,40 @@ __device__ __forceinline__ void strain32_pair_body(const Params &P, const Arrays
         return (gradw_factor(r2, q) * ra.w) * dvx;
     };
     if (act) {
-        // both lanes of a pair have the same particle, so the same trip counts
+        // both lanes of a pair have the same particle, so the same trip
+        // counts.  A pass loads E = 2G entries' records (G per lane): for G = 8
+        // two chunks of the list (the chunks past the list read as the
+        // particle itself), else E entries of one chunk.
+        constexpr int E = 2 * G, CH = E > kNbrChunk ? E / kNbrChunk : 1;
         const int nn = A.nnbr[ic];
-        for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
-            const int4 *lp = nbr_chunk(A.nbr, k0, ic, P.nstride);
-            const int4 c0 = gload(lp), c1 = gload(lp + 1);
-            const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        for (int k0 = 0; k0 < nn; k0 += kNbrChunk * CH) {
+            int e[kNbrChunk * CH];
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                int4 c0 = make_int4(ic, ic, ic, ic), c1 = c0;
+                if (c == 0 || k0 + c * kNbrChunk < nn) {
+                    const int4 *lp = nbr_chunk(A.nbr, k0 + c * kNbrChunk, ic, P.nstride);
+                    c0 = gload(lp);
+                    c1 = gload(lp + 1);
+                }
+                const int ec[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+                for (int u = 0; u < kNbrChunk; u++) e[c * kNbrChunk + u] = ec[u];
+            }
             const int m = nn - k0;
 #pragma unroll
-            for (int g0 = 0; g0 < kNbrChunk; g0 += G) {
+            for (int g0 = 0; g0 < kNbrChunk * CH; g0 += E) {
                 if (g0 >= m) break;
                 // entries past nn are padding (the particle itself: a term of +0)
-                float4 ha[G / 2], hb[G / 2];
+                float4 ha[G], hb[G];
 #pragma unroll
-                for (int u = 0; u < G / 2; u++) {
+                for (int u = 0; u < G; u++) {
                     ha[u] = rec32_half(A.fs, (uint32_t)e[g0 + 2 * u], par);
                     hb[u] = rec32_half(A.fs, (uint32_t)e[g0 + 2 * u + 1], par ^ 1u);
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int u = 0; u < G / 2; u++) {
+                for (int u = 0; u < G; u++) {
                     if (g0 + 2 * u >= m) break;
                     // lane 2t: record a = (ha, swapped ha of lane 2t + 1);
                     // lane 2t + 1: record b = (hb, swapped hb of lane 2t)
@@ -519,8 +533,8 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         // four pairs' terms per compensated addition, summed as two sums of
         // two (a padding entry's term with the particle itself is exactly +0:
         // r = 0 gives grad W = 0); at least four records in flight per lane
-        constexpr int GQ = G < 4 ? 4 : G;
         if constexpr (!PAIRS) {
+            constexpr int GQ = G < 4 ? 4 : G;
             for_each_nbr_quads<GQ, true>(A.nbr, i, A.nnbr[i], P.nstride, load,
                                          [&](const Rec &a, const Rec &b, const Rec &c, const Rec &e) {
                 const Term ta = term(a), tb = term(b), tc = term(c), te = term(e);
@@ -530,20 +544,36 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         } else {
             // entries a, b of an entry pair: lane 2t takes half 0 of a's record
             // and half 1 of b's, lane 2t + 1 the other halves; one swap gives
-            // lane 2t record a and lane 2t + 1 record b
+            // lane 2t record a and lane 2t + 1 record b.  A pass loads E = 2GP
+            // entries' records (GP = max(G, 4) per lane, a whole chunk at least:
+            // a pass over half a chunk is not unrolled and its entries spill):
+            // for G = 8 two chunks of the list (the chunks past the list read as
+            // the particle itself).
+            constexpr int GP = G < 4 ? 4 : G;
+            constexpr int E = 2 * GP, CH = E > kNbrChunk ? E / kNbrChunk : 1;
             const int nn = A.nnbr[i];
-            for (int k0 = 0; k0 < nn; k0 += kNbrChunk) {
-                const int4 *lp = nbr_chunk(A.nbr, k0, i, P.nstride);
-                const int4 c0 = gload(lp), c1 = gload(lp + 1);
-                const int e[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            for (int k0 = 0; k0 < nn; k0 += kNbrChunk * CH) {
+                int e[kNbrChunk * CH];
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    int4 c0 = make_int4(i, i, i, i), c1 = c0;
+                    if (c == 0 || k0 + c * kNbrChunk < nn) {
+                        const int4 *lp = nbr_chunk(A.nbr, k0 + c * kNbrChunk, i, P.nstride);
+                        c0 = gload(lp);
+                        c1 = gload(lp + 1);
+                    }
+                    const int ec[kNbrChunk] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+                    for (int u = 0; u < kNbrChunk; u++) e[c * kNbrChunk + u] = ec[u];
+                }
                 const int m = nn - k0;
 #pragma unroll
-                for (int g0 = 0; g0 < kNbrChunk; g0 += GQ) {
+                for (int g0 = 0; g0 < kNbrChunk * CH; g0 += E) {
                     if (g0 >= m) break;
-                    float4 ha[GQ / 2], hb[GQ / 2];
-                    float2 hc[GQ / 2];
+                    float4 ha[GP], hb[GP];
+                    float2 hc[GP];
 #pragma unroll
-                    for (int u = 0; u < GQ / 2; u++) {
+                    for (int u = 0; u < GP; u++) {
                         ha[u] = rec32_half(A.fm, (uint32_t)e[g0 + 2 * u], par);
                         hb[u] = rec32_half(A.fm, (uint32_t)e[g0 + 2 * u + 1], par ^ 1u);
                         if (KIND == 0) hc[u] = gload(A.fmC + e[g0 + 2 * u + (int)par]);
@@ -557,8 +587,9 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
                         if (KIND == 0) r.c = hc[w];
                         return r;
                     };
+                    // quads of entries (a quad never straddles a chunk)
 #pragma unroll
-                    for (int u = 0; u < GQ / 2; u += 2) {
+                    for (int u = 0; u < GP; u += 2) {
                         if (g0 + 2 * u >= m) break;
                         const Term t1 = term(own(u)), t2 = term(own(u + 1));
 #pragma unroll
