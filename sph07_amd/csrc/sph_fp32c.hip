@@ -592,11 +592,24 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
                     for (int u = 0; u < GP; u += 2) {
                         if (g0 + 2 * u >= m) break;
                         const Term t1 = term(own(u)), t2 = term(own(u + 1));
+                        float q4[D];
 #pragma unroll
-                        for (int d = 0; d < D; d++)
-                            two_sum_acc(as[d], ac[d], (t1.t[d] + pair_swap(t1.t[d])) + (t2.t[d] + pair_swap(t2.t[d])));
+                        for (int d = 0; d < D; d++) q4[d] = (t1.t[d] + pair_swap(t1.t[d])) + (t2.t[d] + pair_swap(t2.t[d]));
+                        if constexpr (D == 3) {
+                            // the quad sums are the same in both lanes: lane 2t
+                            // accumulates x, lane 2t + 1 z (both in slot 0), both y
+                            two_sum_acc(as[0], ac[0], par ? q4[2] : q4[0]);
+                            two_sum_acc(as[1], ac[1], q4[1]);
+                        } else {
+#pragma unroll
+                            for (int d = 0; d < D; d++) two_sum_acc(as[d], ac[d], q4[d]);
+                        }
                     }
                 }
+            }
+            if constexpr (D == 3) {
+                as[2] = pair_swap(as[0]);
+                ac[2] = pair_swap(ac[0]);
             }
         }
 #pragma unroll
