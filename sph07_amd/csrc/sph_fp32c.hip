@@ -760,6 +760,8 @@ void launch_strain32(const Params &P, const Arrays &A, hipStream_t st, int batch
 
 void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st, int batch)
 {
+    // two lanes per particle pass over a whole chunk at least: batch 2 is batch 4
+    if (P.ndim == 3 && momentum32_pairs<3>() && batch == 2) batch = 4;
     switch (batch) {
     case 2: momentum32_g<2>(P, A, st); break;
     case 8: momentum32_g<8>(P, A, st); break;
