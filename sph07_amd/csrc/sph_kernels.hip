@@ -850,7 +850,13 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
 // list in candidate order, i.e. in the reference's order.  The target's count
 // stays wave-uniform across its chunks.
 // ---------------------------------------------------------------------------
-constexpr int kCellWaves = kBlock / 64;
+// waves per block of the cell kernel: the waves of a block hold its LDS until
+// the slowest of them is done, and waves of one cell kernel vary with their
+// cells' work, so blocks of one wave free each wave's LDS by itself
+#ifndef SPH_NBR_CELL_WAVES
+#define SPH_NBR_CELL_WAVES 1
+#endif
+constexpr int kCellWaves = SPH_NBR_CELL_WAVES, kCellBlock = 64 * kCellWaves;
 // candidate chunks held in registers per pass: 4 for the common one-material
 // search, 2 where per-candidate materials or distances take registers too
 #ifndef SPH_NBR_BLK
@@ -2018,7 +2024,7 @@ template <int D> __global__ __launch_bounds__(kBlock) void k_neighbours(Params P
 // kernel that holds all four, need more registers: one wave fewer
 template <bool ONEMAT, bool MIND> constexpr int nbr_waves() { return ONEMAT && !MIND ? SPH_NBR_WAVES : SPH_NBR_WAVES - 1; }
 template <int D, bool ONEMAT, bool MIND>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(nbr_waves<ONEMAT, MIND>(), 8))) void k_nbr_cells(Params P, Arrays A)
+__global__ __launch_bounds__(kCellBlock) __attribute__((amdgpu_waves_per_eu(nbr_waves<ONEMAT, MIND>(), 8))) void k_nbr_cells(Params P, Arrays A)
 {
     __shared__ int stage[kCellWaves][kCandCap];
     __shared__ double tpos[kCellWaves][kNbrTposLds ? 3 * 64 : 1];
@@ -2032,7 +2038,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(nbr_wave
 // (the scene's waves split its slots as in k_nbr_cells); the variant is
 // chosen per scene (uniform per block).
 template <int D>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES - 1, 8))) void kb_nbr_cells(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int4 *map)
+__global__ __launch_bounds__(kCellBlock) __attribute__((amdgpu_waves_per_eu(SPH_NBR_WAVES - 1, 8))) void kb_nbr_cells(const Params *__restrict__ Pt, const Arrays *__restrict__ At, const int4 *map)
 {
     __shared__ int stage[kCellWaves][kCandCap];
     __shared__ double tpos[kCellWaves][kNbrTposLds ? 3 * 64 : 1];
@@ -2251,7 +2257,7 @@ void launch_neighbours(const Params &P, const Arrays &A, hipStream_t st, int cel
         DISPATCH_D(P.ndim, hipLaunchKernelGGL(k_neighbours<DD>, dim3(xcd_grid(nblk(P.N))), dim3(kBlock), 0, st, P, A));
         return;
     }
-    const dim3 grid(cell_blocks(P.N, cell_slots)), blk(kBlock);
+    const dim3 grid(cell_blocks(P.N, cell_slots)), blk(kCellBlock);
     if (P.nmat == 1 && !P.want_mindist) {
         DISPATCH_D(P.ndim, hipLaunchKernelGGL((k_nbr_cells<DD, true, false>), grid, blk, 0, st, P, A));
     } else if (P.nmat == 1) {
@@ -2497,7 +2503,7 @@ void launch_batch_step(const BatchView &B, hipStream_t st)
             }
     }
     if (B.ncell_blocks > 0) {
-        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_cells<DD>, dim3(B.ncell_blocks), dim3(kBlock), 0, st, B.P, A2, B.cmap));
+        DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_cells<DD>, dim3(B.ncell_blocks), dim3(kCellBlock), 0, st, B.P, A2, B.cmap));
         DISPATCH_D(B.ndim, hipLaunchKernelGGL(kb_nbr_ovf<DD>, dim3(ns * kOvfBlocksPerScene), dim3(kBlock), 0, st, B.P, A2, B.scenes));
     }
     if (B.nwalk_blocks > 0)
