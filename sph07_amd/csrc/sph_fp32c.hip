@@ -128,6 +128,17 @@ __device__ __forceinline__ double wave_min32(double v)
     return v;
 }
 
+// The wave's minimum critical step into reduction slot (w % kBboxSlots) with
+// one atomic per wave: no block barrier, so a wave that is done leaves while
+// the other waves of its block still sweep (the minimum does not depend on
+// the grouping)
+__device__ __forceinline__ void crit_min_wave(const Arrays &A, int w, double crit)
+{
+    crit = wave_min32(crit);
+    if ((threadIdx.x & 63) == 0)
+        atomicMin(A.bbox + (size_t)(w % kBboxSlots) * kBboxStride + kCritSlot, dkey(crit));
+}
+
 // continuity sweep + RhoUpdate/EOS + critical step (fused mode of k_strain);
 // G neighbour records in flight per lane
 template <int D, int G = kStrain32Batch, bool ONEMAT = false>
@@ -212,15 +223,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
         const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
         if (act && !(fi & kRigid)) crit = ct;
     }
-    __shared__ double red[kBlock / 64];
-    crit = wave_min32(crit);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = crit;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int k = 1; k < kBlock / 64; k++) m = fmin(m, red[k]);
-        atomicMin(A.bbox + (size_t)(bid % kBboxSlots) * kBboxStride + kCritSlot, dkey(m));
-    }
+    crit_min_wave(A, bid * (kBlock / 64) + (int)(threadIdx.x >> 6), crit);
 }
 
 // The continuity sweep with two lanes per particle.  A 32-byte record takes
@@ -390,15 +393,7 @@ __device__ __forceinline__ void strain32_pair_body(const Params &P, const Arrays
         const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
         if (act && !(fi & kRigid)) crit = ct;
     }
-    __shared__ double red[2 * kBlock / 64];
-    crit = wave_min32(crit);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = crit;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double mn = red[0];
-        for (int k = 1; k < 2 * kBlock / 64; k++) mn = fmin(mn, red[k]);
-        atomicMin(A.bbox + (size_t)(bid % kBboxSlots) * kBboxStride + kCritSlot, dkey(mn));
-    }
+    crit_min_wave(A, bid * (2 * kBlock / 64) + (int)(threadIdx.x >> 6), crit);
 }
 
 // momentum sweep: pressure gradient, Morris viscosity, Monaghan term.

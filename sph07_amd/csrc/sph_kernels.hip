@@ -1180,20 +1180,16 @@ __device__ __forceinline__ double crit_one(const Params &P, const Arrays &A, con
     return ct;
 }
 
-// block minimum into the block's reduction slot (atomics spread over lines)
+// the minimum over the block's waves into the reduction slots, one atomic per
+// wave in slot (wave % kBboxSlots) (slots on separate lines): no block barrier,
+// so a wave that is done leaves while the block's other waves still sweep
+// (the minimum does not depend on the grouping)
 __device__ __forceinline__ void block_min_to(const Arrays &A, int bid, double v)
 {
-    unsigned long long *key = A.bbox + (size_t)(bid % kBboxSlots) * kBboxStride + kCritSlot;
-    __shared__ double red[kBlock / 64];
     v = wave_min(v);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) red[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int k = 1; k < kBlock / 64; k++) m = fmin(m, red[k]);
-        atomicMin(key, dkey(m));
-    }
+    const int w = bid * (kBlock / 64) + (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0)
+        atomicMin(A.bbox + (size_t)(w % kBboxSlots) * kBboxStride + kCritSlot, dkey(v));
 }
 
 // ---------------------------------------------------------------------------
