@@ -307,68 +307,75 @@ __device__ __forceinline__ void permute_body(const Params &P, const Arrays &S, c
     const int i = bid * kBlock + threadIdx.x;
     if (i >= P.N) return;
     const bool moving = !CTL_STOP(S.ctl);
+    // Every source field is read before any store: the destination arrays are
+    // other allocations, but the compiler cannot know it, and with loads and
+    // stores interleaved each load waited for the store before it.
+    const int c = moving ? S.cell_of[i] : -1;  // < 0 only with the out-of-grid error set
+    const int oi = S.orig[i];
+    const uint32_t fi = S.info[i];
+    // a fused step rewrites every a in its momentum sweep before reading it
+    // (the critical step reads aa): a moves only for the other paths, and
+    // when the loop has stopped (nothing rewrites it then)
+    const bool mva = move_a || !moving;
+    double xv[D], vv[D], av[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        xv[d] = S.x[d][i];
+        vv[d] = S.v[d][i];
+        av[d] = mva ? S.a[d][i] : 0.0;
+    }
+    const double aa = S.aa[i], rho = S.rho[i], m = S.m[i];
+    const double q = P.disctype == 0 ? S.q[i] : 0.0;
+    const int rr = P.rigid ? S.rrank[i] : -1;
+    const double dt = S.ctl->dt;
     // Order inside a cell: descending deck number, the order head insertion
     // leaves in the reference's linked lists (MSolution.cpp:143-144).  The
     // destination slot is the cell's first slot plus the number of its
     // members with a higher deck number (k_bin_scatter left their numbers in
     // tmp; with an error the storage stays as it is).
     int j = i;
-    if (moving) {
-        const int c = S.cell_of[i];     // < 0 only with the out-of-grid error set
-        if (c >= 0) {
-            const int s0 = S.cell_start[c], e0 = S.cell_start[c + 1];
-            const int oi = S.orig[i];
-            int r = 0;
-            for (int t = s0; t < e0; t++) r += S.tmp[t] > oi;
-            j = s0 + r;
-        }
+    if (c >= 0) {
+        const int s0 = S.cell_start[c], e0 = S.cell_start[c + 1];
+        int r = 0;
+        for (int t = s0; t < e0; t++) r += S.tmp[t] > oi;
+        j = s0 + r;
     }
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        T.x[d][j] = S.x[d][i];
-        T.v[d][j] = S.v[d][i];
+        T.x[d][j] = xv[d];
+        T.v[d][j] = vv[d];
+        if (mva) T.a[d][j] = av[d];
     }
-    // a fused step rewrites every a in its momentum sweep before reading it
-    // (the critical step reads aa): a moves only for the other paths, and
-    // when the loop has stopped (nothing rewrites it then)
-    if (move_a || !moving)
-#pragma unroll
-        for (int d = 0; d < D; d++) T.a[d][j] = S.a[d][i];
-    T.aa[j] = S.aa[i];
-    T.rho[j] = S.rho[i];
-    T.m[j] = S.m[i];
-    T.info[j] = S.info[i];
-    T.orig[j] = S.orig[i];
-    if (P.disctype == 0) T.q[j] = S.q[i];
+    T.aa[j] = aa;
+    T.rho[j] = rho;
+    T.m[j] = m;
+    T.info[j] = fi;
+    T.orig[j] = oi;
+    if (P.disctype == 0) T.q[j] = q;
     if (P.rigid) {
-        const int rr = S.rrank[i];
         T.rrank[j] = rr;
         if (rr >= 0) T.rigid_idx[rr] = j;
     }
     if (!moving) return;
     // cell code of every sorted slot (the inactive bucket is ncode)
-    T.occ[j] = S.cell_of[i];
-    if (!(S.info[i] & kActive)) T.nnbr[j] = 0;
+    T.occ[j] = c;
+    if (!(fi & kActive)) T.nnbr[j] = 0;
     {
         // MSolution::Strain's half step back (:334-338) and the continuity
         // record, written with the move (the neighbour search reads x only)
-        const Ctl *ctl = S.ctl;
-        const uint32_t fi = S.info[i];
         const bool act = fi & kActive;
-        const double dt = ctl->dt;
-        double xs[D], vi[D];
+        double xs[D];
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            vi[d] = S.v[d][i];
-            xs[d] = act ? S.x[d][i] - 0.5 * vi[d] * dt : S.x[d][i];
+            xs[d] = act ? xv[d] - 0.5 * vv[d] * dt : xv[d];
             // the FP32c continuity sweep derives xs from x itself
             if (P.arm != 1) T.xs[d][j] = xs[d];
         }
-        const double vol = S.m[i] / S.rho[i];
+        const double vol = m / rho;
         if (P.arm == 1) {
-            put_srec32<D>(P, T, j, xs, vol, vi, fi);
+            put_srec32<D>(P, T, j, xs, vol, vv, fi);
         } else {
-            put_srec<D>(T, j, xs, vol, vi, fi);
+            put_srec<D>(T, j, xs, vol, vv, fi);
         }
     }
 }
