@@ -85,9 +85,12 @@ __device__ __forceinline__ double eos_p32(const Mat &mt, double rho)
     return mt.eos_b * (pow(rho / mt.rho0, mt.eos_gamma) - 1.0);
 }
 
+// critical step of one particle, MSolution::CalcCriticalTimestep (:34-94),
+// from values its caller has read: aa (|a|^2 after the last step, option 3),
+// v, mindist (option 2), whether this is the first step
 template <int D>
-__device__ __forceinline__ double crit_one32(const Params &P, const Arrays &A, const Ctl *ctl, int i,
-                                             const Mat &mt, double rho_i)
+__device__ __forceinline__ double crit_of32(const Params &P, const Mat &mt, double rho_i, double aa,
+                                            const double (&v)[D], double mind, bool step1)
 {
     double ct;
     const double h = mt.h;
@@ -96,7 +99,7 @@ __device__ __forceinline__ double crit_one32(const Params &P, const Arrays &A, c
         const double b = 0.125 * (h * h) / mt.nu;
         ct = ct < b ? ct : b;
         // |a|^2 summed over the axes in order, after the last step (aa)
-        const double aabs = sqrt(A.aa[i]);
+        const double aabs = sqrt(aa);
         const double c3 = 0.25 * sqrt(h / aabs);
         ct = ct < c3 ? ct : c3;
         if (P.st) {
@@ -106,13 +109,57 @@ __device__ __forceinline__ double crit_one32(const Params &P, const Arrays &A, c
     } else {
         double va = 0.0;
 #pragma unroll
-        for (int d = 0; d < D; d++) va += A.v[d][i] * A.v[d][i];
+        for (int d = 0; d < D; d++) va += v[d] * v[d];
         va = sqrt(va);
-        const double len = P.tcrit_opt == 1 ? h : A.mindist[i];
+        const double len = P.tcrit_opt == 1 ? h : mind;
         ct = len / (mt.c + va);
-        if (ctl->timestep == 1) ct = 1.e-09;
+        if (step1) ct = 1.e-09;
     }
     return ct;
+}
+
+// The particle's epilogue of the continuity sweep: the half step forward
+// again, RhoUpdate, EOS, the momentum record and the critical step.  Every
+// input is read before any store (the compiler cannot tell the arrays apart
+// and would wait for each store before the next load).
+template <int D>
+__device__ __forceinline__ double strain32_finish(const Params &P, const Arrays &A, const Ctl *ctl, int i, bool act,
+                                                  uint32_t fi, const Mat &mt, double dt, float gs, float gc)
+{
+    double rho_i = A.rho[i];
+    double x0[D], vi[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        x0[d] = A.x[d][i];
+        vi[d] = A.v[d][i];
+    }
+    const double mi = A.m[i];
+    const double aa = P.tcrit_opt == 3 ? A.aa[i] : 0.0;
+    const double mind = P.tcrit_opt == 2 ? A.mindist[i] : 0.0;
+    const bool step1 = ctl->timestep == 1;
+    if (act) {
+        const double tr = (double)gs + (double)gc;
+        A.trd[i] = tr;
+        double xr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            // MSolution::Strain's half step back and forward again
+            // (:334-338, :353-357), each rounded as written (no FMA): the
+            // back step is the one k_permute used for the records
+#pragma clang fp contract(off)
+            const double xi = x0[d] - 0.5 * vi[d] * dt;
+            xr[d] = xi + 0.5 * vi[d] * dt;
+            A.x[d][i] = xr[d];
+        }
+        rho_i = rho_i * (1.0 - tr * dt);
+        const double p = eos_p32(mt, rho_i);
+        const double pr = p / (rho_i * rho_i);
+        A.p[i] = p;
+        A.prho2[i] = pr;
+        put_mrec32<D>(P, A, i, xr, mi, vi, rho_i, pr, fi);
+    }
+    A.rho_new[i] = rho_i;
+    return crit_of32<D>(P, mt, rho_i, aa, vi, mind, step1);
 }
 
 // a wave-uniform float kept in a scalar register
@@ -196,31 +243,7 @@ __device__ __forceinline__ void strain32_body(const Params &P, const Arrays &A, 
         });
     }
     if (inr) {
-        double rho_i = A.rho[i];
-        if (act) {
-            const double tr = (double)gs + (double)gc;
-            A.trd[i] = tr;
-            double xi[D], vi[D], xr[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-                // MSolution::Strain's half step back and forward again
-                // (:334-338, :353-357), each rounded as written (no FMA): the
-                // back step is the one k_permute used for the records
-#pragma clang fp contract(off)
-                vi[d] = A.v[d][i];
-                xi[d] = A.x[d][i] - 0.5 * vi[d] * dt;
-                xr[d] = xi[d] + 0.5 * vi[d] * dt;
-                A.x[d][i] = xr[d];
-            }
-            rho_i = rho_i * (1.0 - tr * dt);
-            const double p = eos_p32(mt, rho_i);
-            const double pr = p / (rho_i * rho_i);
-            A.p[i] = p;
-            A.prho2[i] = pr;
-            put_mrec32<D>(P, A, i, xr, A.m[i], vi, rho_i, pr, fi);
-        }
-        A.rho_new[i] = rho_i;
-        const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
+        const double ct = strain32_finish<D>(P, A, ctl, i, act, fi, mt, dt, gs, gc);
         if (act && !(fi & kRigid)) crit = ct;
     }
     crit_min_wave(A, bid * (kBlock / 64) + (int)(threadIdx.x >> 6), crit);
@@ -369,28 +392,7 @@ __device__ __forceinline__ void strain32_pair_body(const Params &P, const Arrays
         }
     }
     if (inr && par == 0) {
-        double rho_i = A.rho[i];
-        if (act) {
-            const double tr = (double)gs + (double)gc;
-            A.trd[i] = tr;
-            double xi[D], vi[D], xr[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-#pragma clang fp contract(off)
-                vi[d] = A.v[d][i];
-                xi[d] = A.x[d][i] - 0.5 * vi[d] * dt;
-                xr[d] = xi[d] + 0.5 * vi[d] * dt;
-                A.x[d][i] = xr[d];
-            }
-            rho_i = rho_i * (1.0 - tr * dt);
-            const double p = eos_p32(mt, rho_i);
-            const double pr = p / (rho_i * rho_i);
-            A.p[i] = p;
-            A.prho2[i] = pr;
-            put_mrec32<D>(P, A, i, xr, A.m[i], vi, rho_i, pr, fi);
-        }
-        A.rho_new[i] = rho_i;
-        const double ct = crit_one32<D>(P, A, ctl, i, mt, rho_i);
+        const double ct = strain32_finish<D>(P, A, ctl, i, act, fi, mt, dt, gs, gc);
         if (act && !(fi & kRigid)) crit = ct;
     }
     crit_min_wave(A, bid * (2 * kBlock / 64) + (int)(threadIdx.x >> 6), crit);
