@@ -410,3 +410,10 @@ def test_walk_shape_same_bits(name):
     assert np.array_equal(nn_c, nn_w) and np.array_equal(ids_c, ids_w)
     for f in (0, 1, 2, 3):
         assert np.array_equal(cell.field(f), walk.field(f)), f
+    # the phase entry point searches again on the final state (the walk's 2D
+    # positions refreshed from x)
+    cell.LinkedListNeighbours()
+    walk.LinkedListNeighbours()
+    nn_c, ids_c = cell.neighbours()
+    nn_w, ids_w = walk.neighbours()
+    assert np.array_equal(nn_c, nn_w) and np.array_equal(ids_c, ids_w)
