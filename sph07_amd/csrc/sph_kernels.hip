@@ -581,25 +581,26 @@ __device__ __forceinline__ void ghost_emit_body(const Params &P, const Arrays &A
         for (int r = lane; r < cnt; r += 64) {
             const int src = s0 + r;              // descending deck number
             const int gi = P.N + g0 + r;
+            // the source's fields are read before any store (the compiler
+            // cannot tell the image slots from the source slots)
+            double gx[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
             for (int d = 0; d < P.ndim; d++) {
-                const double xv = A.x[d][src] + addv[d];
-                A.x[d][gi] = xv;
-                A.xs[d][gi] = xv;
-                A.v[d][gi] = A.v[d][src];
+                gx[d] = A.x[d][src] + addv[d];
+                gv[d] = A.v[d][src];
             }
             const double rho = A.rho[src], mass = A.m[src];
+            const uint32_t ginfo = (A.info[src] & (kMatMask | kRigid | (0xFFFFu << kColorShift))) | kGhost;
+            for (int d = 0; d < P.ndim; d++) {
+                A.x[d][gi] = gx[d];
+                A.xs[d][gi] = gx[d];
+                A.v[d][gi] = gv[d];
+            }
             A.m[gi] = mass;
             A.rho[gi] = rho;
             A.rho_new[gi] = rho;
             A.p[gi] = 0.0;        // images keep p = 0 (not copied, :28)
             A.prho2[gi] = 0.0;
-            const uint32_t ginfo = (A.info[src] & (kMatMask | kRigid | (0xFFFFu << kColorShift))) | kGhost;
             A.info[gi] = ginfo;
-            double gx[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
-            for (int d = 0; d < P.ndim; d++) {
-                gx[d] = A.x[d][gi];
-                gv[d] = A.v[d][gi];
-            }
             switch (P.ndim) {
             case 1: put_srec<1>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<1>(P, A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
             case 2: put_srec<2>(A, gi, gx, mass / rho, gv, ginfo); put_mrec<2>(P, A, gi, gx, mass, gv, rho, 0.0, ginfo); break;
