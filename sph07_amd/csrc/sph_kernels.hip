@@ -865,10 +865,11 @@ __device__ __forceinline__ void neighbours_body(const Params &P, const Arrays &A
 #define SPH_NBR_CELL_WAVES 1
 #endif
 constexpr int kCellWaves = SPH_NBR_CELL_WAVES, kCellBlock = 64 * kCellWaves;
-// candidate chunks held in registers per pass: 4 for the common one-material
-// search, 2 where per-candidate materials or distances take registers too
+// candidate chunks held in registers per pass: 5 for the common one-material
+// search (69 VGPRs; 6 spill), 2 where per-candidate materials or distances
+// take registers too
 #ifndef SPH_NBR_BLK
-#define SPH_NBR_BLK 4
+#define SPH_NBR_BLK 5
 #endif
 template <bool ONEMAT, bool MIND> constexpr int nbr_blk() { return ONEMAT && !MIND ? SPH_NBR_BLK : 2; }
 
