@@ -478,15 +478,18 @@ void fused_step(Context *c, bool first = true, bool last = true)
         timed(c, K_STRAIN, [&] { launch_strain(P, A, M_STRAIN | M_CONST | M_CRIT, st, c->tune.strain_batch); });
     if (P.st) timed(c, K_NORMALS, [&] { launch_normals(P, A, st); });
     timed(c, K_ADVANCE, [&] { launch_advance(P, A, st); });
+    // the FP32c sweep integrates its particles itself where no rigid body
+    // needs the whole sweep's forces first (k_integrate is not launched)
+    const bool fuse_int = c->arm == SPH_ARM_FP32C && !P.rigid;
     timed(c, K_MOMENTUM, [&] {
         if (c->arm == SPH_ARM_FP32C) {
-            launch_momentum32(P, A, st, c->tune.momentum_batch);
+            launch_momentum32(P, A, st, c->tune.momentum_batch, fuse_int ? 1 : 0);
             launch_rigid_reduce(P, A, 1, st);
         } else {
             launch_momentum(P, A, 1, st, c->tune.momentum_batch);
         }
     });
-    timed(c, K_INTEGRATE, [&] { launch_integrate(P, A, M_BC | M_VEL | M_MOVE | M_RHO, st); });
+    if (!fuse_int) timed(c, K_INTEGRATE, [&] { launch_integrate(P, A, M_BC | M_VEL | M_MOVE | M_RHO, st); });
     timed(c, K_FINISH, [&] { launch_finish(P, A, 1, 1, st, last ? 0 : 1); });
 }
 

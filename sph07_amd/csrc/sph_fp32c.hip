@@ -406,8 +406,12 @@ __device__ __forceinline__ void strain32_pair_body(const Params &P, const Arrays
 // (the same sums in the same order: four terms per compensated addition,
 // (t_a + t_b) + (t_c + t_d), with t_a, t_c from lane 2t and t_b, t_d from
 // lane 2t + 1); block: kBlock particles on 2 * kBlock lanes.
+// fuse: the storing lane also integrates its particle (integrate_particle,
+// the full pass of k_integrate without a rigid body) and leaves the moved
+// position in lo/hi for the caller's bounding box.
 template <int D, int G = kMomentum32Batch, int KIND = 0, bool ONEMAT = false, bool PAIRS = false>
-__device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A, const int bid)
+__device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A, const int bid, int fuse,
+                                                double (&lo)[D], double (&hi)[D])
 {
     static_assert(KIND != 2 || ONEMAT, "the one-material record has no material bits");
     const Ctl *ctl = A.ctl;
@@ -618,8 +622,39 @@ __device__ __forceinline__ void momentum32_body(const Params &P, const Arrays &A
         }
     }
     if (PAIRS && par) return;          // both lanes hold the sums; one stores them
+    if (fuse) {
+        // k_integrate rewrites a for the computable particles only
+        if (!(fi & kActive))
+#pragma unroll
+            for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
+        integrate_particle<D>(P, A, ctl, i, fi, I_BC | I_VEL | I_MOVE | I_RHO, ai, lo, hi);
+        return;
+    }
 #pragma unroll
     for (int d = 0; d < D; d++) A.a[d][i] = ai[d];
+}
+
+// the wave's bounding box of moved positions into reduction slot
+// (w % kBboxSlots), one atomic per bound (every lane of the wave takes part)
+template <int D>
+__device__ __forceinline__ void bbox_wave(const Arrays &A, int w, double (&lo)[D], double (&hi)[D])
+{
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        unsigned long long *slot = A.bbox + (size_t)(w % kBboxSlots) * kBboxStride;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            atomicMin(&slot[d], dkey(lo[d]));
+            atomicMax(&slot[3 + d], dkey(hi[d]));
+        }
+    }
 }
 
 template <int D, int G, bool ONEMAT> __global__ __launch_bounds__(strain32_threads<D>()) void k_strain32(Params P, Arrays A)
@@ -638,17 +673,26 @@ __global__ __launch_bounds__(strain32_threads<D>()) void kb_strain32(const Param
     else strain32_body<D, kStrain32Batch, ONEMAT>(Pt[e.x], At[e.x], e.y);
 }
 template <int D, int G, int KIND, bool ONEMAT>
-__global__ __launch_bounds__(momentum32_threads<D>()) void k_momentum32(Params P, Arrays A)
+__global__ __launch_bounds__(momentum32_threads<D>()) void k_momentum32(Params P, Arrays A, int fuse)
 {
     const int nb = (P.N + kBlock - 1) / kBlock, b = xcd_block(nb);
-    if (b < nb) momentum32_body<D, G, KIND, ONEMAT, momentum32_pairs<D>()>(P, A, b);
+    if (b >= nb) return;
+    double lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        lo[d] = 1.0e20;
+        hi[d] = -1.0e20;
+    }
+    momentum32_body<D, G, KIND, ONEMAT, momentum32_pairs<D>()>(P, A, b, fuse, lo, hi);
+    if (fuse) bbox_wave<D>(A, (int)(blockIdx.x * (momentum32_threads<D>() / 64) + (threadIdx.x >> 6)), lo, hi);
 }
 template <int D, int KIND, bool ONEMAT>
 __global__ __launch_bounds__(momentum32_threads<D>()) void kb_momentum32(const Params *__restrict__ Pt, const Arrays *__restrict__ At,
                                                                       const int2 *map)
 {
     const int2 e = map[blockIdx.x];
-    momentum32_body<D, kMomentum32Batch, KIND, ONEMAT, momentum32_pairs<D>()>(Pt[e.x], At[e.x], e.y);
+    double lo[D], hi[D];
+    momentum32_body<D, kMomentum32Batch, KIND, ONEMAT, momentum32_pairs<D>()>(Pt[e.x], At[e.x], e.y, 0, lo, hi);
 }
 
 template <int D>
@@ -721,28 +765,28 @@ static void strain32_g(const Params &P, const Arrays &A, hipStream_t st)
 }
 
 template <int G, int KIND, bool ONEMAT>
-static void momentum32_gc(const Params &P, const Arrays &A, hipStream_t st)
+static void momentum32_gc(const Params &P, const Arrays &A, hipStream_t st, int fuse)
 {
     const dim3 grid(xcd_grid(nblk32(P.N)));
     switch (P.ndim) {
-    case 1: hipLaunchKernelGGL((k_momentum32<1, G, KIND, ONEMAT>), grid, dim3(momentum32_threads<1>()), 0, st, P, A); break;
-    case 2: hipLaunchKernelGGL((k_momentum32<2, G, KIND, ONEMAT>), grid, dim3(momentum32_threads<2>()), 0, st, P, A); break;
-    default: hipLaunchKernelGGL((k_momentum32<3, G, KIND, ONEMAT>), grid, dim3(momentum32_threads<3>()), 0, st, P, A); break;
+    case 1: hipLaunchKernelGGL((k_momentum32<1, G, KIND, ONEMAT>), grid, dim3(momentum32_threads<1>()), 0, st, P, A, fuse); break;
+    case 2: hipLaunchKernelGGL((k_momentum32<2, G, KIND, ONEMAT>), grid, dim3(momentum32_threads<2>()), 0, st, P, A, fuse); break;
+    default: hipLaunchKernelGGL((k_momentum32<3, G, KIND, ONEMAT>), grid, dim3(momentum32_threads<3>()), 0, st, P, A, fuse); break;
     }
 }
 
 template <int G>
-static void momentum32_g(const Params &P, const Arrays &A, hipStream_t st)
+static void momentum32_g(const Params &P, const Arrays &A, hipStream_t st, int fuse)
 {
     switch (P.mcompact) {
-    case 2: momentum32_gc<G, 2, true>(P, A, st); break;
+    case 2: momentum32_gc<G, 2, true>(P, A, st, fuse); break;
     case 1:
-        if (P.nmat == 1) momentum32_gc<G, 1, true>(P, A, st);
-        else momentum32_gc<G, 1, false>(P, A, st);
+        if (P.nmat == 1) momentum32_gc<G, 1, true>(P, A, st, fuse);
+        else momentum32_gc<G, 1, false>(P, A, st, fuse);
         break;
     default:
-        if (P.nmat == 1) momentum32_gc<G, 0, true>(P, A, st);
-        else momentum32_gc<G, 0, false>(P, A, st);
+        if (P.nmat == 1) momentum32_gc<G, 0, true>(P, A, st, fuse);
+        else momentum32_gc<G, 0, false>(P, A, st, fuse);
     }
 }
 
@@ -755,14 +799,14 @@ void launch_strain32(const Params &P, const Arrays &A, hipStream_t st, int batch
     }
 }
 
-void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st, int batch)
+void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st, int batch, int fuse)
 {
     // two lanes per particle pass over a whole chunk at least: batch 2 is batch 4
     if (P.ndim == 3 && momentum32_pairs<3>() && batch == 2) batch = 4;
     switch (batch) {
-    case 2: momentum32_g<2>(P, A, st); break;
-    case 8: momentum32_g<8>(P, A, st); break;
-    default: momentum32_g<4>(P, A, st); break;
+    case 2: momentum32_g<2>(P, A, st, fuse); break;
+    case 8: momentum32_g<8>(P, A, st, fuse); break;
+    default: momentum32_g<4>(P, A, st, fuse); break;
     }
 }
 

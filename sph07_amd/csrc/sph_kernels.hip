@@ -1725,24 +1725,9 @@ __device__ __forceinline__ void rigid_kin_body(Ctl *ctl)
 // ---------------------------------------------------------------------------
 // Per-particle integration: MSolution::BoundCond (:700-726), UpdateVelocity
 // (:602-637) + the particle half of RigidBodyVelocity (:1578-1588),
-// MoveParticles (:769-809) with the periodic wrap and the bounding box.
-// dispbc 8 (extension) keeps its position.
+// MoveParticles (:769-809): integrate_particle per particle, and the block's
+// bounding box of the moved positions folded into one reduction slot.
 // ---------------------------------------------------------------------------
-enum { I_BC = 1, I_VEL = 2, I_MOVE = 4, I_RHO = 8 };
-
-__device__ __forceinline__ void constrain(double *q, int code, int D)
-{
-    switch (code) {
-    case 1: q[0] = 0.0; break;
-    case 2: if (D > 1) q[1] = 0.0; break;
-    case 3: if (D > 2) q[2] = 0.0; break;
-    case 4: q[0] = 0.0; if (D > 1) q[1] = 0.0; break;
-    case 5: if (D > 1) q[1] = 0.0; if (D > 2) q[2] = 0.0; break;
-    case 6: if (D > 2) q[2] = 0.0; q[0] = 0.0; break;
-    case 7: case 8: for (int d = 0; d < D; d++) q[d] = 0.0; break;
-    }
-}
-
 template <int D>
 __device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A, int mask, const int bid)
 {
@@ -1756,75 +1741,10 @@ __device__ __forceinline__ void integrate_body(const Params &P, const Arrays &A,
         hi[d] = -1.0e20;
     }
     if (i < P.N) {
-        const uint32_t fi = A.info[i];
-        const bool act = fi & kActive, rig = fi & kRigid;
-        const int disp = info_disp(fi);
         double a[D];
 #pragma unroll
         for (int d = 0; d < D; d++) a[d] = (mask & (I_BC | I_VEL)) ? A.a[d][i] : 0.0;
-        if ((mask & I_BC) && act) {
-            if (P.baseaccel && !rig)
-#pragma unroll
-                for (int d = 0; d < D; d++) a[d] += P.base_a[d];
-            constrain(a, disp, D);
-#pragma unroll
-            for (int d = 0; d < D; d++) A.a[d][i] = a[d];
-        }
-        if (mask & I_VEL) {
-            // the next critical step's |a|^2 (CalcCriticalTimestep's sum, :41-44)
-            double s = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; d++) s += a[d] * a[d];
-            A.aa[i] = s;
-        }
-        double x[D], v[D];
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            x[d] = A.x[d][i];
-            v[d] = A.v[d][i];
-        }
-        if ((mask & I_VEL) && act) {
-            if (!rig) {
-                const double dtn = 0.5 * (ctl->dt + ctl->dtold);
-#pragma unroll
-                for (int d = 0; d < D; d++) v[d] = v[d] + dtn * a[d];
-            } else {
-                double rl[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-                for (int d = 0; d < D; d++) rl[d] = x[d] - ctl->rbR[d];
-                const double *V = ctl->rbV, *W = ctl->rbW;
-                double vr[3];
-                vr[0] = V[0] + W[1] * rl[2] - W[2] * rl[1];
-                vr[1] = V[1] + W[2] * rl[0] - W[0] * rl[2];
-                vr[2] = V[2] + W[0] * rl[1] - W[1] * rl[0];
-#pragma unroll
-                for (int d = 0; d < D; d++) v[d] = vr[d];
-            }
-#pragma unroll
-            for (int d = 0; d < D; d++) A.v[d][i] = v[d];
-        }
-        if (mask & I_MOVE) {
-            if (act && disp != 8) {
-                const double dt = ctl->dt;
-#pragma unroll
-                for (int d = 0; d < D; d++) x[d] += v[d] * dt;
-            }
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-                if (P.bcode[1][d] == 2 && x[d] >= P.bx[1][d]) {
-                    const double over = x[d] - P.bx[1][d];
-                    x[d] = P.bx[0][d] + over;
-                }
-                if (P.bcode[0][d] == 2 && x[d] <= P.bx[0][d]) {
-                    const double over = P.bx[0][d] - x[d];
-                    x[d] = P.bx[0][d] - over;
-                }
-                A.x[d][i] = x[d];
-                lo[d] = x[d];
-                hi[d] = x[d];
-            }
-        }
-        if (mask & I_RHO) A.rho[i] = A.rho_new[i];
+        integrate_particle<D>(P, A, ctl, i, A.info[i], mask, a, lo, hi);
     }
     if (mask & I_MOVE) {
         __shared__ double red[2][D][kBlock / 64];

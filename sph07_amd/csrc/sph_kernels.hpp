@@ -246,8 +246,107 @@ void launch_gather_ctl(const Arrays *At, const int *sc, int ns, Ctl *out, hipStr
 void launch_batch_strain32(const BatchView &B, hipStream_t st);
 void launch_batch_momentum32(const BatchView &B, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// Per-particle integration: MSolution::BoundCond (:700-726), UpdateVelocity
+// (:602-637) + the particle half of RigidBodyVelocity (:1578-1588),
+// MoveParticles (:769-809) with the periodic wrap; lo/hi receive the moved
+// position (the caller folds the bounding box).  dispbc 8 (extension) keeps
+// its position.  Inline in both sources that call it (k_integrate and the
+// FP32c momentum sweep's fused form) with contraction off, so that both
+// compute the same bits.
+// ---------------------------------------------------------------------------
+enum { I_BC = 1, I_VEL = 2, I_MOVE = 4, I_RHO = 8 };
+
+__device__ __forceinline__ void constrain(double *q, int code, int D)
+{
+    switch (code) {
+    case 1: q[0] = 0.0; break;
+    case 2: if (D > 1) q[1] = 0.0; break;
+    case 3: if (D > 2) q[2] = 0.0; break;
+    case 4: q[0] = 0.0; if (D > 1) q[1] = 0.0; break;
+    case 5: if (D > 1) q[1] = 0.0; if (D > 2) q[2] = 0.0; break;
+    case 6: if (D > 2) q[2] = 0.0; q[0] = 0.0; break;
+    case 7: case 8: for (int d = 0; d < D; d++) q[d] = 0.0; break;
+    }
+}
+
+// a: the particle's acceleration as the momentum sweep left it
+template <int D>
+__device__ __forceinline__ void integrate_particle(const Params &P, const Arrays &A, const Ctl *ctl, int i, uint32_t fi,
+                                                   int mask, double (&a)[D], double (&lo)[D], double (&hi)[D])
+{
+#pragma clang fp contract(off)
+    const bool act = fi & kActive, rig = fi & kRigid;
+    const int disp = info_disp(fi);
+    if ((mask & I_BC) && act) {
+        if (P.baseaccel && !rig)
+#pragma unroll
+            for (int d = 0; d < D; d++) a[d] += P.base_a[d];
+        constrain(a, disp, D);
+#pragma unroll
+        for (int d = 0; d < D; d++) A.a[d][i] = a[d];
+    }
+    if (mask & I_VEL) {
+        // the next critical step's |a|^2 (CalcCriticalTimestep's sum, :41-44)
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) s += a[d] * a[d];
+        A.aa[i] = s;
+    }
+    double x[D], v[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        x[d] = A.x[d][i];
+        v[d] = A.v[d][i];
+    }
+    if ((mask & I_VEL) && act) {
+        if (!rig) {
+            const double dtn = 0.5 * (ctl->dt + ctl->dtold);
+#pragma unroll
+            for (int d = 0; d < D; d++) v[d] = v[d] + dtn * a[d];
+        } else {
+            double rl[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int d = 0; d < D; d++) rl[d] = x[d] - ctl->rbR[d];
+            const double *V = ctl->rbV, *W = ctl->rbW;
+            double vr[3];
+            vr[0] = V[0] + W[1] * rl[2] - W[2] * rl[1];
+            vr[1] = V[1] + W[2] * rl[0] - W[0] * rl[2];
+            vr[2] = V[2] + W[0] * rl[1] - W[1] * rl[0];
+#pragma unroll
+            for (int d = 0; d < D; d++) v[d] = vr[d];
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++) A.v[d][i] = v[d];
+    }
+    if (mask & I_MOVE) {
+        if (act && disp != 8) {
+            const double dt = ctl->dt;
+#pragma unroll
+            for (int d = 0; d < D; d++) x[d] += v[d] * dt;
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            if (P.bcode[1][d] == 2 && x[d] >= P.bx[1][d]) {
+                const double over = x[d] - P.bx[1][d];
+                x[d] = P.bx[0][d] + over;
+            }
+            if (P.bcode[0][d] == 2 && x[d] <= P.bx[0][d]) {
+                const double over = P.bx[0][d] - x[d];
+                x[d] = P.bx[0][d] - over;
+            }
+            A.x[d][i] = x[d];
+            lo[d] = x[d];
+            hi[d] = x[d];
+        }
+    }
+    if (mask & I_RHO) A.rho[i] = A.rho_new[i];
+}
+
 // FP32-compensated pair sweeps (sph_fp32c.hip); fused-step forms only
 void launch_strain32(const Params &P, const Arrays &A, hipStream_t st, int batch = 8);
-void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st, int batch = 4);
+// fuse != 0: each particle is also integrated (k_integrate's full pass, no
+// rigid body) and the moved bounding box folded; k_integrate is not launched
+void launch_momentum32(const Params &P, const Arrays &A, hipStream_t st, int batch = 4, int fuse = 0);
 
 }  // namespace sph
